@@ -1,0 +1,199 @@
+/*
+ * sbpgpu.h — C ABI of the MI355X (gfx950) Euler RK4 time-step library.
+ *
+ * Plain pointers and sizes only. Each entry point names the reference
+ * interface it replaces (paths relative to the SummationByParts repository).
+ * All arrays are C-order. A field/state is (4, ny, nx): rho, rhou, rhov, e.
+ * Metrics are (5, ny, nx): J, J*xi_x, J*xi_y, J*eta_x, J*eta_y
+ * (sbp/src/grid.rs:15-21). "Device" pointers are HIP device memory of the
+ * context's device; "host" pointers are ordinary CPU memory.
+ *
+ * Every function returns an sbp_status. Errors map one-to-one to the panics
+ * of the reference path; the reference's Rust side would re-panic on them.
+ */
+#ifndef SBPGPU_H
+#define SBPGPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+  SBP_OK = 0,
+  SBP_ERR_INVALID = 1,     /* bad handle, enum value or pointer */
+  SBP_ERR_SHAPE = 2,       /* n < 2M or n < N (sbp/src/operators/algos.rs:118-121) */
+  SBP_ERR_DETJ = 3,        /* det J <= 0 (sbp/src/grid.rs:117) */
+  SBP_ERR_RHO = 4,         /* rho <= 0 (euler/src/lib.rs:633, euler/src/vortex.rs:74,81,107) */
+  SBP_ERR_PRESSURE = 5,    /* p <= 0 (euler/src/lib.rs:642, euler/src/vortex.rs:79) */
+  SBP_ERR_GAMMA = 6,       /* GAMMA not set (euler/src/lib.rs:626,686,1166) */
+  SBP_ERR_EDGE = 7,        /* edge lengths differ (euler/src/lib.rs:1161-1165) */
+  SBP_ERR_HIP = 8,         /* HIP runtime failure */
+  SBP_ERR_NOT_UPWIND = 9,  /* "This is not an upwind operator" (euler/src/lib.rs:597) */
+  SBP_ERR_OOM = 10,        /* device allocation failed */
+  SBP_ERR_GAMMA_SET = 11   /* GAMMA already set to another value (OnceCell::set, euler/src/lib.rs:15) */
+} sbp_status;
+
+/* sbp/src/operators.rs:152-165 */
+typedef enum {
+  SBP_UPWIND4 = 0,
+  SBP_UPWIND4H2 = 1,
+  SBP_UPWIND9 = 2,
+  SBP_UPWIND9H2 = 3,
+  SBP_SBP4 = 4,
+  SBP_SBP8 = 5
+} sbp_operator;
+
+typedef enum { SBP_DIFF = 0, SBP_DISS = 1 } sbp_kind;
+
+/* sbp/src/utils.rs:15-129 (Direction) */
+typedef enum { SBP_NORTH = 0, SBP_SOUTH = 1, SBP_EAST = 2, SBP_WEST = 3 } sbp_direction;
+
+/* euler/src/lib.rs:737-745 (BoundaryCharacteristic: This, Grid(j), Vortex) */
+typedef enum { SBP_BC_THIS = 0, SBP_BC_GRID = 1, SBP_BC_VORTEX = 2 } sbp_bc_kind;
+
+/* RHS selection: AUTO follows multigrid/src/system.rs:521-525 (upwind iff both
+ * operators are upwind); TRAD = euler::System::advance / RHS_trad;
+ * UPWIND = euler::System::advance_upwind / RHS_upwind. */
+typedef enum { SBP_RHS_AUTO = 0, SBP_RHS_TRAD = 1, SBP_RHS_UPWIND = 2 } sbp_rhs_kind;
+
+/* FP64: the drop-in. MIXED: FP32 flux/stencil/dissipation arithmetic with
+ * FP64 state, stage sums and metrics. */
+typedef enum { SBP_FP64 = 0, SBP_MIXED = 1 } sbp_precision;
+
+/* euler/src/vortex.rs:3-21 (Vortice, VortexParameters; at most 5 vortices) */
+typedef struct {
+  double x0, y0, rstar, eps;
+} sbp_vortice;
+
+typedef struct {
+  int32_t nvortices;
+  int32_t pad_;
+  sbp_vortice vortices[5];
+  double mach;
+} sbp_vortex_params;
+
+typedef struct sbp_context_s* sbp_context;
+typedef struct sbp_system_s* sbp_system;
+
+/* Human-readable text of a status code. */
+const char* sbp_status_string(int status);
+
+/* euler::GAMMA (euler/src/lib.rs:15; set once, multigrid/src/parsing.rs:48).
+ * Setting the value it already holds is OK; another value is SBP_ERR_GAMMA_SET. */
+int sbp_set_gamma(double gamma);
+int sbp_get_gamma(double* gamma);
+
+/* ---- Operator traits (sbp/src/operators.rs:11-106) ---------------------- */
+
+/* Block sizes: M closure rows, N closure columns, D interior width, whether
+ * the operator is h2 (SbpOperator1d::is_h2) and upwind (::upwind().is_some()). */
+int sbp_operator_info(int op, int* M, int* N, int* D, int* is_h2, int* is_upwind);
+/* SbpOperator1d::h (M values). */
+int sbp_operator_h(int op, double* h_out);
+/* The coefficient blocks as the kernels use them: start (M*N), diag (D), end (M*N). */
+int sbp_operator_coefficients(int op, int kind, double* start, double* diag, double* end);
+
+/* ---- Context ------------------------------------------------------------ */
+
+/* One context per device; owns a HIP stream (or uses the caller's). */
+int sbp_context_create(int device, sbp_context* out);
+int sbp_context_destroy(sbp_context ctx);
+/* Use `hip_stream` (a hipStream_t) for all work of this context; NULL = own stream. */
+int sbp_context_set_stream(sbp_context ctx, void* hip_stream);
+int sbp_context_synchronize(sbp_context ctx);
+
+/* ---- Operator application on device buffers ----------------------------
+ * prev, fut: (ny, nx) device arrays with row pitch ld_prev / ld_fut elements.
+ * fut is fully overwritten (sbp/src/operators/algos.rs:228,306). */
+
+/* SbpOperator2d::diffxi (sbp/src/operators.rs:56-61; algos.rs:101-187) */
+int sbp_diffxi(sbp_context ctx, int op, const double* prev, int64_t ld_prev, double* fut, int64_t ld_fut, int ny, int nx);
+/* SbpOperator2d::diffeta (sbp/src/operators.rs:62-64; algos.rs:371-519) */
+int sbp_diffeta(sbp_context ctx, int op, const double* prev, int64_t ld_prev, double* fut, int64_t ld_fut, int ny, int nx);
+/* UpwindOperator2d::dissxi (sbp/src/operators.rs:91-96) */
+int sbp_dissxi(sbp_context ctx, int op, const double* prev, int64_t ld_prev, double* fut, int64_t ld_fut, int ny, int nx);
+/* UpwindOperator2d::disseta (sbp/src/operators.rs:98-100) */
+int sbp_disseta(sbp_context ctx, int op, const double* prev, int64_t ld_prev, double* fut, int64_t ld_fut, int ny, int nx);
+
+/* ---- Block-level right-hand side on device buffers ----------------------
+ * euler::RHS_trad / RHS_upwind (euler/src/lib.rs:483-515, :571-616) for one
+ * block with operator pair (op_eta, op_xi) (the reference's tuple order,
+ * multigrid/src/parsing.rs:164). k (out), y (in): (4, ny, nx) contiguous;
+ * metrics: (5, ny, nx) contiguous; boundaries[dir]: 4 x n contiguous arrays
+ * (n = nx for north/south, ny for east/west) holding the boundary data g of
+ * BoundaryTerms (euler/src/lib.rs:760). kind: SBP_RHS_TRAD or SBP_RHS_UPWIND. */
+int sbp_rhs(sbp_context ctx, int kind, int op_eta, int op_xi, int ny, int nx, double* k, const double* y,
+            const double* metrics, const double* const* boundaries);
+
+/* Grid::metrics (sbp/src/grid.rs:64-66,95-139) on device buffers: x, y (ny, nx)
+ * contiguous in, metrics (5, ny, nx) out. */
+int sbp_metrics(sbp_context ctx, int op_eta, int op_xi, int ny, int nx, const double* x, const double* y, double* metrics);
+
+/* ---- Systems: one or many blocks stepped by fused RK4 -------------------
+ * Replaces euler::System (euler/src/lib.rs:20-186) for one block with periodic
+ * BCs, and multigrid's SingleThreadedSystem / DistributedSystem
+ * (multigrid/src/system.rs:456-539, :638-1172) for many. */
+
+/* Blocks with sizes (ny[i], nx[i]), operator pairs (op_eta[i], op_xi[i]) and
+ * host coordinate arrays x[i], y[i] of shape (ny[i], nx[i]). Metrics are
+ * computed on the device (BaseSystem::create, multigrid/src/system.rs:73-78).
+ * Boundary conditions default to SBP_BC_THIS on every edge. */
+int sbp_system_create(sbp_context ctx, int nblocks, const int* ny, const int* nx, const int* op_eta, const int* op_xi,
+                      const double* const* x, const double* const* y, sbp_system* out);
+int sbp_system_destroy(sbp_system sys);
+/* BoundaryCharacteristic of one edge: SBP_BC_THIS, SBP_BC_GRID (neighbour),
+ * SBP_BC_VORTEX (vortex required). */
+int sbp_system_set_bc(sbp_system sys, int block, int dir, int kind, int neighbour, const sbp_vortex_params* vortex);
+int sbp_system_set_rhs_kind(sbp_system sys, int kind);
+int sbp_system_set_precision(sbp_system sys, int precision);
+/* State of one block, (4, ny, nx) host arrays. */
+int sbp_system_set_state(sbp_system sys, int block, const double* host);
+int sbp_system_get_state(sbp_system sys, int block, double* host);
+/* Device pointer and row pitch of a block's current state (4 planes, plane
+ * stride returned in *plane_stride); valid until the next call on `sys`. */
+int sbp_system_state_device(sbp_system sys, int block, double** dev, int64_t* ld, int64_t* plane_stride);
+/* Metrics of one block, (5, ny, nx) host array. */
+int sbp_system_get_metrics(sbp_system sys, int block, double* host);
+/* Field::vortex on every block (euler/src/lib.rs:319-328, vortex.rs:23-113). */
+int sbp_system_vortex(sbp_system sys, double t, const sbp_vortex_params* params);
+/* One RHS evaluation of the current state at time t (the closure passed to
+ * integrate, multigrid/src/system.rs:508-527), k copied to k_host:
+ * concatenation of each block's (4, ny, nx). */
+int sbp_system_rhs(sbp_system sys, double t, double* k_host);
+/* nsteps classical RK4 steps of size dt (integrate::<Rk4>,
+ * utils/integrate/src/lib.rs:176-222), fused and graph-captured. */
+int sbp_system_advance(sbp_system sys, double dt, int64_t nsteps);
+int sbp_system_get_time(sbp_system sys, double* t);
+int sbp_system_set_time(sbp_system sys, double t);
+/* System::max_dt (multigrid/src/system.rs:542-589). */
+int sbp_system_max_dt(sbp_system sys, double* out);
+/* System::error against the vortex at the system time: the sum over blocks of
+ * Field::h2_err (multigrid/src/system.rs:618-635; euler/src/lib.rs:377-432),
+ * compensated FP64 sums on the device. */
+int sbp_system_error_vortex(sbp_system sys, const sbp_vortex_params* params, double* out);
+/* ||q||_H^2 summed over blocks and components (the H-norm energy used for
+ * drift reports), compensated FP64 sums on the device. */
+int sbp_system_energy(sbp_system sys, double* out);
+/* Sticky device error word of the last advance (0 = none); cleared by reading. */
+int sbp_system_check(sbp_system sys);
+
+/* ---- Measurement ---------------------------------------------------------
+ * Times nsteps RK4 steps (state is advanced) with HIP events on the context
+ * stream: *ms_total for the whole region, and per kernel family the average
+ * duration of one stage-kernel launch, measured by bracketing each launch of
+ * a separate timing pass with events. */
+int sbp_system_profile(sbp_system sys, double dt, int64_t nsteps, double* ms_total, double* ms_stage_avg);
+/* Tile shape currently used for block 0 (after autotuning). */
+int sbp_system_tile_shape(sbp_system sys, int* tx, int* ty, int* ry);
+/* Force a tile shape (index into the library's shape list) or -1 = autotune. */
+int sbp_system_set_tile_shape(sbp_system sys, int shape_index);
+int sbp_tile_shape_count(void);
+int sbp_tile_shape_get(int index, int* tx, int* ty, int* ry);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SBPGPU_H */

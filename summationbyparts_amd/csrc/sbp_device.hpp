@@ -1,0 +1,69 @@
+// Device-side descriptors shared by the HIP kernels and the host runtime.
+#pragma once
+#include <stdint.h>
+
+#include "../../include/sbpgpu.h"
+
+namespace sbp {
+
+// Roles of the four state buffers of a block during one fused RK4 step.
+enum BufRole : int { BUF_YN = 0, BUF_ACC = 1, BUF_YA = 2, BUF_YB = 3 };
+
+// Boundary-data source of one block edge inside a kernel.
+enum EdgeSrc : int {
+  EDGE_THIS = SBP_BC_THIS,      // own opposite edge of the stage input (periodic)
+  EDGE_GRID = SBP_BC_GRID,      // neighbour block's opposite edge of the stage input
+  EDGE_VORTEX = SBP_BC_VORTEX,  // analytic vortex on this edge's coordinates at stage time
+  EDGE_EXPLICIT = 100,          // caller-provided 4 x n array (operator-level rhs)
+};
+
+// Stage-kernel mode: RHS only, or one of the four fused RK4 stages.
+enum StageMode : int { MODE_RHS = -1, MODE_S0 = 0, MODE_S1 = 1, MODE_S2 = 2, MODE_S3 = 3 };
+
+struct VortexDev {
+  int32_t n;
+  int32_t pad_;
+  double x0[5], y0[5], rstar[5], eps[5];
+  double mach;
+};
+
+// One block (grid patch) as the kernels see it. All planes share a row pitch
+// `ld` (elements) and plane stride `ps`; state buffers hold 4 planes
+// (rho, rhou, rhov, e), metrics 5 planes (J, J·ξx, J·ξy, J·ηx, J·ηy).
+struct BlockDev {
+  int nx, ny, ld, op_xi, op_eta, upwind;
+  int ntx, nty;  // tiles along ξ and η for the active tile shape
+  int64_t ps;
+  const double* metrics;
+  const double* xc;  // ξ/η grid coordinates x(η, ξ), y(η, ξ)
+  const double* yc;
+  double* buf[4];  // BufRole
+  double* rhs_in;  // MODE_RHS input state (4 planes)
+  double* rhs_out;  // MODE_RHS output k (4 planes)
+  double idx_xi, idx_eta;  // 1/dξ and 1/dη (algos.rs:125-130)
+  double hi_xi, hi_eta;    // SAT factors (n-1)/H0 (lib.rs:1034-1038,1095-1099)
+  int edge_src[4];         // EdgeSrc per sbp_direction
+  int edge_nb[4];          // neighbour block for EDGE_GRID
+  const double* edge_ptr[4];  // EDGE_EXPLICIT data
+  int64_t edge_cs[4];         // component stride of EDGE_EXPLICIT data
+  VortexDev vortex[4];        // EDGE_VORTEX parameters
+};
+
+struct TileDev {
+  int32_t block;
+  int16_t tx, ty;
+};
+
+struct StageArgs {
+  const BlockDev* blocks;
+  const TileDev* tiles;
+  const double* time;  // device-resident simulation time (integrate's `time`)
+  unsigned int* error;  // sticky error word (bit per sbp_status)
+  double dt;
+  double gamma;
+  double rk_scale_acc;  // b_s·dt
+  double rk_scale_next;  // a_s·dt (stage input of the next stage)
+  double time_offset;  // c_s·dt added to *time for boundary data
+};
+
+}  // namespace sbp

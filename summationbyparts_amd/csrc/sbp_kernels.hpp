@@ -1,0 +1,51 @@
+// Host-side declarations of the kernel launchers (implemented in the .hip files).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "sbp_device.hpp"
+
+namespace sbp {
+
+// Tile shapes of the fused stage kernel: TX points along ξ, TY along η, each
+// thread owning RY consecutive η-points of one ξ-column.
+struct TileShape {
+  int tx, ty, ry;
+};
+constexpr int kNumTileShapes = 3;
+extern const TileShape kTileShapes[kNumTileShapes];
+
+// Operator application along one axis (0 = ξ, 1 = η) of a (ny, nx) array.
+hipError_t launch_apply_op(int op, int kind, int axis, const double* prev, int64_t ldp, double* fut, int64_t ldf, int ny,
+                           int nx, hipStream_t stream);
+
+// Fused stage kernel over `ntiles` tiles of blocks sharing one operator pair.
+// mode: StageMode. Returns hipErrorInvalidValue for an unsupported combination.
+hipError_t launch_stage(int op_eta, int op_xi, bool upwind, int shape, int mode, const StageArgs& args, int ntiles,
+                        hipStream_t stream);
+// Launch bounds (threads per workgroup) and LDS bytes for a tile shape.
+int stage_threads(int shape);
+
+// Pointwise metrics from the four derivative planes (grid.rs:110-131).
+hipError_t launch_metrics_finish(const double* dx_dxi, const double* dx_deta, const double* dy_dxi, const double* dy_deta,
+                                 double* metrics, int64_t ps, int64_t ld, int ny, int nx, unsigned* err, hipStream_t stream);
+// Vortex initial state on (ny, nx) coordinates into 4 planes.
+hipError_t launch_vortex(const VortexDev& vp, double gamma, double t, const double* xc, const double* yc, double* out,
+                         int64_t ps, int64_t ld, int ny, int nx, unsigned* err, hipStream_t stream);
+// Per-workgroup compensated partial sums of Σ_c Σ_ij w_ij (a-b)^2 with
+// w = hx_i hy_j; b = vortex(t) if vp != nullptr, b = 0 if b == nullptr.
+// partials: 2 doubles (sum, compensation) per workgroup; returns #workgroups.
+int reduce_h2_workgroups(int ny, int nx);
+hipError_t launch_h2_reduce(int op_eta, int op_xi, const double* a, const double* b, const VortexDev* vp, double gamma,
+                            double t, const double* xc, const double* yc, int64_t ps, int64_t ld, int ny, int nx,
+                            double* partials, hipStream_t stream);
+// Per-workgroup max of |û| and |v̂| (system.rs:561-578); returns #workgroups.
+int reduce_max_workgroups(int ny, int nx);
+hipError_t launch_maxspeed(const double* q, const double* metrics, int64_t ps, int64_t ld, int ny, int nx, double* partials,
+                           hipStream_t stream);
+// time += dt on the device.
+hipError_t launch_advance_time(double* time, double dt, hipStream_t stream);
+// Copy between pitched planes (4 or 5 planes) on the device.
+hipError_t launch_copy_planes(const double* src, int64_t sps, int64_t sld, double* dst, int64_t dps, int64_t dld, int nplanes,
+                              int ny, int nx, hipStream_t stream);
+
+}  // namespace sbp

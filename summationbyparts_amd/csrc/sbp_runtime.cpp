@@ -1,0 +1,1183 @@
+// Host runtime of libsbpgpu: contexts, systems (blocks, boundary conditions,
+// device buffers), fused RK4 stepping captured in HIP graphs, the tile-shape
+// autotuner and the C ABI declared in include/sbpgpu.h.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <vector>
+
+#include "../../include/sbpgpu.h"
+#include "sbp_device.hpp"
+#include "sbp_kernels.hpp"
+#include "sbp_operators.hpp"
+
+namespace sbp {
+hipError_t launch_stage_opxi_0(int, bool, int, int, const StageArgs&, int, hipStream_t);
+hipError_t launch_stage_opxi_1(int, bool, int, int, const StageArgs&, int, hipStream_t);
+hipError_t launch_stage_opxi_2(int, bool, int, int, const StageArgs&, int, hipStream_t);
+hipError_t launch_stage_opxi_3(int, bool, int, int, const StageArgs&, int, hipStream_t);
+hipError_t launch_stage_opxi_4(int, bool, int, int, const StageArgs&, int, hipStream_t);
+hipError_t launch_stage_opxi_5(int, bool, int, int, const StageArgs&, int, hipStream_t);
+
+hipError_t launch_stage(int op_eta, int op_xi, bool upwind, int shape, int mode, const StageArgs& args, int ntiles,
+                        hipStream_t stream) {
+  if (ntiles <= 0) return hipSuccess;
+  switch (op_xi) {
+    case 0: return launch_stage_opxi_0(op_eta, upwind, shape, mode, args, ntiles, stream);
+    case 1: return launch_stage_opxi_1(op_eta, upwind, shape, mode, args, ntiles, stream);
+    case 2: return launch_stage_opxi_2(op_eta, upwind, shape, mode, args, ntiles, stream);
+    case 3: return launch_stage_opxi_3(op_eta, upwind, shape, mode, args, ntiles, stream);
+    case 4: return launch_stage_opxi_4(op_eta, upwind, shape, mode, args, ntiles, stream);
+    case 5: return launch_stage_opxi_5(op_eta, upwind, shape, mode, args, ntiles, stream);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+int stage_threads(int shape) { return kTileShapes[shape].tx * (kTileShapes[shape].ty / kTileShapes[shape].ry); }
+}  // namespace sbp
+
+using namespace sbp;
+
+namespace {
+
+struct OpInfo {
+  int M, N, D, h2, upwind;
+  const double* h;
+};
+
+template <class OP>
+OpInfo info_of() {
+  return {OP::M, OP::N, OP::D, OP::is_h2, OP::is_upwind, OP::h};
+}
+
+bool op_info(int op, OpInfo& out) {
+  switch (op) {
+    case UPWIND4: out = info_of<ops::Upwind4>(); return true;
+    case UPWIND4H2: out = info_of<ops::Upwind4h2>(); return true;
+    case UPWIND9: out = info_of<ops::Upwind9>(); return true;
+    case UPWIND9H2: out = info_of<ops::Upwind9h2>(); return true;
+    case SBP4: out = info_of<ops::SBP4>(); return true;
+    case SBP8: out = info_of<ops::SBP8>(); return true;
+    default: return false;
+  }
+}
+
+template <class OP>
+void coefs_of(int kind, double* start, double* diag, double* end, bool& ok) {
+  ok = true;
+  const int M = OP::M, N = OP::N, D = OP::D;
+  if (kind == SBP_DIFF) {
+    for (int i = 0; i < M; ++i)
+      for (int k = 0; k < N; ++k) {
+        start[i * N + k] = OP::diff_start[i][k];
+        end[i * N + k] = -OP::diff_start[M - 1 - i][N - 1 - k];
+      }
+    for (int d = 0; d < D; ++d) diag[d] = OP::diff_diag[d];
+  } else {
+    if constexpr (OP::is_upwind) {
+      for (int i = 0; i < M; ++i)
+        for (int k = 0; k < N; ++k) {
+          start[i * N + k] = OP::diss_start[i][k];
+          end[i * N + k] = OP::diss_start[M - 1 - i][N - 1 - k];
+        }
+      for (int d = 0; d < D; ++d) diag[d] = OP::diss_diag[d];
+    } else {
+      ok = false;
+    }
+  }
+}
+
+std::mutex g_gamma_mu;
+double g_gamma = 0.0;
+bool g_gamma_set = false;
+
+bool gamma_get(double& g) {
+  std::lock_guard<std::mutex> lk(g_gamma_mu);
+  g = g_gamma;
+  return g_gamma_set;
+}
+
+int from_hip(hipError_t e) {
+  if (e == hipSuccess) return SBP_OK;
+  if (e == hipErrorOutOfMemory) return SBP_ERR_OOM;
+  return SBP_ERR_HIP;
+}
+
+#define SBP_TRY_HIP(expr)                   \
+  do {                                      \
+    hipError_t e_ = (expr);                 \
+    if (e_ != hipSuccess) return from_hip(e_); \
+  } while (0)
+
+#define SBP_TRY(expr)            \
+  do {                           \
+    int s_ = (expr);             \
+    if (s_ != SBP_OK) return s_; \
+  } while (0)
+
+int status_from_errword(unsigned w) {
+  if (!w) return SBP_OK;
+  for (int b = 1; b < 32; ++b)
+    if (w & (1u << b)) return b;
+  return SBP_ERR_INVALID;
+}
+
+VortexDev to_dev(const sbp_vortex_params& p) {
+  VortexDev v{};
+  v.n = std::max(0, std::min(5, (int)p.nvortices));
+  for (int i = 0; i < 5; ++i) {
+    v.x0[i] = p.vortices[i].x0;
+    v.y0[i] = p.vortices[i].y0;
+    v.rstar[i] = p.vortices[i].rstar;
+    v.eps[i] = p.vortices[i].eps;
+  }
+  v.mach = p.mach;
+  return v;
+}
+
+int64_t pitch_of(int nx) { return ((int64_t)nx + 15) / 16 * 16; }
+
+}  // namespace
+
+struct sbp_context_s {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  bool own_stream = false;
+  unsigned* d_err = nullptr;
+  BlockDev* d_tmpblock = nullptr;
+  TileDev* d_tmptiles = nullptr;
+  int tmptiles_cap = 0;
+  // autotuner cache: (op_eta, op_xi, upwind, total points, nblocks) -> shape
+  std::map<std::tuple<int, int, int, int64_t, int>, int> tuned;
+};
+
+namespace {
+
+struct HostBlock {
+  int nx, ny, op_eta, op_xi;
+  int64_t ld, ps;
+  double* d_metrics = nullptr;
+  double* d_x = nullptr;
+  double* d_y = nullptr;
+  double* d_buf[4] = {nullptr, nullptr, nullptr, nullptr};
+  double* d_k = nullptr;
+  int bc_kind[4] = {SBP_BC_THIS, SBP_BC_THIS, SBP_BC_THIS, SBP_BC_THIS};
+  int bc_nb[4] = {0, 0, 0, 0};
+  sbp_vortex_params bc_vortex[4];
+};
+
+struct Group {
+  int op_eta, op_xi;
+  bool upwind;
+  std::vector<int> blocks;
+  int shape = -1;
+  int tile_offset = 0, ntiles = 0;
+};
+
+}  // namespace
+
+struct sbp_system_s {
+  sbp_context ctx = nullptr;
+  std::vector<HostBlock> blocks;
+  std::vector<Group> groups;
+  BlockDev* d_blocks = nullptr;
+  TileDev* d_tiles = nullptr;
+  double* d_time = nullptr;      // system time
+  double* d_time_rhs = nullptr;  // explicit time for sbp_system_rhs
+  unsigned* d_err = nullptr;
+  double* d_partials = nullptr;
+  int partials_cap = 0;
+  int rhs_kind = SBP_RHS_AUTO;
+  int precision = SBP_FP64;
+  int forced_shape = -1;
+  bool dirty = true;  // device descriptors / tiles need rebuilding
+  // captured steps: graph of `graph_steps` RK4 steps for step size graph_dt
+  hipGraphExec_t graph = nullptr;
+  hipGraph_t graph_def = nullptr;
+  double graph_dt = NAN;
+  int graph_steps = 0;
+};
+
+namespace {
+
+bool block_upwind(const sbp_system_s* s, const HostBlock& b, int* err) {
+  OpInfo oe, ox;
+  op_info(b.op_eta, oe);
+  op_info(b.op_xi, ox);
+  const bool both = oe.upwind && ox.upwind;
+  *err = SBP_OK;
+  if (s->rhs_kind == SBP_RHS_TRAD) return false;
+  if (s->rhs_kind == SBP_RHS_UPWIND) {
+    if (!both) *err = SBP_ERR_NOT_UPWIND;
+    return true;
+  }
+  return both;
+}
+
+void destroy_graph(sbp_system_s* s) {
+  if (s->graph) hipGraphExecDestroy(s->graph);
+  if (s->graph_def) hipGraphDestroy(s->graph_def);
+  s->graph = nullptr;
+  s->graph_def = nullptr;
+  s->graph_dt = NAN;
+  s->graph_steps = 0;
+}
+
+int ntiles_along(int n, int t) { return n <= t ? 1 : (n + t - 1) / t; }
+
+// Device descriptor for block i with the tile counts of `shape`.
+BlockDev make_dev(const sbp_system_s* s, int i, int shape) {
+  const HostBlock& h = s->blocks[i];
+  BlockDev b{};
+  b.nx = h.nx;
+  b.ny = h.ny;
+  b.ld = (int)h.ld;
+  b.ps = h.ps;
+  b.op_xi = h.op_xi;
+  b.op_eta = h.op_eta;
+  const TileShape& ts = kTileShapes[shape < 0 ? 0 : shape];
+  b.ntx = ntiles_along(h.nx, ts.tx);
+  b.nty = ntiles_along(h.ny, ts.ty);
+  b.metrics = h.d_metrics;
+  b.xc = h.d_x;
+  b.yc = h.d_y;
+  for (int r = 0; r < 4; ++r) b.buf[r] = h.d_buf[r];
+  b.rhs_in = h.d_buf[BUF_YN];
+  b.rhs_out = h.d_k;
+  OpInfo ox, oe;
+  op_info(h.op_xi, ox);
+  op_info(h.op_eta, oe);
+  b.idx_xi = 1.0 / (ox.h2 ? 1.0 / (double)(h.nx - 2) : 1.0 / (double)(h.nx - 1));
+  b.idx_eta = 1.0 / (oe.h2 ? 1.0 / (double)(h.ny - 2) : 1.0 / (double)(h.ny - 1));
+  b.hi_xi = (ox.h2 ? (double)(h.nx - 2) : (double)(h.nx - 1)) / ox.h[0];
+  b.hi_eta = (oe.h2 ? (double)(h.ny - 2) : (double)(h.ny - 1)) / oe.h[0];
+  int e;
+  b.upwind = block_upwind(s, h, &e);
+  for (int d = 0; d < 4; ++d) {
+    b.edge_src[d] = h.bc_kind[d];
+    b.edge_nb[d] = h.bc_nb[d];
+    b.vortex[d] = to_dev(h.bc_vortex[d]);
+  }
+  return b;
+}
+
+// Tile list of a group ordered for the 8 XCDs of an MI355X: workgroup g runs
+// on XCD g mod 8, so the natural (block, row, column) order is cut into 8
+// contiguous chunks and interleaved; each XCD then sweeps one compact region
+// and the halo rows shared by neighbouring tiles stay in its L2.
+void build_tiles(const sbp_system_s* s, const Group& g, int shape, std::vector<TileDev>& out) {
+  const TileShape& ts = kTileShapes[shape];
+  std::vector<TileDev> nat;
+  for (int bi : g.blocks) {
+    const HostBlock& h = s->blocks[bi];
+    const int kx = ntiles_along(h.nx, ts.tx), ky = ntiles_along(h.ny, ts.ty);
+    for (int ty = 0; ty < ky; ++ty)
+      for (int tx = 0; tx < kx; ++tx) nat.push_back({bi, (int16_t)tx, (int16_t)ty});
+  }
+  const int T = (int)nat.size();
+  const int X = 8;
+  std::vector<int> start(X + 1, 0);
+  for (int x = 0; x < X; ++x) start[x + 1] = start[x] + T / X + (x < T % X ? 1 : 0);
+  const int maxlen = T / X + (T % X ? 1 : 0);
+  for (int j = 0; j < maxlen; ++j)
+    for (int x = 0; x < X; ++x)
+      if (start[x] + j < start[x + 1]) out.push_back(nat[start[x] + j]);
+}
+
+int rebuild(sbp_system_s* s) {
+  if (!s->dirty) return SBP_OK;
+  hipStream_t st = s->ctx->stream;
+  // groups by operator pair and RHS kind
+  s->groups.clear();
+  for (int i = 0; i < (int)s->blocks.size(); ++i) {
+    int e;
+    bool up = block_upwind(s, s->blocks[i], &e);
+    if (e) return e;
+    bool found = false;
+    for (auto& g : s->groups)
+      if (g.op_eta == s->blocks[i].op_eta && g.op_xi == s->blocks[i].op_xi && g.upwind == up) {
+        g.blocks.push_back(i);
+        found = true;
+      }
+    if (!found) s->groups.push_back({s->blocks[i].op_eta, s->blocks[i].op_xi, up, {i}});
+  }
+  // tile shapes: forced, cached, or autotuned after the descriptors exist
+  std::vector<TileDev> tiles;
+  std::vector<BlockDev> devs(s->blocks.size());
+  for (auto& g : s->groups) {
+    int64_t pts = 0;
+    for (int bi : g.blocks) pts += (int64_t)s->blocks[bi].nx * s->blocks[bi].ny;
+    auto key = std::make_tuple(g.op_eta, g.op_xi, (int)g.upwind, pts, (int)g.blocks.size());
+    if (s->forced_shape >= 0) {
+      g.shape = s->forced_shape;
+    } else {
+      auto it = s->ctx->tuned.find(key);
+      g.shape = it != s->ctx->tuned.end() ? it->second : -1;
+    }
+  }
+  bool need_tune = false;
+  for (auto& g : s->groups) need_tune = need_tune || g.shape < 0;
+  auto upload = [&](void) -> int {
+    tiles.clear();
+    for (auto& g : s->groups) {
+      const int sh = g.shape < 0 ? 0 : g.shape;
+      g.tile_offset = (int)tiles.size();
+      build_tiles(s, g, sh, tiles);
+      g.ntiles = (int)tiles.size() - g.tile_offset;
+      for (int bi : g.blocks) devs[bi] = make_dev(s, bi, sh);
+    }
+    if (s->d_blocks) hipFree(s->d_blocks);
+    if (s->d_tiles) hipFree(s->d_tiles);
+    s->d_blocks = nullptr;
+    s->d_tiles = nullptr;
+    SBP_TRY_HIP(hipMalloc(&s->d_blocks, sizeof(BlockDev) * devs.size()));
+    SBP_TRY_HIP(hipMalloc(&s->d_tiles, sizeof(TileDev) * std::max<size_t>(1, tiles.size())));
+    SBP_TRY_HIP(hipMemcpyAsync(s->d_blocks, devs.data(), sizeof(BlockDev) * devs.size(), hipMemcpyHostToDevice, st));
+    SBP_TRY_HIP(hipMemcpyAsync(s->d_tiles, tiles.data(), sizeof(TileDev) * tiles.size(), hipMemcpyHostToDevice, st));
+    SBP_TRY_HIP(hipStreamSynchronize(st));
+    return SBP_OK;
+  };
+  if (need_tune) {
+    // Autotuner: time MODE_S0 launches (they read y_n and write only the
+    // accumulator and the next stage input, so the state is untouched) for
+    // every tile shape and keep the fastest per group.
+    double gamma;
+    gamma_get(gamma);
+    std::vector<int> best(s->groups.size(), 0);
+    std::vector<float> best_ms(s->groups.size(), 1e30f);
+    hipEvent_t e0, e1;
+    SBP_TRY_HIP(hipEventCreate(&e0));
+    SBP_TRY_HIP(hipEventCreate(&e1));
+    for (int sh = 0; sh < kNumTileShapes; ++sh) {
+      // temporarily give every untuned group this shape
+      std::vector<int> saved;
+      for (auto& g : s->groups) {
+        saved.push_back(g.shape);
+        if (g.shape < 0) g.shape = sh;
+      }
+      SBP_TRY(upload());
+      for (size_t gi = 0; gi < s->groups.size(); ++gi) {
+        Group& g = s->groups[gi];
+        if (saved[gi] >= 0) continue;
+        StageArgs a{};
+        a.blocks = s->d_blocks;
+        a.tiles = s->d_tiles + g.tile_offset;
+        a.time = s->d_time;
+        a.error = s->d_err;
+        a.dt = 0.0;
+        a.gamma = gamma;
+        a.rk_scale_acc = 0.0;
+        a.rk_scale_next = 0.0;
+        a.time_offset = 0.0;
+        SBP_TRY_HIP(launch_stage(g.op_eta, g.op_xi, g.upwind, sh, MODE_S0, a, g.ntiles, st));  // warm-up
+        const int reps = 5;
+        SBP_TRY_HIP(hipEventRecord(e0, st));
+        for (int r = 0; r < reps; ++r) SBP_TRY_HIP(launch_stage(g.op_eta, g.op_xi, g.upwind, sh, MODE_S0, a, g.ntiles, st));
+        SBP_TRY_HIP(hipEventRecord(e1, st));
+        SBP_TRY_HIP(hipEventSynchronize(e1));
+        float ms = 0.f;
+        SBP_TRY_HIP(hipEventElapsedTime(&ms, e0, e1));
+        if (ms < best_ms[gi]) {
+          best_ms[gi] = ms;
+          best[gi] = sh;
+        }
+      }
+      for (size_t gi = 0; gi < s->groups.size(); ++gi) s->groups[gi].shape = saved[gi];
+    }
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    // the timing launches may have raised error bits on garbage accumulators: clear
+    SBP_TRY_HIP(hipMemsetAsync(s->d_err, 0, sizeof(unsigned), st));
+    for (size_t gi = 0; gi < s->groups.size(); ++gi) {
+      Group& g = s->groups[gi];
+      if (g.shape >= 0) continue;
+      g.shape = best[gi];
+      int64_t pts = 0;
+      for (int bi : g.blocks) pts += (int64_t)s->blocks[bi].nx * s->blocks[bi].ny;
+      s->ctx->tuned[std::make_tuple(g.op_eta, g.op_xi, (int)g.upwind, pts, (int)g.blocks.size())] = g.shape;
+    }
+  }
+  SBP_TRY(upload());
+  destroy_graph(s);
+  s->dirty = false;
+  return SBP_OK;
+}
+
+int check_err(sbp_system_s* s) {
+  unsigned w = 0;
+  SBP_TRY_HIP(hipMemcpyAsync(&w, s->d_err, sizeof(unsigned), hipMemcpyDeviceToHost, s->ctx->stream));
+  SBP_TRY_HIP(hipStreamSynchronize(s->ctx->stream));
+  if (w) SBP_TRY_HIP(hipMemsetAsync(s->d_err, 0, sizeof(unsigned), s->ctx->stream));
+  return status_from_errword(w);
+}
+
+// One fused RK4 step: four stage launches per group and the time update
+// (integrate::<Rk4>, utils/integrate/src/lib.rs:176-222).
+int enqueue_step(sbp_system_s* s, double dt, hipStream_t st) {
+  static const double A_next[3] = {0.5, 0.5, 1.0};                  // Rk4::A nonzero entries
+  static const double Bw[4] = {1.0 / 6.0, 1.0 / 3.0, 1.0 / 3.0, 1.0 / 6.0};  // Rk4::B
+  static const double C[3] = {0.5, 0.5, 1.0};                       // Rk4::C
+  double gamma;
+  gamma_get(gamma);
+  for (int stage = 0; stage < 4; ++stage) {
+    for (const Group& g : s->groups) {
+      StageArgs a{};
+      a.blocks = s->d_blocks;
+      a.tiles = s->d_tiles + g.tile_offset;
+      a.time = s->d_time;
+      a.error = s->d_err;
+      a.dt = dt;
+      a.gamma = gamma;
+      a.rk_scale_acc = Bw[stage] * dt;
+      a.rk_scale_next = stage < 3 ? A_next[stage] * dt : 0.0;
+      a.time_offset = stage == 0 ? 0.0 : dt * C[stage - 1];
+      SBP_TRY_HIP(launch_stage(g.op_eta, g.op_xi, g.upwind, g.shape, stage, a, g.ntiles, st));
+    }
+  }
+  SBP_TRY_HIP(launch_advance_time(s->d_time, dt, st));
+  return SBP_OK;
+}
+
+int ensure_graph(sbp_system_s* s, double dt, int steps) {
+  if (s->graph && s->graph_dt == dt && s->graph_steps == steps) return SBP_OK;
+  destroy_graph(s);
+  hipStream_t st = s->ctx->stream;
+  SBP_TRY_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+  int rc = SBP_OK;
+  for (int i = 0; i < steps && rc == SBP_OK; ++i) rc = enqueue_step(s, dt, st);
+  hipGraph_t g = nullptr;
+  hipError_t e = hipStreamEndCapture(st, &g);
+  if (rc != SBP_OK) {
+    if (g) hipGraphDestroy(g);
+    return rc;
+  }
+  SBP_TRY_HIP(e);
+  s->graph_def = g;
+  SBP_TRY_HIP(hipGraphInstantiate(&s->graph, g, nullptr, nullptr, 0));
+  s->graph_dt = dt;
+  s->graph_steps = steps;
+  return SBP_OK;
+}
+
+// Steps per captured graph: small systems are launch-bound, so they batch
+// several steps into one graph.
+int steps_per_graph(const sbp_system_s* s) {
+  int64_t pts = 0;
+  for (auto& b : s->blocks) pts += (int64_t)b.nx * b.ny;
+  if (pts >= (int64_t)1 << 22) return 1;
+  if (pts >= (int64_t)1 << 18) return 4;
+  return 16;
+}
+
+int advance_impl(sbp_system_s* s, double dt, int64_t nsteps) {
+  SBP_TRY(rebuild(s));
+  const int G = steps_per_graph(s);
+  int64_t done = 0;
+  if (nsteps >= G) {
+    SBP_TRY(ensure_graph(s, dt, G));
+    for (; done + G <= nsteps; done += G) SBP_TRY_HIP(hipGraphLaunch(s->graph, s->ctx->stream));
+  }
+  for (; done < nsteps; ++done) SBP_TRY(enqueue_step(s, dt, s->ctx->stream));
+  return SBP_OK;
+}
+
+int ensure_partials(sbp_system_s* s, int n) {
+  if (s->partials_cap >= n) return SBP_OK;
+  if (s->d_partials) hipFree(s->d_partials);
+  s->d_partials = nullptr;
+  SBP_TRY_HIP(hipMalloc(&s->d_partials, sizeof(double) * 2 * n));
+  s->partials_cap = n;
+  return SBP_OK;
+}
+
+int h2_sum(sbp_system_s* s, int bi, const double* b, const VortexDev* vp_host, double t, long double& sum) {
+  HostBlock& h = s->blocks[bi];
+  const int nwg = reduce_h2_workgroups(h.ny, h.nx);
+  SBP_TRY(ensure_partials(s, nwg + 64));
+  VortexDev* d_vp = nullptr;
+  hipStream_t st = s->ctx->stream;
+  double gamma;
+  gamma_get(gamma);
+  if (vp_host) {
+    // the parameters live right after the partials
+    d_vp = reinterpret_cast<VortexDev*>(s->d_partials + 2 * nwg);
+    SBP_TRY_HIP(hipMemcpyAsync(d_vp, vp_host, sizeof(VortexDev), hipMemcpyHostToDevice, st));
+  }
+  SBP_TRY_HIP(launch_h2_reduce(h.op_eta, h.op_xi, h.d_buf[BUF_YN], b, d_vp, gamma, t, h.d_x, h.d_y, h.ps, h.ld, h.ny, h.nx,
+                               s->d_partials, st));
+  std::vector<double> part(2 * nwg);
+  SBP_TRY_HIP(hipMemcpyAsync(part.data(), s->d_partials, sizeof(double) * 2 * nwg, hipMemcpyDeviceToHost, st));
+  SBP_TRY_HIP(hipStreamSynchronize(st));
+  long double acc = 0.0L;
+  for (int i = 0; i < nwg; ++i) acc += (long double)part[2 * i] + (long double)part[2 * i + 1];
+  sum = acc;
+  return SBP_OK;
+}
+
+}  // namespace
+
+// =========================================================================
+// C ABI
+// =========================================================================
+extern "C" {
+
+const char* sbp_status_string(int status) {
+  switch (status) {
+    case SBP_OK: return "ok";
+    case SBP_ERR_INVALID: return "invalid argument";
+    case SBP_ERR_SHAPE: return "operator needs n >= 2M and n >= N";
+    case SBP_ERR_DETJ: return "grid has det J <= 0";
+    case SBP_ERR_RHO: return "density <= 0";
+    case SBP_ERR_PRESSURE: return "pressure <= 0";
+    case SBP_ERR_GAMMA: return "GAMMA is not defined";
+    case SBP_ERR_EDGE: return "neighbouring edges differ in length";
+    case SBP_ERR_HIP: return "HIP runtime error";
+    case SBP_ERR_NOT_UPWIND: return "This is not an upwind operator";
+    case SBP_ERR_OOM: return "device out of memory";
+    case SBP_ERR_GAMMA_SET: return "GAMMA already set";
+    default: return "unknown status";
+  }
+}
+
+int sbp_set_gamma(double gamma) {
+  std::lock_guard<std::mutex> lk(g_gamma_mu);
+  if (g_gamma_set) return gamma == g_gamma ? SBP_OK : SBP_ERR_GAMMA_SET;
+  if (!(gamma > 1.0)) return SBP_ERR_INVALID;
+  g_gamma = gamma;
+  g_gamma_set = true;
+  return SBP_OK;
+}
+
+int sbp_get_gamma(double* gamma) {
+  if (!gamma) return SBP_ERR_INVALID;
+  double g;
+  if (!gamma_get(g)) return SBP_ERR_GAMMA;
+  *gamma = g;
+  return SBP_OK;
+}
+
+int sbp_operator_info(int op, int* M, int* N, int* D, int* is_h2, int* is_upwind) {
+  OpInfo o;
+  if (!op_info(op, o)) return SBP_ERR_INVALID;
+  if (M) *M = o.M;
+  if (N) *N = o.N;
+  if (D) *D = o.D;
+  if (is_h2) *is_h2 = o.h2;
+  if (is_upwind) *is_upwind = o.upwind;
+  return SBP_OK;
+}
+
+int sbp_operator_h(int op, double* h_out) {
+  OpInfo o;
+  if (!op_info(op, o) || !h_out) return SBP_ERR_INVALID;
+  for (int i = 0; i < o.M; ++i) h_out[i] = o.h[i];
+  return SBP_OK;
+}
+
+int sbp_operator_coefficients(int op, int kind, double* start, double* diag, double* end) {
+  if (!start || !diag || !end || (kind != SBP_DIFF && kind != SBP_DISS)) return SBP_ERR_INVALID;
+  bool ok = false;
+  switch (op) {
+    case UPWIND4: coefs_of<ops::Upwind4>(kind, start, diag, end, ok); break;
+    case UPWIND4H2: coefs_of<ops::Upwind4h2>(kind, start, diag, end, ok); break;
+    case UPWIND9: coefs_of<ops::Upwind9>(kind, start, diag, end, ok); break;
+    case UPWIND9H2: coefs_of<ops::Upwind9h2>(kind, start, diag, end, ok); break;
+    case SBP4: coefs_of<ops::SBP4>(kind, start, diag, end, ok); break;
+    case SBP8: coefs_of<ops::SBP8>(kind, start, diag, end, ok); break;
+    default: return SBP_ERR_INVALID;
+  }
+  return ok ? SBP_OK : SBP_ERR_NOT_UPWIND;
+}
+
+int sbp_context_create(int device, sbp_context* out) {
+  if (!out) return SBP_ERR_INVALID;
+  int n = 0;
+  SBP_TRY_HIP(hipGetDeviceCount(&n));
+  if (device < 0 || device >= n) return SBP_ERR_INVALID;
+  SBP_TRY_HIP(hipSetDevice(device));
+  auto* c = new sbp_context_s();
+  c->device = device;
+  hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    delete c;
+    return from_hip(e);
+  }
+  c->own_stream = true;
+  if ((e = hipMalloc(&c->d_err, sizeof(unsigned))) != hipSuccess || (e = hipMemset(c->d_err, 0, sizeof(unsigned))) != hipSuccess ||
+      (e = hipMalloc(&c->d_tmpblock, sizeof(BlockDev))) != hipSuccess) {
+    hipStreamDestroy(c->stream);
+    delete c;
+    return from_hip(e);
+  }
+  *out = c;
+  return SBP_OK;
+}
+
+int sbp_context_destroy(sbp_context ctx) {
+  if (!ctx) return SBP_ERR_INVALID;
+  hipSetDevice(ctx->device);
+  hipStreamSynchronize(ctx->stream);
+  if (ctx->own_stream) hipStreamDestroy(ctx->stream);
+  hipFree(ctx->d_err);
+  hipFree(ctx->d_tmpblock);
+  if (ctx->d_tmptiles) hipFree(ctx->d_tmptiles);
+  delete ctx;
+  return SBP_OK;
+}
+
+int sbp_context_set_stream(sbp_context ctx, void* hip_stream) {
+  if (!ctx) return SBP_ERR_INVALID;
+  SBP_TRY_HIP(hipSetDevice(ctx->device));
+  SBP_TRY_HIP(hipStreamSynchronize(ctx->stream));
+  if (ctx->own_stream) hipStreamDestroy(ctx->stream);
+  if (hip_stream) {
+    ctx->stream = (hipStream_t)hip_stream;
+    ctx->own_stream = false;
+  } else {
+    SBP_TRY_HIP(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+    ctx->own_stream = true;
+  }
+  return SBP_OK;
+}
+
+int sbp_context_synchronize(sbp_context ctx) {
+  if (!ctx) return SBP_ERR_INVALID;
+  SBP_TRY_HIP(hipStreamSynchronize(ctx->stream));
+  return SBP_OK;
+}
+
+static int apply(sbp_context ctx, int op, int kind, int axis, const double* prev, int64_t ldp, double* fut, int64_t ldf, int ny,
+                 int nx) {
+  if (!ctx || !prev || !fut || ny <= 0 || nx <= 0 || ldp < nx || ldf < nx) return SBP_ERR_INVALID;
+  OpInfo o;
+  if (!op_info(op, o)) return SBP_ERR_INVALID;
+  if (kind == SBP_DISS && !o.upwind) return SBP_ERR_NOT_UPWIND;
+  const int n = axis == 0 ? nx : ny;
+  if (n < 2 * o.M || n < o.N) return SBP_ERR_SHAPE;
+  SBP_TRY_HIP(hipSetDevice(ctx->device));
+  SBP_TRY_HIP(launch_apply_op(op, kind, axis, prev, ldp, fut, ldf, ny, nx, ctx->stream));
+  return SBP_OK;
+}
+
+int sbp_diffxi(sbp_context ctx, int op, const double* prev, int64_t ld_prev, double* fut, int64_t ld_fut, int ny, int nx) {
+  return apply(ctx, op, SBP_DIFF, 0, prev, ld_prev, fut, ld_fut, ny, nx);
+}
+int sbp_diffeta(sbp_context ctx, int op, const double* prev, int64_t ld_prev, double* fut, int64_t ld_fut, int ny, int nx) {
+  return apply(ctx, op, SBP_DIFF, 1, prev, ld_prev, fut, ld_fut, ny, nx);
+}
+int sbp_dissxi(sbp_context ctx, int op, const double* prev, int64_t ld_prev, double* fut, int64_t ld_fut, int ny, int nx) {
+  return apply(ctx, op, SBP_DISS, 0, prev, ld_prev, fut, ld_fut, ny, nx);
+}
+int sbp_disseta(sbp_context ctx, int op, const double* prev, int64_t ld_prev, double* fut, int64_t ld_fut, int ny, int nx) {
+  return apply(ctx, op, SBP_DISS, 1, prev, ld_prev, fut, ld_fut, ny, nx);
+}
+
+static int check_shape(int op_eta, int op_xi, int ny, int nx) {
+  OpInfo oe, ox;
+  if (!op_info(op_eta, oe) || !op_info(op_xi, ox)) return SBP_ERR_INVALID;
+  if (nx < 2 * ox.M || nx < ox.N || ny < 2 * oe.M || ny < oe.N) return SBP_ERR_SHAPE;
+  return SBP_OK;
+}
+
+int sbp_metrics(sbp_context ctx, int op_eta, int op_xi, int ny, int nx, const double* x, const double* y, double* metrics) {
+  if (!ctx || !x || !y || !metrics) return SBP_ERR_INVALID;
+  SBP_TRY(check_shape(op_eta, op_xi, ny, nx));
+  SBP_TRY_HIP(hipSetDevice(ctx->device));
+  const int64_t ps = (int64_t)nx * ny;
+  double* tmp = nullptr;
+  SBP_TRY_HIP(hipMallocAsync((void**)&tmp, sizeof(double) * 4 * ps, ctx->stream));
+  hipStream_t st = ctx->stream;
+  int rc = SBP_OK;
+  if (rc == SBP_OK) rc = from_hip(launch_apply_op(op_xi, SBP_DIFF, 0, x, nx, tmp, nx, ny, nx, st));
+  if (rc == SBP_OK) rc = from_hip(launch_apply_op(op_eta, SBP_DIFF, 1, x, nx, tmp + ps, nx, ny, nx, st));
+  if (rc == SBP_OK) rc = from_hip(launch_apply_op(op_xi, SBP_DIFF, 0, y, nx, tmp + 2 * ps, nx, ny, nx, st));
+  if (rc == SBP_OK) rc = from_hip(launch_apply_op(op_eta, SBP_DIFF, 1, y, nx, tmp + 3 * ps, nx, ny, nx, st));
+  if (rc == SBP_OK)
+    rc = from_hip(launch_metrics_finish(tmp, tmp + ps, tmp + 2 * ps, tmp + 3 * ps, metrics, ps, nx, ny, nx, ctx->d_err, st));
+  hipFreeAsync(tmp, st);
+  if (rc != SBP_OK) return rc;
+  unsigned w = 0;
+  SBP_TRY_HIP(hipMemcpyAsync(&w, ctx->d_err, sizeof(unsigned), hipMemcpyDeviceToHost, st));
+  SBP_TRY_HIP(hipStreamSynchronize(st));
+  if (w) SBP_TRY_HIP(hipMemsetAsync(ctx->d_err, 0, sizeof(unsigned), st));
+  return status_from_errword(w);
+}
+
+int sbp_rhs(sbp_context ctx, int kind, int op_eta, int op_xi, int ny, int nx, double* k, const double* y, const double* metrics,
+            const double* const* boundaries) {
+  if (!ctx || !k || !y || !metrics || !boundaries) return SBP_ERR_INVALID;
+  if (kind != SBP_RHS_TRAD && kind != SBP_RHS_UPWIND) return SBP_ERR_INVALID;
+  SBP_TRY(check_shape(op_eta, op_xi, ny, nx));
+  OpInfo oe, ox;
+  op_info(op_eta, oe);
+  op_info(op_xi, ox);
+  const bool up = kind == SBP_RHS_UPWIND;
+  if (up && !(oe.upwind && ox.upwind)) return SBP_ERR_NOT_UPWIND;
+  double gamma;
+  if (!gamma_get(gamma)) return SBP_ERR_GAMMA;
+  for (int d = 0; d < 4; ++d)
+    if (!boundaries[d]) return SBP_ERR_INVALID;
+  SBP_TRY_HIP(hipSetDevice(ctx->device));
+  const int shape = 0;
+  const TileShape& ts = kTileShapes[shape];
+  BlockDev b{};
+  b.nx = nx;
+  b.ny = ny;
+  b.ld = nx;
+  b.ps = (int64_t)nx * ny;
+  b.op_xi = op_xi;
+  b.op_eta = op_eta;
+  b.upwind = up;
+  b.ntx = ntiles_along(nx, ts.tx);
+  b.nty = ntiles_along(ny, ts.ty);
+  b.metrics = metrics;
+  b.rhs_in = const_cast<double*>(y);
+  b.rhs_out = k;
+  b.idx_xi = 1.0 / (ox.h2 ? 1.0 / (double)(nx - 2) : 1.0 / (double)(nx - 1));
+  b.idx_eta = 1.0 / (oe.h2 ? 1.0 / (double)(ny - 2) : 1.0 / (double)(ny - 1));
+  b.hi_xi = (ox.h2 ? (double)(nx - 2) : (double)(nx - 1)) / ox.h[0];
+  b.hi_eta = (oe.h2 ? (double)(ny - 2) : (double)(ny - 1)) / oe.h[0];
+  for (int d = 0; d < 4; ++d) {
+    b.edge_src[d] = EDGE_EXPLICIT;
+    b.edge_ptr[d] = boundaries[d];
+    b.edge_cs[d] = (d == SBP_NORTH || d == SBP_SOUTH) ? nx : ny;
+  }
+  std::vector<TileDev> tiles;
+  for (int ty = 0; ty < b.nty; ++ty)
+    for (int tx = 0; tx < b.ntx; ++tx) tiles.push_back({0, (int16_t)tx, (int16_t)ty});
+  hipStream_t st = ctx->stream;
+  SBP_TRY_HIP(hipStreamSynchronize(st));
+  if (ctx->tmptiles_cap < (int)tiles.size()) {
+    if (ctx->d_tmptiles) hipFree(ctx->d_tmptiles);
+    ctx->d_tmptiles = nullptr;
+    SBP_TRY_HIP(hipMalloc(&ctx->d_tmptiles, sizeof(TileDev) * tiles.size()));
+    ctx->tmptiles_cap = (int)tiles.size();
+  }
+  SBP_TRY_HIP(hipMemcpy(ctx->d_tmpblock, &b, sizeof(BlockDev), hipMemcpyHostToDevice));
+  SBP_TRY_HIP(hipMemcpy(ctx->d_tmptiles, tiles.data(), sizeof(TileDev) * tiles.size(), hipMemcpyHostToDevice));
+  StageArgs a{};
+  a.blocks = ctx->d_tmpblock;
+  a.tiles = ctx->d_tmptiles;
+  a.time = nullptr;
+  a.error = ctx->d_err;
+  a.gamma = gamma;
+  SBP_TRY_HIP(launch_stage(op_eta, op_xi, up, shape, MODE_RHS, a, (int)tiles.size(), st));
+  unsigned w = 0;
+  SBP_TRY_HIP(hipMemcpyAsync(&w, ctx->d_err, sizeof(unsigned), hipMemcpyDeviceToHost, st));
+  SBP_TRY_HIP(hipStreamSynchronize(st));
+  if (w) SBP_TRY_HIP(hipMemsetAsync(ctx->d_err, 0, sizeof(unsigned), st));
+  return status_from_errword(w);
+}
+
+int sbp_system_create(sbp_context ctx, int nblocks, const int* ny, const int* nx, const int* op_eta, const int* op_xi,
+                      const double* const* x, const double* const* y, sbp_system* out) {
+  if (!ctx || nblocks <= 0 || nblocks > 32767 || !ny || !nx || !op_eta || !op_xi || !x || !y || !out) return SBP_ERR_INVALID;
+  for (int i = 0; i < nblocks; ++i) {
+    SBP_TRY(check_shape(op_eta[i], op_xi[i], ny[i], nx[i]));
+    if (!x[i] || !y[i] || nx[i] > 32767 * 16 || ny[i] > 32767 * 16) return SBP_ERR_INVALID;
+  }
+  SBP_TRY_HIP(hipSetDevice(ctx->device));
+  std::unique_ptr<sbp_system_s> s(new sbp_system_s());
+  s->ctx = ctx;
+  hipStream_t st = ctx->stream;
+  auto fail = [&](int rc) {
+    sbp_system_destroy(s.release());
+    return rc;
+  };
+  s->blocks.resize(nblocks);
+  for (int i = 0; i < nblocks; ++i) {
+    HostBlock& h = s->blocks[i];
+    h.nx = nx[i];
+    h.ny = ny[i];
+    h.op_eta = op_eta[i];
+    h.op_xi = op_xi[i];
+    h.ld = pitch_of(h.nx);
+    h.ps = h.ld * h.ny;
+    std::memset(h.bc_vortex, 0, sizeof(h.bc_vortex));
+  }
+  int rc = SBP_OK;
+  for (int i = 0; i < nblocks && rc == SBP_OK; ++i) {
+    HostBlock& h = s->blocks[i];
+    const size_t plane = sizeof(double) * (size_t)h.ps;
+    if ((rc = from_hip(hipMalloc((void**)&h.d_metrics, 5 * plane))) != SBP_OK) break;
+    if ((rc = from_hip(hipMalloc((void**)&h.d_x, plane))) != SBP_OK) break;
+    if ((rc = from_hip(hipMalloc((void**)&h.d_y, plane))) != SBP_OK) break;
+    for (int r = 0; r < 4 && rc == SBP_OK; ++r) rc = from_hip(hipMalloc((void**)&h.d_buf[r], 4 * plane));
+    if (rc != SBP_OK) break;
+    // zero everything once so that pitch padding never holds garbage
+    for (int r = 0; r < 4 && rc == SBP_OK; ++r) rc = from_hip(hipMemsetAsync(h.d_buf[r], 0, 4 * plane, st));
+    if (rc == SBP_OK) rc = from_hip(hipMemsetAsync(h.d_metrics, 0, 5 * plane, st));
+    if (rc == SBP_OK)
+      rc = from_hip(hipMemcpy2DAsync(h.d_x, h.ld * 8, x[i], (size_t)h.nx * 8, (size_t)h.nx * 8, h.ny, hipMemcpyHostToDevice, st));
+    if (rc == SBP_OK)
+      rc = from_hip(hipMemcpy2DAsync(h.d_y, h.ld * 8, y[i], (size_t)h.nx * 8, (size_t)h.nx * 8, h.ny, hipMemcpyHostToDevice, st));
+    // metrics (grid.rs:95-139); the accumulator buffer holds the four derivatives
+    double* t = h.d_buf[BUF_ACC];
+    if (rc == SBP_OK) rc = from_hip(launch_apply_op(h.op_xi, SBP_DIFF, 0, h.d_x, h.ld, t, h.ld, h.ny, h.nx, st));
+    if (rc == SBP_OK) rc = from_hip(launch_apply_op(h.op_eta, SBP_DIFF, 1, h.d_x, h.ld, t + h.ps, h.ld, h.ny, h.nx, st));
+    if (rc == SBP_OK) rc = from_hip(launch_apply_op(h.op_xi, SBP_DIFF, 0, h.d_y, h.ld, t + 2 * h.ps, h.ld, h.ny, h.nx, st));
+    if (rc == SBP_OK) rc = from_hip(launch_apply_op(h.op_eta, SBP_DIFF, 1, h.d_y, h.ld, t + 3 * h.ps, h.ld, h.ny, h.nx, st));
+    if (rc == SBP_OK)
+      rc = from_hip(launch_metrics_finish(t, t + h.ps, t + 2 * h.ps, t + 3 * h.ps, h.d_metrics, h.ps, h.ld, h.ny, h.nx, ctx->d_err,
+                                          st));
+  }
+  if (rc != SBP_OK) return fail(rc);
+  if ((rc = from_hip(hipMalloc(&s->d_time, sizeof(double) * 2))) != SBP_OK) return fail(rc);
+  s->d_time_rhs = s->d_time + 1;
+  if ((rc = from_hip(hipMemsetAsync(s->d_time, 0, sizeof(double) * 2, st))) != SBP_OK) return fail(rc);
+  if ((rc = from_hip(hipMalloc(&s->d_err, sizeof(unsigned)))) != SBP_OK) return fail(rc);
+  if ((rc = from_hip(hipMemsetAsync(s->d_err, 0, sizeof(unsigned), st))) != SBP_OK) return fail(rc);
+  unsigned w = 0;
+  if ((rc = from_hip(hipMemcpyAsync(&w, ctx->d_err, sizeof(unsigned), hipMemcpyDeviceToHost, st))) != SBP_OK) return fail(rc);
+  if ((rc = from_hip(hipStreamSynchronize(st))) != SBP_OK) return fail(rc);
+  if (w) {
+    hipMemsetAsync(ctx->d_err, 0, sizeof(unsigned), st);
+    return fail(status_from_errword(w));
+  }
+  *out = s.release();
+  return SBP_OK;
+}
+
+int sbp_system_destroy(sbp_system s) {
+  if (!s) return SBP_ERR_INVALID;
+  hipSetDevice(s->ctx->device);
+  hipStreamSynchronize(s->ctx->stream);
+  destroy_graph(s);
+  for (auto& h : s->blocks) {
+    hipFree(h.d_metrics);
+    hipFree(h.d_x);
+    hipFree(h.d_y);
+    for (int r = 0; r < 4; ++r) hipFree(h.d_buf[r]);
+    if (h.d_k) hipFree(h.d_k);
+  }
+  if (s->d_blocks) hipFree(s->d_blocks);
+  if (s->d_tiles) hipFree(s->d_tiles);
+  if (s->d_time) hipFree(s->d_time);
+  if (s->d_err) hipFree(s->d_err);
+  if (s->d_partials) hipFree(s->d_partials);
+  delete s;
+  return SBP_OK;
+}
+
+int sbp_system_set_bc(sbp_system s, int block, int dir, int kind, int neighbour, const sbp_vortex_params* vortex) {
+  if (!s || block < 0 || block >= (int)s->blocks.size() || dir < 0 || dir > 3) return SBP_ERR_INVALID;
+  HostBlock& h = s->blocks[block];
+  if (kind == SBP_BC_GRID) {
+    if (neighbour < 0 || neighbour >= (int)s->blocks.size()) return SBP_ERR_INVALID;
+    const HostBlock& o = s->blocks[neighbour];
+    const bool ns = dir == SBP_NORTH || dir == SBP_SOUTH;
+    if ((ns ? h.nx : h.ny) != (ns ? o.nx : o.ny)) return SBP_ERR_EDGE;
+  } else if (kind == SBP_BC_VORTEX) {
+    if (!vortex || vortex->nvortices < 0 || vortex->nvortices > 5) return SBP_ERR_INVALID;
+    h.bc_vortex[dir] = *vortex;
+  } else if (kind != SBP_BC_THIS) {
+    return SBP_ERR_INVALID;
+  }
+  h.bc_kind[dir] = kind;
+  h.bc_nb[dir] = neighbour;
+  s->dirty = true;
+  return SBP_OK;
+}
+
+int sbp_system_set_rhs_kind(sbp_system s, int kind) {
+  if (!s || kind < SBP_RHS_AUTO || kind > SBP_RHS_UPWIND) return SBP_ERR_INVALID;
+  for (auto& b : s->blocks) {
+    int e;
+    int saved = s->rhs_kind;
+    s->rhs_kind = kind;
+    block_upwind(s, b, &e);
+    s->rhs_kind = saved;
+    if (e) return e;
+  }
+  s->rhs_kind = kind;
+  s->dirty = true;
+  return SBP_OK;
+}
+
+int sbp_system_set_precision(sbp_system s, int precision) {
+  if (!s || (precision != SBP_FP64 && precision != SBP_MIXED)) return SBP_ERR_INVALID;
+  if (precision == SBP_MIXED) return SBP_ERR_INVALID;  // FP32-RHS kernels not built yet
+  s->precision = precision;
+  s->dirty = true;
+  return SBP_OK;
+}
+
+int sbp_system_set_state(sbp_system s, int block, const double* host) {
+  if (!s || !host || block < 0 || block >= (int)s->blocks.size()) return SBP_ERR_INVALID;
+  HostBlock& h = s->blocks[block];
+  SBP_TRY_HIP(hipSetDevice(s->ctx->device));
+  for (int c = 0; c < 4; ++c)
+    SBP_TRY_HIP(hipMemcpy2DAsync(h.d_buf[BUF_YN] + c * h.ps, h.ld * 8, host + (size_t)c * h.nx * h.ny, (size_t)h.nx * 8,
+                                 (size_t)h.nx * 8, h.ny, hipMemcpyHostToDevice, s->ctx->stream));
+  SBP_TRY_HIP(hipStreamSynchronize(s->ctx->stream));
+  return SBP_OK;
+}
+
+int sbp_system_get_state(sbp_system s, int block, double* host) {
+  if (!s || !host || block < 0 || block >= (int)s->blocks.size()) return SBP_ERR_INVALID;
+  HostBlock& h = s->blocks[block];
+  SBP_TRY_HIP(hipSetDevice(s->ctx->device));
+  for (int c = 0; c < 4; ++c)
+    SBP_TRY_HIP(hipMemcpy2DAsync(host + (size_t)c * h.nx * h.ny, (size_t)h.nx * 8, h.d_buf[BUF_YN] + c * h.ps, h.ld * 8,
+                                 (size_t)h.nx * 8, h.ny, hipMemcpyDeviceToHost, s->ctx->stream));
+  SBP_TRY_HIP(hipStreamSynchronize(s->ctx->stream));
+  return SBP_OK;
+}
+
+int sbp_system_state_device(sbp_system s, int block, double** dev, int64_t* ld, int64_t* plane_stride) {
+  if (!s || !dev || block < 0 || block >= (int)s->blocks.size()) return SBP_ERR_INVALID;
+  HostBlock& h = s->blocks[block];
+  *dev = h.d_buf[BUF_YN];
+  if (ld) *ld = h.ld;
+  if (plane_stride) *plane_stride = h.ps;
+  return SBP_OK;
+}
+
+int sbp_system_get_metrics(sbp_system s, int block, double* host) {
+  if (!s || !host || block < 0 || block >= (int)s->blocks.size()) return SBP_ERR_INVALID;
+  HostBlock& h = s->blocks[block];
+  SBP_TRY_HIP(hipSetDevice(s->ctx->device));
+  for (int c = 0; c < 5; ++c)
+    SBP_TRY_HIP(hipMemcpy2DAsync(host + (size_t)c * h.nx * h.ny, (size_t)h.nx * 8, h.d_metrics + c * h.ps, h.ld * 8,
+                                 (size_t)h.nx * 8, h.ny, hipMemcpyDeviceToHost, s->ctx->stream));
+  SBP_TRY_HIP(hipStreamSynchronize(s->ctx->stream));
+  return SBP_OK;
+}
+
+int sbp_system_vortex(sbp_system s, double t, const sbp_vortex_params* params) {
+  if (!s || !params || params->nvortices < 0 || params->nvortices > 5) return SBP_ERR_INVALID;
+  double gamma;
+  if (!gamma_get(gamma)) return SBP_ERR_GAMMA;
+  SBP_TRY_HIP(hipSetDevice(s->ctx->device));
+  const VortexDev vp = to_dev(*params);
+  for (auto& h : s->blocks)
+    SBP_TRY_HIP(launch_vortex(vp, gamma, t, h.d_x, h.d_y, h.d_buf[BUF_YN], h.ps, h.ld, h.ny, h.nx, s->d_err, s->ctx->stream));
+  return check_err(s);
+}
+
+int sbp_system_rhs(sbp_system s, double t, double* k_host) {
+  if (!s || !k_host) return SBP_ERR_INVALID;
+  double gamma;
+  if (!gamma_get(gamma)) return SBP_ERR_GAMMA;
+  SBP_TRY_HIP(hipSetDevice(s->ctx->device));
+  hipStream_t st = s->ctx->stream;
+  bool alloc = false;
+  for (auto& h : s->blocks)
+    if (!h.d_k) {
+      SBP_TRY_HIP(hipMalloc(&h.d_k, sizeof(double) * 4 * h.ps));
+      SBP_TRY_HIP(hipMemsetAsync(h.d_k, 0, sizeof(double) * 4 * h.ps, st));
+      alloc = true;
+    }
+  if (alloc) s->dirty = true;
+  SBP_TRY(rebuild(s));
+  SBP_TRY_HIP(hipMemcpyAsync(s->d_time_rhs, &t, sizeof(double), hipMemcpyHostToDevice, st));
+  for (const Group& g : s->groups) {
+    StageArgs a{};
+    a.blocks = s->d_blocks;
+    a.tiles = s->d_tiles + g.tile_offset;
+    a.time = s->d_time_rhs;
+    a.error = s->d_err;
+    a.gamma = gamma;
+    SBP_TRY_HIP(launch_stage(g.op_eta, g.op_xi, g.upwind, g.shape, MODE_RHS, a, g.ntiles, st));
+  }
+  size_t off = 0;
+  for (auto& h : s->blocks) {
+    for (int c = 0; c < 4; ++c)
+      SBP_TRY_HIP(hipMemcpy2DAsync(k_host + off + (size_t)c * h.nx * h.ny, (size_t)h.nx * 8, h.d_k + c * h.ps, h.ld * 8,
+                                   (size_t)h.nx * 8, h.ny, hipMemcpyDeviceToHost, st));
+    off += (size_t)4 * h.nx * h.ny;
+  }
+  return check_err(s);
+}
+
+int sbp_system_advance(sbp_system s, double dt, int64_t nsteps) {
+  if (!s || nsteps < 0 || !std::isfinite(dt)) return SBP_ERR_INVALID;
+  double gamma;
+  if (!gamma_get(gamma)) return SBP_ERR_GAMMA;
+  SBP_TRY_HIP(hipSetDevice(s->ctx->device));
+  SBP_TRY(advance_impl(s, dt, nsteps));
+  return check_err(s);
+}
+
+int sbp_system_get_time(sbp_system s, double* t) {
+  if (!s || !t) return SBP_ERR_INVALID;
+  SBP_TRY_HIP(hipMemcpyAsync(t, s->d_time, sizeof(double), hipMemcpyDeviceToHost, s->ctx->stream));
+  SBP_TRY_HIP(hipStreamSynchronize(s->ctx->stream));
+  return SBP_OK;
+}
+
+int sbp_system_set_time(sbp_system s, double t) {
+  if (!s) return SBP_ERR_INVALID;
+  SBP_TRY_HIP(hipMemcpyAsync(s->d_time, &t, sizeof(double), hipMemcpyHostToDevice, s->ctx->stream));
+  SBP_TRY_HIP(hipStreamSynchronize(s->ctx->stream));
+  return SBP_OK;
+}
+
+int sbp_system_max_dt(sbp_system s, double* out) {
+  if (!s || !out) return SBP_ERR_INVALID;
+  SBP_TRY_HIP(hipSetDevice(s->ctx->device));
+  bool is_h2 = false;
+  for (auto& h : s->blocks) {
+    OpInfo oe, ox;
+    op_info(h.op_eta, oe);
+    op_info(h.op_xi, ox);
+    is_h2 = is_h2 || oe.h2 || ox.h2;
+  }
+  const double c_max = is_h2 ? 0.5 : 1.0;
+  double max_dt = INFINITY;
+  hipStream_t st = s->ctx->stream;
+  for (auto& h : s->blocks) {
+    const int nwg = reduce_max_workgroups(h.ny, h.nx);
+    SBP_TRY(ensure_partials(s, nwg));
+    SBP_TRY_HIP(launch_maxspeed(h.d_buf[BUF_YN], h.d_metrics, h.ps, h.ld, h.ny, h.nx, s->d_partials, st));
+    std::vector<double> part(2 * nwg);
+    SBP_TRY_HIP(hipMemcpyAsync(part.data(), s->d_partials, sizeof(double) * 2 * nwg, hipMemcpyDeviceToHost, st));
+    SBP_TRY_HIP(hipStreamSynchronize(st));
+    double mu = 0.0, mv = 0.0;
+    for (int i = 0; i < nwg; ++i) {
+      mu = std::fmax(mu, part[2 * i]);
+      mv = std::fmax(mv, part[2 * i + 1]);
+    }
+    const double dx = 1.0 / (double)h.nx, dy = 1.0 / (double)h.ny;
+    const double c_dt = std::fmax(mu / dx, mv / dy);
+    max_dt = std::fmin(max_dt, c_max / c_dt);
+  }
+  *out = max_dt;
+  return SBP_OK;
+}
+
+int sbp_system_error_vortex(sbp_system s, const sbp_vortex_params* params, double* out) {
+  if (!s || !params || !out || params->nvortices < 0 || params->nvortices > 5) return SBP_ERR_INVALID;
+  if (double g; !gamma_get(g)) return SBP_ERR_GAMMA;
+  SBP_TRY_HIP(hipSetDevice(s->ctx->device));
+  double t;
+  SBP_TRY(sbp_system_get_time(s, &t));
+  const VortexDev vp = to_dev(*params);
+  double e = 0.0;
+  for (int i = 0; i < (int)s->blocks.size(); ++i) {
+    long double sum;
+    SBP_TRY(h2_sum(s, i, nullptr, &vp, t, sum));
+    e += std::sqrt((double)sum);
+  }
+  *out = e;
+  return SBP_OK;
+}
+
+int sbp_system_energy(sbp_system s, double* out) {
+  if (!s || !out) return SBP_ERR_INVALID;
+  SBP_TRY_HIP(hipSetDevice(s->ctx->device));
+  long double e = 0.0L;
+  for (int i = 0; i < (int)s->blocks.size(); ++i) {
+    long double sum;
+    SBP_TRY(h2_sum(s, i, nullptr, nullptr, 0.0, sum));
+    e += sum;
+  }
+  *out = (double)e;
+  return SBP_OK;
+}
+
+int sbp_system_check(sbp_system s) {
+  if (!s) return SBP_ERR_INVALID;
+  return check_err(s);
+}
+
+int sbp_system_profile(sbp_system s, double dt, int64_t nsteps, double* ms_total, double* ms_stage_avg) {
+  if (!s || nsteps <= 0 || !ms_total || !ms_stage_avg) return SBP_ERR_INVALID;
+  double gamma;
+  if (!gamma_get(gamma)) return SBP_ERR_GAMMA;
+  SBP_TRY_HIP(hipSetDevice(s->ctx->device));
+  SBP_TRY(rebuild(s));
+  hipStream_t st = s->ctx->stream;
+  hipEvent_t e0, e1;
+  SBP_TRY_HIP(hipEventCreate(&e0));
+  SBP_TRY_HIP(hipEventCreate(&e1));
+  // graphs are captured before the timed region
+  const int G = steps_per_graph(s);
+  if (nsteps >= G) SBP_TRY(ensure_graph(s, dt, G));
+  SBP_TRY_HIP(hipStreamSynchronize(st));
+  SBP_TRY_HIP(hipEventRecord(e0, st));
+  SBP_TRY(advance_impl(s, dt, nsteps));
+  SBP_TRY_HIP(hipEventRecord(e1, st));
+  SBP_TRY_HIP(hipEventSynchronize(e1));
+  float ms = 0.f;
+  SBP_TRY_HIP(hipEventElapsedTime(&ms, e0, e1));
+  *ms_total = ms;
+  // per-launch durations of the stage kernels: one more step with every stage
+  // launch bracketed by its own events
+  static const double A_next[3] = {0.5, 0.5, 1.0};
+  static const double Bw[4] = {1.0 / 6.0, 1.0 / 3.0, 1.0 / 3.0, 1.0 / 6.0};
+  static const double C[3] = {0.5, 0.5, 1.0};
+  std::vector<hipEvent_t> ev;
+  double sum = 0.0;
+  int nl = 0;
+  for (int stage = 0; stage < 4; ++stage)
+    for (const Group& g : s->groups) {
+      hipEvent_t a0, a1;
+      SBP_TRY_HIP(hipEventCreate(&a0));
+      SBP_TRY_HIP(hipEventCreate(&a1));
+      StageArgs a{};
+      a.blocks = s->d_blocks;
+      a.tiles = s->d_tiles + g.tile_offset;
+      a.time = s->d_time;
+      a.error = s->d_err;
+      a.dt = dt;
+      a.gamma = gamma;
+      a.rk_scale_acc = Bw[stage] * dt;
+      a.rk_scale_next = stage < 3 ? A_next[stage] * dt : 0.0;
+      a.time_offset = stage == 0 ? 0.0 : dt * C[stage - 1];
+      SBP_TRY_HIP(hipEventRecord(a0, st));
+      SBP_TRY_HIP(launch_stage(g.op_eta, g.op_xi, g.upwind, g.shape, stage, a, g.ntiles, st));
+      SBP_TRY_HIP(hipEventRecord(a1, st));
+      ev.push_back(a0);
+      ev.push_back(a1);
+    }
+  SBP_TRY_HIP(launch_advance_time(s->d_time, dt, st));
+  SBP_TRY_HIP(hipStreamSynchronize(st));
+  for (size_t i = 0; i < ev.size(); i += 2) {
+    float m = 0.f;
+    SBP_TRY_HIP(hipEventElapsedTime(&m, ev[i], ev[i + 1]));
+    sum += m;
+    ++nl;
+    hipEventDestroy(ev[i]);
+    hipEventDestroy(ev[i + 1]);
+  }
+  *ms_stage_avg = nl ? sum / nl : 0.0;
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  return check_err(s);
+}
+
+int sbp_system_tile_shape(sbp_system s, int* tx, int* ty, int* ry) {
+  if (!s) return SBP_ERR_INVALID;
+  SBP_TRY(rebuild(s));
+  const TileShape& t = kTileShapes[s->groups.empty() ? 0 : s->groups[0].shape];
+  if (tx) *tx = t.tx;
+  if (ty) *ty = t.ty;
+  if (ry) *ry = t.ry;
+  return SBP_OK;
+}
+
+int sbp_system_set_tile_shape(sbp_system s, int shape_index) {
+  if (!s || shape_index < -1 || shape_index >= kNumTileShapes) return SBP_ERR_INVALID;
+  s->forced_shape = shape_index;
+  s->dirty = true;
+  return SBP_OK;
+}
+
+int sbp_tile_shape_count(void) { return kNumTileShapes; }
+
+int sbp_tile_shape_get(int index, int* tx, int* ty, int* ry) {
+  if (index < 0 || index >= kNumTileShapes) return SBP_ERR_INVALID;
+  if (tx) *tx = kTileShapes[index].tx;
+  if (ty) *ty = kTileShapes[index].ty;
+  if (ry) *ry = kTileShapes[index].ry;
+  return SBP_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,51 @@
+// Instantiations of the fused stage kernel for one ξ-operator (SBP_OPX, set
+// by the build) and every η-operator, RHS kind and tile shape.
+#include <hip/hip_runtime.h>
+
+#include "sbp_kernels.hpp"
+#include "sbp_stage.hpp"
+
+#ifndef SBP_OPX
+#error "SBP_OPX (operator type) and SBP_OPX_ID must be defined by the build"
+#endif
+
+#define SBP_CAT2(a, b) a##b
+#define SBP_CAT(a, b) SBP_CAT2(a, b)
+
+namespace sbp {
+
+template <class OX, class OY, bool UP>
+static hipError_t launch_shape(int shape, int mode, const StageArgs& args, int ntiles, hipStream_t stream) {
+  switch (shape) {
+    case 0: stage_kernel<OX, OY, UP, 32, 32, 4><<<ntiles, StageGeom<OX, OY, 32, 32, 4>::NT, 0, stream>>>(args, mode); break;
+    case 1: stage_kernel<OX, OY, UP, 64, 16, 4><<<ntiles, StageGeom<OX, OY, 64, 16, 4>::NT, 0, stream>>>(args, mode); break;
+    case 2: stage_kernel<OX, OY, UP, 32, 16, 2><<<ntiles, StageGeom<OX, OY, 32, 16, 2>::NT, 0, stream>>>(args, mode); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+template <class OX, class OY>
+static hipError_t launch_pair(bool up, int shape, int mode, const StageArgs& args, int ntiles, hipStream_t stream) {
+  if (up) {
+    if constexpr (OX::is_upwind && OY::is_upwind) return launch_shape<OX, OY, true>(shape, mode, args, ntiles, stream);
+    return hipErrorInvalidValue;
+  }
+  return launch_shape<OX, OY, false>(shape, mode, args, ntiles, stream);
+}
+
+hipError_t SBP_CAT(launch_stage_opxi_, SBP_OPX_ID)(int op_eta, bool up, int shape, int mode, const StageArgs& args, int ntiles,
+                                                   hipStream_t stream) {
+  using OX = ops::SBP_OPX;
+  switch (op_eta) {
+    case UPWIND4: return launch_pair<OX, ops::Upwind4>(up, shape, mode, args, ntiles, stream);
+    case UPWIND4H2: return launch_pair<OX, ops::Upwind4h2>(up, shape, mode, args, ntiles, stream);
+    case UPWIND9: return launch_pair<OX, ops::Upwind9>(up, shape, mode, args, ntiles, stream);
+    case UPWIND9H2: return launch_pair<OX, ops::Upwind9h2>(up, shape, mode, args, ntiles, stream);
+    case SBP4: return launch_pair<OX, ops::SBP4>(up, shape, mode, args, ntiles, stream);
+    case SBP8: return launch_pair<OX, ops::SBP8>(up, shape, mode, args, ntiles, stream);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+}  // namespace sbp

@@ -1,0 +1,61 @@
+"""Multi-block systems: multigrid's `System` (multigrid/src/system.rs:368-454)
+over the fused HIP path. All blocks of one operator pair advance together in
+one stage launch per RK4 stage; interface SATs read the neighbour's edge
+straight from its stage-input buffer.
+
+    sys = System(grids=[(x0, y0), (x1, y1)], operators=[Upwind4(), Upwind4()])
+    sys.set_boundary(0, NORTH, Grid(1)); sys.set_boundary(1, SOUTH, Grid(0)) ...
+    sys.vortex(0.0, params); sys.set_dt(sys.max_dt()); sys.advance(100)
+"""
+from dataclasses import dataclass
+
+from . import _lib
+from .euler import BlockSystem, VortexParameters
+
+NORTH, SOUTH, EAST, WEST = _lib.NORTH, _lib.SOUTH, _lib.EAST, _lib.WEST
+
+
+class This:
+    """BoundaryCharacteristic::This (euler/src/lib.rs:738): own opposite edge."""
+
+
+@dataclass
+class Grid:
+    """BoundaryCharacteristic::Grid(j) (euler/src/lib.rs:739)."""
+    block: int
+
+
+@dataclass
+class Vortex:
+    """BoundaryCharacteristic::Vortex(params) (euler/src/lib.rs:740)."""
+    params: VortexParameters
+
+
+class System(BlockSystem):
+    def __init__(self, grids, operators):
+        super().__init__(grids, operators)
+        self.dt = float("nan")
+
+    def set_boundary(self, block, direction, bc):
+        if isinstance(bc, This) or bc is This:
+            self.set_bc(block, direction, _lib.BC_THIS)
+        elif isinstance(bc, Grid):
+            self.set_bc(block, direction, _lib.BC_GRID, bc.block)
+        elif isinstance(bc, Vortex):
+            self.set_bc(block, direction, _lib.BC_VORTEX, 0, bc.params)
+        else:
+            raise TypeError(f"unsupported boundary condition {bc!r}")
+
+    def set_dt(self, dt):
+        self.dt = dt
+
+    def advance(self, nsteps):
+        """SingleThreadedSystem::advance (system.rs:491-498)."""
+        self.advance_steps(self.dt, nsteps)
+
+    def error(self, params):
+        """System::error for vortex initial conditions (system.rs:618-635)."""
+        return self.error_vortex(params)
+
+    def fields(self):
+        return [self.get_state(i) for i in range(len(self.shapes))]

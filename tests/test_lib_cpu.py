@@ -1,0 +1,80 @@
+"""CPU checks of the C-ABI library: it loads, exports every function that
+include/sbpgpu.h declares, and its host-side tables and traits agree with the
+oracle. No compute call is made here (no GPU in this process)."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def header_functions():
+    text = open(os.path.join(ROOT, "include", "sbpgpu.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"^\s*(?:const\s+)?\w+\**\s+\**(sbp_\w+)\s*\(", text, flags=re.M)))
+
+
+def test_library_exports_header():
+    from summationbyparts_amd import _lib
+    L = _lib.lib()
+    names = header_functions()
+    assert len(names) >= 35
+    for n in names:
+        assert hasattr(L, n), n
+    assert sorted(_lib.EXPORTED) == names
+
+
+@pytest.mark.parametrize("name,op", list(O.OPERATORS.items()))
+def test_tables_match_oracle(name, op):
+    from summationbyparts_amd import operators
+    g = operators.BY_NAME[name]()
+    info = O.op_info(op)
+    assert (g.M, g.N, g.D, g.is_h2(), g.upwind() is not None) == (info["M"], info["N"], info["D"], info["is_h2"],
+                                                                 info["is_upwind"])
+    assert np.array_equal(g.h(), O.op_h(op))
+    kinds = [O.DIFF] + ([O.DISS] if info["is_upwind"] else [])
+    for kind in kinds:
+        for a, b in zip(g.coefficients(kind), O.op_block(op, kind)):
+            assert np.array_equal(a, b)
+
+
+def test_status_strings_and_invalid_args():
+    from summationbyparts_amd import _lib
+    L = _lib.lib()
+    assert _lib.status_string(_lib.ERR_NOT_UPWIND) == "This is not an upwind operator"
+    assert _lib.status_string(_lib.ERR_GAMMA) == "GAMMA is not defined"
+    M = ctypes.c_int()
+    assert L.sbp_operator_info(17, ctypes.byref(M), None, None, None, None) == _lib.ERR_INVALID
+    buf = np.zeros(200)
+    dp = buf.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+    assert L.sbp_operator_coefficients(_lib.SBP8, _lib.DISS, dp, dp, dp) == _lib.ERR_NOT_UPWIND
+    assert L.sbp_system_advance(None, 0.1, 1) == _lib.ERR_INVALID
+    assert L.sbp_tile_shape_count() >= 1
+
+
+def test_gamma_oncecell_semantics():
+    from summationbyparts_amd import _lib
+    L = _lib.lib()
+    g = ctypes.c_double()
+    # this process may not have set gamma yet
+    st = L.sbp_get_gamma(ctypes.byref(g))
+    assert st in (_lib.OK, _lib.ERR_GAMMA)
+    assert L.sbp_set_gamma(1.4) == _lib.OK
+    assert L.sbp_set_gamma(1.4) == _lib.OK
+    assert L.sbp_set_gamma(1.3) == _lib.ERR_GAMMA_SET
+    assert L.sbp_get_gamma(ctypes.byref(g)) == _lib.OK and g.value == 1.4
+
+
+def test_product_does_not_touch_oracle():
+    """The package never imports or loads anything from oracle/."""
+    pkg = os.path.join(ROOT, "summationbyparts_amd")
+    for dirpath, _, files in os.walk(pkg):
+        for f in files:
+            if f.endswith((".py", ".cpp", ".hpp", ".hip", ".h")) or f == "Makefile":
+                text = open(os.path.join(dirpath, f), errors="ignore").read()
+                assert "oracle" not in text.lower(), os.path.join(dirpath, f)

@@ -8,6 +8,50 @@
 
 namespace sbp {
 
+// Reciprocal and square root from the hardware estimates (v_rcp_f64 /
+// v_rsq_f64, ~2^-23 relative) refined by Newton-Raphson / Goldschmidt steps
+// to within about one ulp, without the range scaling of the IEEE-exact
+// library versions. Arguments are normal, positive numbers (densities,
+// squared sound speeds); checks on them happen before use.
+__device__ __forceinline__ double frcp(double x) {
+  double r = __builtin_amdgcn_rcp(x);
+  double e = __builtin_fma(-x, r, 1.0);
+  r = __builtin_fma(r, e, r);
+  e = __builtin_fma(-x, r, 1.0);
+  return __builtin_fma(r, e, r);
+}
+
+__device__ __forceinline__ double fsqrt(double x) {
+  const double y = __builtin_amdgcn_rsq(x);
+  double g = x * y, h = 0.5 * y;
+  const double r = __builtin_fma(-g, h, 0.5);
+  g = __builtin_fma(g, r, g);
+  h = __builtin_fma(h, r, h);
+  double d = __builtin_fma(-g, g, x);
+  g = __builtin_fma(d, h, g);
+  d = __builtin_fma(-g, g, x);
+  return __builtin_fma(d, h, g);
+}
+
+// Load through a uniform base pointer and a 32-bit element offset (selects
+// the SGPR-base + VGPR-offset addressing of global loads).
+template <typename T>
+__device__ __forceinline__ T ldg(const T* base, uint32_t idx) {
+  return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + (idx << (sizeof(T) == 8 ? 3 : 2)));
+}
+template <typename T>
+__device__ __forceinline__ void stg(T* base, uint32_t idx, T v) {
+  *reinterpret_cast<T*>(reinterpret_cast<char*>(base) + (idx << (sizeof(T) == 8 ? 3 : 2))) = v;
+}
+
+// The sticky device error word keeps the first error raised (the reference
+// panics at its first failed assertion): a thread with several failures at
+// one point reports the lowest bit, the check the reference makes first
+// (density before pressure).
+__device__ __forceinline__ void record_error(unsigned* word, unsigned bits) {
+  atomicCAS(word, 0u, bits & (~bits + 1u));
+}
+
 // Error bits of the sticky device error word.
 constexpr unsigned ERRBIT_RHO = 1u << SBP_ERR_RHO;
 constexpr unsigned ERRBIT_PRESSURE = 1u << SBP_ERR_PRESSURE;
@@ -94,21 +138,24 @@ __device__ __forceinline__ void tile_place(int t, int K, int n, int T, int M, in
 // upwind dissipation weight times the state (euler/src/lib.rs:619-735):
 //   û = jx u + jy v,  flux = [ρû, ρu û + jx p, ρv û + jy p, (e+p) û]
 //   α = |û| + c √(jx²+jy²),  aq = α q.
-template <bool UP, typename T>
-__device__ __forceinline__ void face_terms(T rho, T ru, T rv, T e, T jx, T jy, T gamma, T f[4], T aq[4], unsigned& err) {
-  const T ir = T(1) / rho;
-  const T u = ru * ir, v = rv * ir;
-  const T p = (gamma - T(1)) * (e - (ru * ru + rv * rv) * (T(0.5) * ir));
-  const T uh = jx * u + jy * v;
+//   c √(jx²+jy²) is taken as one square root, √(γ p/ρ · (jx²+jy²)).
+template <bool UP>
+__device__ __forceinline__ void face_terms(double rho, double ru, double rv, double e, double jx, double jy, double gamma,
+                                           double f[4], double aq[4], double& alpha, unsigned& err) {
+  const double ir = frcp(rho);
+  const double u = ru * ir, v = rv * ir;
+  const double p = (gamma - 1.0) * (e - (ru * ru + rv * rv) * (0.5 * ir));
+  const double uh = jx * u + jy * v;
   f[0] = rho * uh;
   f[1] = ru * uh + jx * p;
   f[2] = rv * uh + jy * p;
   f[3] = (e + p) * uh;
   if constexpr (UP) {
-    if (!(rho > T(0))) err |= ERRBIT_RHO;
-    if (!(p > T(0))) err |= ERRBIT_PRESSURE;
-    const T c = sqrt(gamma * p * ir);
-    const T al = fabs(uh) + c * sqrt(jx * jx + jy * jy);
+    if (!(rho > 0.0)) err |= ERRBIT_RHO;
+    if (!(p > 0.0)) err |= ERRBIT_PRESSURE;
+    const double c2g = gamma * p * ir * (jx * jx + jy * jy);
+    const double al = fabs(uh) + fsqrt(fmax(c2g, 1e-300));
+    alpha = al;
     aq[0] = al * rho;
     aq[1] = al * ru;
     aq[2] = al * rv;
@@ -211,6 +258,43 @@ __device__ __forceinline__ void vortex_point(const VortexDev& vp, double gamma, 
       q[2] += rv_ * v - rho_inf * v_inf;
       q[3] += (p / (gamma - 1.0) + rv_ * (u * u + v * v) / 2.0) - e_inf;
     }
+  }
+}
+
+__device__ __forceinline__ const double* stage_input(const BlockDev& B, int mode) {
+  switch (mode) {
+    case MODE_S0: return B.buf[BUF_YN];
+    case MODE_S1: return B.buf[BUF_YA];
+    case MODE_S2: return B.buf[BUF_YB];
+    case MODE_S3: return B.buf[BUF_YA];
+    default: return B.rhs_in;
+  }
+}
+
+// Boundary data g of edge `dir` at edge index t (boundary_extracts, lib.rs:869-915).
+__device__ __forceinline__ void edge_data(const StageArgs& A, const BlockDev& B, int dir, int t, int x, int y, int mode,
+                                          double time, double g[4], unsigned& err) {
+  const int src = B.edge_src[dir];
+  if (src == EDGE_THIS || src == EDGE_GRID) {
+    const BlockDev& S = src == EDGE_THIS ? B : A.blocks[B.edge_nb[dir]];
+    const double* in = stage_input(S, mode);
+    int64_t o;
+    switch (dir) {
+      case SBP_NORTH: o = t; break;                                  // source south row
+      case SBP_SOUTH: o = (int64_t)(S.ny - 1) * S.ld + t; break;     // source north row
+      case SBP_EAST: o = (int64_t)t * S.ld; break;                   // source west column
+      default: o = (int64_t)t * S.ld + (S.nx - 1); break;             // source east column
+    }
+#pragma unroll
+    for (int c = 0; c < 4; ++c) g[c] = in[c * S.ps + o];
+  } else if (src == EDGE_VORTEX) {
+    const int64_t o = (int64_t)y * B.ld + x;
+    g[2] = 0.0;
+    vortex_point(B.vortex[dir], A.gamma, time, B.xc[o], B.yc[o], g, err);
+  } else {
+    const double* p = B.edge_ptr[dir];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) g[c] = p[c * B.edge_cs[dir] + t];
   }
 }
 

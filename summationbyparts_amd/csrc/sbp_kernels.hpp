@@ -6,12 +6,13 @@
 
 namespace sbp {
 
-// Tile shapes of the fused stage kernel: TX points along ξ, TY along η, each
-// thread owning RY consecutive η-points of one ξ-column.
+// Configurations of the fused stage kernel: TX points along ξ, TY along η,
+// each thread owning RY consecutive η-points of one ξ-column, and the
+// variant of keeping the dissipation terms between passes.
 struct TileShape {
-  int tx, ty, ry;
+  int tx, ty, ry, var;
 };
-constexpr int kNumTileShapes = 3;
+constexpr int kNumTileShapes = 7;
 extern const TileShape kTileShapes[kNumTileShapes];
 
 // Operator application along one axis (0 = ξ, 1 = η) of a (ny, nx) array.
@@ -25,6 +26,9 @@ hipError_t launch_stage(int op_eta, int op_xi, bool upwind, int shape, int mode,
 // Launch bounds (threads per workgroup) and LDS bytes for a tile shape.
 int stage_threads(int shape);
 
+// Edge SATs of every block (args.blocks[0..nblocks)) for one stage, after
+// its stage kernels.
+hipError_t launch_sat_edges(int mode, const StageArgs& args, int nblocks, int max_perimeter, hipStream_t stream);
 // Pointwise metrics from the four derivative planes (grid.rs:110-131).
 hipError_t launch_metrics_finish(const double* dx_dxi, const double* dx_deta, const double* dy_dxi, const double* dy_deta,
                                  double* metrics, int64_t ps, int64_t ld, int ny, int nx, unsigned* err, hipStream_t stream);

@@ -231,6 +231,12 @@ void destroy_graph(sbp_system_s* s) {
 
 int ntiles_along(int n, int t) { return n <= t ? 1 : (n + t - 1) / t; }
 
+int max_perimeter(const sbp_system_s* s) {
+  int m = 0;
+  for (auto& b : s->blocks) m = std::max(m, 2 * b.nx + 2 * (b.ny - 2));
+  return m;
+}
+
 // Device descriptor for block i with the tile counts of `shape`.
 BlockDev make_dev(const sbp_system_s* s, int i, int shape) {
   const HostBlock& h = s->blocks[i];
@@ -375,7 +381,9 @@ int rebuild(sbp_system_s* s) {
         a.rk_scale_acc = 0.0;
         a.rk_scale_next = 0.0;
         a.time_offset = 0.0;
-        SBP_TRY_HIP(launch_stage(g.op_eta, g.op_xi, g.upwind, sh, MODE_S0, a, g.ntiles, st));  // warm-up
+        const hipError_t w = launch_stage(g.op_eta, g.op_xi, g.upwind, sh, MODE_S0, a, g.ntiles, st);  // warm-up
+        if (w == hipErrorNotSupported) continue;  // configuration not built for this operator pair
+        SBP_TRY_HIP(w);
         const int reps = 5;
         SBP_TRY_HIP(hipEventRecord(e0, st));
         for (int r = 0; r < reps; ++r) SBP_TRY_HIP(launch_stage(g.op_eta, g.op_xi, g.upwind, sh, MODE_S0, a, g.ntiles, st));
@@ -417,6 +425,27 @@ int check_err(sbp_system_s* s) {
   return status_from_errword(w);
 }
 
+// Arguments of stage `stage` (0..3) of an RK4 step of size dt; the tile
+// pointer is set per group by the caller.
+StageArgs stage_args(const sbp_system_s* s, double dt, int stage) {
+  static const double A_next[3] = {0.5, 0.5, 1.0};                        // Rk4::A nonzero entries
+  static const double Bw[4] = {1.0 / 6.0, 1.0 / 3.0, 1.0 / 3.0, 1.0 / 6.0};  // Rk4::B
+  static const double C[3] = {0.5, 0.5, 1.0};                             // Rk4::C
+  double gamma;
+  gamma_get(gamma);
+  StageArgs a{};
+  a.blocks = s->d_blocks;
+  a.tiles = s->d_tiles;
+  a.time = s->d_time;
+  a.error = s->d_err;
+  a.dt = dt;
+  a.gamma = gamma;
+  a.rk_scale_acc = Bw[stage] * dt;
+  a.rk_scale_next = stage < 3 ? A_next[stage] * dt : 0.0;
+  a.time_offset = stage == 0 ? 0.0 : dt * C[stage - 1];
+  return a;
+}
+
 // One fused RK4 step: four stage launches per group and the time update
 // (integrate::<Rk4>, utils/integrate/src/lib.rs:176-222).
 int enqueue_step(sbp_system_s* s, double dt, hipStream_t st) {
@@ -439,6 +468,8 @@ int enqueue_step(sbp_system_s* s, double dt, hipStream_t st) {
       a.time_offset = stage == 0 ? 0.0 : dt * C[stage - 1];
       SBP_TRY_HIP(launch_stage(g.op_eta, g.op_xi, g.upwind, g.shape, stage, a, g.ntiles, st));
     }
+    StageArgs a = stage_args(s, dt, stage);
+    SBP_TRY_HIP(launch_sat_edges(stage, a, (int)s->blocks.size(), max_perimeter(s), st));
   }
   SBP_TRY_HIP(launch_advance_time(s->d_time, dt, st));
   return SBP_OK;
@@ -714,6 +745,7 @@ int sbp_rhs(sbp_context ctx, int kind, int op_eta, int op_xi, int ny, int nx, do
   if (!ctx || !k || !y || !metrics || !boundaries) return SBP_ERR_INVALID;
   if (kind != SBP_RHS_TRAD && kind != SBP_RHS_UPWIND) return SBP_ERR_INVALID;
   SBP_TRY(check_shape(op_eta, op_xi, ny, nx));
+  if (5 * (int64_t)nx * ny * 8 >= ((int64_t)1 << 32)) return SBP_ERR_INVALID;
   OpInfo oe, ox;
   op_info(op_eta, oe);
   op_info(op_xi, ox);
@@ -768,6 +800,7 @@ int sbp_rhs(sbp_context ctx, int kind, int op_eta, int op_xi, int ny, int nx, do
   a.error = ctx->d_err;
   a.gamma = gamma;
   SBP_TRY_HIP(launch_stage(op_eta, op_xi, up, shape, MODE_RHS, a, (int)tiles.size(), st));
+  SBP_TRY_HIP(launch_sat_edges(MODE_RHS, a, 1, 2 * nx + 2 * (ny - 2), st));
   unsigned w = 0;
   SBP_TRY_HIP(hipMemcpyAsync(&w, ctx->d_err, sizeof(unsigned), hipMemcpyDeviceToHost, st));
   SBP_TRY_HIP(hipStreamSynchronize(st));
@@ -799,6 +832,8 @@ int sbp_system_create(sbp_context ctx, int nblocks, const int* ny, const int* nx
     h.op_xi = op_xi[i];
     h.ld = pitch_of(h.nx);
     h.ps = h.ld * h.ny;
+    // the stage kernels address 5 planes with 32-bit byte offsets
+    if (5 * h.ps * 8 >= ((int64_t)1 << 32)) return fail(SBP_ERR_INVALID);
     std::memset(h.bc_vortex, 0, sizeof(h.bc_vortex));
   }
   int rc = SBP_OK;
@@ -986,6 +1021,14 @@ int sbp_system_rhs(sbp_system s, double t, double* k_host) {
     a.gamma = gamma;
     SBP_TRY_HIP(launch_stage(g.op_eta, g.op_xi, g.upwind, g.shape, MODE_RHS, a, g.ntiles, st));
   }
+  {
+    StageArgs a{};
+    a.blocks = s->d_blocks;
+    a.time = s->d_time_rhs;
+    a.error = s->d_err;
+    a.gamma = gamma;
+    SBP_TRY_HIP(launch_sat_edges(MODE_RHS, a, (int)s->blocks.size(), max_perimeter(s), st));
+  }
   size_t off = 0;
   for (auto& h : s->blocks) {
     for (int c = 0; c < 4; ++c)
@@ -1136,6 +1179,10 @@ int sbp_system_profile(sbp_system s, double dt, int64_t nsteps, double* ms_total
       SBP_TRY_HIP(hipEventRecord(a1, st));
       ev.push_back(a0);
       ev.push_back(a1);
+      if (&g == &s->groups.back()) {
+        StageArgs sa = stage_args(s, dt, stage);
+        SBP_TRY_HIP(launch_sat_edges(stage, sa, (int)s->blocks.size(), max_perimeter(s), st));
+      }
     }
   SBP_TRY_HIP(launch_advance_time(s->d_time, dt, st));
   SBP_TRY_HIP(hipStreamSynchronize(st));

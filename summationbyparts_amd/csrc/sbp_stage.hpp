@@ -7,7 +7,6 @@
 //   diffxi / diffeta (DIFF)            sbp/src/operators/algos.rs:101-187, :371-519
 //   upwind dissipation + dissxi/eta    euler/src/lib.rs:619-674
 //   k = (-dE - dF + ad_xi + ad_eta)/J  euler/src/lib.rs:606-613 (trad: :507-512)
-//   SAT_characteristics                euler/src/lib.rs:1002-1252
 //   RK4 stage combination              utils/integrate/src/lib.rs:188-221
 //
 // Schedule inside a workgroup (LDS reused by both phases):
@@ -17,16 +16,22 @@
 //     the weighted state (DISS) are read back from LDS.
 //   η-phase: the same over the tile columns widened by the η halo, with
 //     (J·ηx, J·ηy), and η-stencils.
-//   finish: k = (idx_ξ·Sξ + idx_η·Sη)/J, edge SATs, RK4 combination, store.
+//   finish: k = (idx_ξ·Sξ + idx_η·Sη)/J, RK4 combination, store.
+// The characteristic SATs of the edge points run afterwards in
+// sat_edge_kernel: the stage combination is linear in k, so the SAT term
+// enters as acc += b·dt·sat and next += a·dt·sat. Keeping the 4x4
+// characteristic algebra out of this kernel keeps its register count low.
 // Threads own one ξ-column and RY consecutive η-points of the tile.
 #pragma once
 #include "sbp_common.hpp"
 
 namespace sbp {
 
-template <class OX, class OY, int TX, int TY, int RY>
+template <class OX, class OY, int TX, int TY, int RY, int VAR = 0>
 struct StageGeom {
   static constexpr int NT = TX * (TY / RY);
+  // values per point held in LDS at once: 4 (two passes) or 8 (one pass)
+  static constexpr int NV = VAR == 2 ? 8 : 4;
   static constexpr int HX = OX::H, HY = OY::H;
   static constexpr int WX = TX + 2 * HX;  // ξ-phase region width
   static constexpr int HYR = TY + 2 * HY;  // η-phase region height
@@ -35,62 +40,33 @@ struct StageGeom {
   static constexpr int NP = NPX > NPY ? NPX : NPY;
   static constexpr int ITX = (NPX + NT - 1) / NT;
   static constexpr int ITY = (NPY + NT - 1) / NT;
-  static constexpr int LDS_BYTES = 4 * NP * 8;
+  static constexpr int LDS_BYTES = NV * NP * 8;
 };
 
-__device__ __forceinline__ const double* stage_input(const BlockDev& B, int mode) {
-  switch (mode) {
-    case MODE_S0: return B.buf[BUF_YN];
-    case MODE_S1: return B.buf[BUF_YA];
-    case MODE_S2: return B.buf[BUF_YB];
-    case MODE_S3: return B.buf[BUF_YA];
-    default: return B.rhs_in;
-  }
-}
-
-// Boundary data g of edge `dir` at edge index t (boundary_extracts, lib.rs:869-915).
-__device__ __forceinline__ void edge_data(const StageArgs& A, const BlockDev& B, int dir, int t, int x, int y, int mode,
-                                          double time, double g[4], unsigned& err) {
-  const int src = B.edge_src[dir];
-  if (src == EDGE_THIS || src == EDGE_GRID) {
-    const BlockDev& S = src == EDGE_THIS ? B : A.blocks[B.edge_nb[dir]];
-    const double* in = stage_input(S, mode);
-    int64_t o;
-    switch (dir) {
-      case SBP_NORTH: o = t; break;                                  // source south row
-      case SBP_SOUTH: o = (int64_t)(S.ny - 1) * S.ld + t; break;     // source north row
-      case SBP_EAST: o = (int64_t)t * S.ld; break;                   // source west column
-      default: o = (int64_t)t * S.ld + (S.nx - 1); break;             // source east column
-    }
-#pragma unroll
-    for (int c = 0; c < 4; ++c) g[c] = in[c * S.ps + o];
-  } else if (src == EDGE_VORTEX) {
-    const int64_t o = (int64_t)y * B.ld + x;
-    g[2] = 0.0;
-    vortex_point(B.vortex[dir], A.gamma, time, B.xc[o], B.yc[o], g, err);
-  } else {
-    const double* p = B.edge_ptr[dir];
-#pragma unroll
-    for (int c = 0; c < 4; ++c) g[c] = p[c * B.edge_cs[dir] + t];
-  }
-}
-
-template <class OX, class OY, bool UP, int TX, int TY, int RY>
+// Variants (autotuned): VAR 0 keeps the dissipation terms α·q of a thread's
+// region points in registers between the flux and dissipation passes; VAR 1
+// keeps only α and re-reads q (cache hits); VAR 2 stores flux and α·q
+// together (8 values per point in LDS, one pass per direction).
+template <class OX, class OY, bool UP, int TX, int TY, int RY, int VAR>
 __global__ __launch_bounds__(TX * (TY / RY)) void stage_kernel(const StageArgs A, const int mode) {
-  using G = StageGeom<OX, OY, TX, TY, RY>;
+  using G = StageGeom<OX, OY, TX, TY, RY, VAR>;
   constexpr int NT = G::NT, HX = G::HX, HY = G::HY, WX = G::WX, NP = G::NP;
-  __shared__ double lds[4 * NP];
+  constexpr int NV = UP ? G::NV : 4;
+  constexpr int NC = VAR == 0 ? 4 : 1;  // cached values per region point between passes
+  __shared__ double lds[NV * NP];
 
   const TileDev T = A.tiles[blockIdx.x];
   const BlockDev& B = A.blocks[T.block];
   const int nx = B.nx, ny = B.ny, ld = B.ld;
-  const int64_t ps = B.ps;
+  // 32-bit element offsets: the host keeps 5 planes of a block below 2^29 elements
+  const uint32_t ps = (uint32_t)B.ps;
   int x0, wx0, wx1, y0, wy0, wy1;
   tile_place(T.tx, B.ntx, nx, TX, OX::M, x0, wx0, wx1);
   tile_place(T.ty, B.nty, ny, TY, OY::M, y0, wy0, wy1);
   const double* __restrict__ in = stage_input(B, mode);
   const double* __restrict__ met = B.metrics;
   const double gamma = A.gamma;
+  const double idx_xi = B.idx_xi, idx_eta = B.idx_eta;
   unsigned err = 0;
 
   const int tid = threadIdx.x;
@@ -110,19 +86,30 @@ __global__ __launch_bounds__(TX * (TY / RY)) void stage_kernel(const StageArgs A
     const int rx0 = max(0, x0 - HX), rx1 = min(nx, x0 + TX + HX);
     const int xw = rx1 - rx0;
     const bool interior = x0 >= OX::M && x0 + TX <= nx - OX::M;
-    double aq[G::ITX][4];
+    double cache[G::ITX][NC];
 #pragma unroll
     for (int it = 0; it < G::ITX; ++it) {
       const int l = tid + it * NT;
       const int r = l / WX, c = l - r * WX;
       const int gy = y0 + r;
       if (l < G::NPX && c < xw && gy < yend) {
-        const int64_t o = (int64_t)gy * ld + (rx0 + c);
-        double f[4];
-        face_terms<UP>(in[o], in[ps + o], in[2 * ps + o], in[3 * ps + o], met[ps + o], met[2 * ps + o], gamma, f, aq[it],
-                       err);
+        const uint32_t o = (uint32_t)(gy * ld + rx0 + c);
+        double f[4], aq[4], al;
+        face_terms<UP>(ldg(in, o), ldg(in, ps + o), ldg(in, 2 * ps + o), ldg(in, 3 * ps + o), ldg(met, ps + o),
+                       ldg(met, 2 * ps + o), gamma, f, aq, al, err);
 #pragma unroll
         for (int q = 0; q < 4; ++q) lds[q * NP + l] = f[q];
+        if constexpr (UP) {
+          if constexpr (VAR == 0) {
+#pragma unroll
+            for (int q = 0; q < 4; ++q) cache[it][q] = aq[q];
+          } else if constexpr (VAR == 1) {
+            cache[it][0] = al;
+          } else {
+#pragma unroll
+            for (int q = 0; q < 4; ++q) lds[(4 + q) * NP + l] = aq[q];
+          }
+        }
       }
     }
     __syncthreads();
@@ -139,24 +126,34 @@ __global__ __launch_bounds__(TX * (TY / RY)) void stage_kernel(const StageArgs A
     }
     if constexpr (UP) {
       using BS = Blk<OX, SBP_DISS>;
-      __syncthreads();
+      constexpr int QB = VAR == 2 ? 4 : 0;  // LDS slot of α·q
+      if constexpr (VAR != 2) {
+        __syncthreads();
 #pragma unroll
-      for (int it = 0; it < G::ITX; ++it) {
-        const int l = tid + it * NT;
-        const int r = l / WX, c = l - r * WX;
-        if (l < G::NPX && c < xw && y0 + r < yend) {
+        for (int it = 0; it < G::ITX; ++it) {
+          const int l = tid + it * NT;
+          const int r = l / WX, c = l - r * WX;
+          const int gy = y0 + r;
+          if (l < G::NPX && c < xw && gy < yend) {
+            if constexpr (VAR == 0) {
 #pragma unroll
-          for (int q = 0; q < 4; ++q) lds[q * NP + l] = aq[it][q];
+              for (int q = 0; q < 4; ++q) lds[q * NP + l] = cache[it][q];
+            } else {
+              const uint32_t o = (uint32_t)(gy * ld + rx0 + c);
+#pragma unroll
+              for (int q = 0; q < 4; ++q) lds[q * NP + l] = cache[it][0] * ldg(in, q * ps + o);
+            }
+          }
         }
+        __syncthreads();
       }
-      __syncthreads();
 #pragma unroll
       for (int s = 0; s < RY; ++s) {
         const int yl = trow * RY + s;
         if (xin && yb + s >= wy0 && yb + s < wy1) {
 #pragma unroll
           for (int q = 0; q < 4; ++q) {
-            const double* line = lds + q * NP + yl * WX;
+            const double* line = lds + (QB + q) * NP + yl * WX;
             acc[s][q] += interior ? interior_sum<BS>(line + (x - rx0), 1) : row_sum<BS>(line, 1, x, nx, rx0);
           }
         }
@@ -165,7 +162,7 @@ __global__ __launch_bounds__(TX * (TY / RY)) void stage_kernel(const StageArgs A
 #pragma unroll
     for (int s = 0; s < RY; ++s)
 #pragma unroll
-      for (int q = 0; q < 4; ++q) acc[s][q] *= B.idx_xi;
+      for (int q = 0; q < 4; ++q) acc[s][q] *= idx_xi;
     __syncthreads();
   }
 
@@ -176,18 +173,29 @@ __global__ __launch_bounds__(TX * (TY / RY)) void stage_kernel(const StageArgs A
     const int yh = ry1 - ry0;
     const int xend = min(x0 + TX, nx);
     const bool interior = y0 >= OY::M && y0 + TY <= ny - OY::M;
-    double aq[G::ITY][4];
+    double cache[G::ITY][NC];
 #pragma unroll
     for (int it = 0; it < G::ITY; ++it) {
       const int l = tid + it * NT;
       const int r = l / TX, c = l - r * TX;
       if (l < G::NPY && r < yh && x0 + c < xend) {
-        const int64_t o = (int64_t)(ry0 + r) * ld + (x0 + c);
-        double f[4];
-        face_terms<UP>(in[o], in[ps + o], in[2 * ps + o], in[3 * ps + o], met[3 * ps + o], met[4 * ps + o], gamma, f, aq[it],
-                       err);
+        const uint32_t o = (uint32_t)((ry0 + r) * ld + x0 + c);
+        double f[4], aq[4], al;
+        face_terms<UP>(ldg(in, o), ldg(in, ps + o), ldg(in, 2 * ps + o), ldg(in, 3 * ps + o), ldg(met, 3 * ps + o),
+                       ldg(met, 4 * ps + o), gamma, f, aq, al, err);
 #pragma unroll
         for (int q = 0; q < 4; ++q) lds[q * NP + l] = f[q];
+        if constexpr (UP) {
+          if constexpr (VAR == 0) {
+#pragma unroll
+            for (int q = 0; q < 4; ++q) cache[it][q] = aq[q];
+          } else if constexpr (VAR == 1) {
+            cache[it][0] = al;
+          } else {
+#pragma unroll
+            for (int q = 0; q < 4; ++q) lds[(4 + q) * NP + l] = aq[q];
+          }
+        }
       }
     }
     __syncthreads();
@@ -205,24 +213,33 @@ __global__ __launch_bounds__(TX * (TY / RY)) void stage_kernel(const StageArgs A
     }
     if constexpr (UP) {
       using BS = Blk<OY, SBP_DISS>;
-      __syncthreads();
+      constexpr int QB = VAR == 2 ? 4 : 0;
+      if constexpr (VAR != 2) {
+        __syncthreads();
 #pragma unroll
-      for (int it = 0; it < G::ITY; ++it) {
-        const int l = tid + it * NT;
-        const int r = l / TX, c = l - r * TX;
-        if (l < G::NPY && r < yh && x0 + c < xend) {
+        for (int it = 0; it < G::ITY; ++it) {
+          const int l = tid + it * NT;
+          const int r = l / TX, c = l - r * TX;
+          if (l < G::NPY && r < yh && x0 + c < xend) {
+            if constexpr (VAR == 0) {
 #pragma unroll
-          for (int q = 0; q < 4; ++q) lds[q * NP + l] = aq[it][q];
+              for (int q = 0; q < 4; ++q) lds[q * NP + l] = cache[it][q];
+            } else {
+              const uint32_t o = (uint32_t)((ry0 + r) * ld + x0 + c);
+#pragma unroll
+              for (int q = 0; q < 4; ++q) lds[q * NP + l] = cache[it][0] * ldg(in, q * ps + o);
+            }
+          }
         }
+        __syncthreads();
       }
-      __syncthreads();
 #pragma unroll
       for (int s = 0; s < RY; ++s) {
         const int y = yb + s;
         if (xin && y >= wy0 && y < wy1) {
 #pragma unroll
           for (int q = 0; q < 4; ++q) {
-            const double* col = lds + q * NP + tcol;
+            const double* col = lds + (QB + q) * NP + tcol;
             sy[s][q] += interior ? interior_sum<BS>(col + (y - ry0) * TX, TX) : row_sum<BS>(col, TX, y, ny, ry0);
           }
         }
@@ -231,71 +248,51 @@ __global__ __launch_bounds__(TX * (TY / RY)) void stage_kernel(const StageArgs A
 #pragma unroll
     for (int s = 0; s < RY; ++s)
 #pragma unroll
-      for (int q = 0; q < 4; ++q) acc[s][q] += B.idx_eta * sy[s][q];
+      for (int q = 0; q < 4; ++q) acc[s][q] += idx_eta * sy[s][q];
   }
 
   // ------------------------------------------------------------ finish
-  const double time = (A.time ? *A.time : 0.0) + A.time_offset;
+  // (the edge SATs follow in sat_edge_kernel, added through the same linear
+  // stage combination)
 #pragma unroll
   for (int s = 0; s < RY; ++s) {
     const int y = yb + s;
     if (!(xin && y >= wy0 && y < wy1)) continue;
-    const int64_t o = (int64_t)y * ld + x;
-    const double ij = 1.0 / met[o];
+    const uint32_t o = (uint32_t)(y * ld + x);
+    const double ij = frcp(ldg(met, o));
     double k[4];
 #pragma unroll
     for (int q = 0; q < 4; ++q) k[q] = acc[s][q] * ij;
-    if (x == 0 || x == nx - 1 || y == 0 || y == ny - 1) {
-      double qv[4], g[4];
-#pragma unroll
-      for (int q = 0; q < 4; ++q) qv[q] = in[q * ps + o];
-      if (y == ny - 1) {
-        edge_data(A, B, SBP_NORTH, x, x, y, mode, time, g, err);
-        sat_point(k, qv, g, B.hi_eta, -1.0, 1.0, met[o], met[3 * ps + o], met[4 * ps + o], gamma);
-      }
-      if (y == 0) {
-        edge_data(A, B, SBP_SOUTH, x, x, y, mode, time, g, err);
-        sat_point(k, qv, g, B.hi_eta, 1.0, -1.0, met[o], met[3 * ps + o], met[4 * ps + o], gamma);
-      }
-      if (x == nx - 1) {
-        edge_data(A, B, SBP_EAST, y, x, y, mode, time, g, err);
-        sat_point(k, qv, g, B.hi_xi, -1.0, 1.0, met[o], met[ps + o], met[2 * ps + o], gamma);
-      }
-      if (x == 0) {
-        edge_data(A, B, SBP_WEST, y, x, y, mode, time, g, err);
-        sat_point(k, qv, g, B.hi_xi, 1.0, -1.0, met[o], met[ps + o], met[2 * ps + o], gamma);
-      }
-    }
     if (mode == MODE_RHS) {
 #pragma unroll
-      for (int q = 0; q < 4; ++q) B.rhs_out[q * ps + o] = k[q];
+      for (int q = 0; q < 4; ++q) stg(B.rhs_out, q * ps + o, k[q]);
     } else if (mode == MODE_S0) {
       double* accb = B.buf[BUF_ACC];
       double* nxt = B.buf[BUF_YA];
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
-        const double yn = in[q * ps + o];
-        accb[q * ps + o] = yn + A.rk_scale_acc * k[q];
-        nxt[q * ps + o] = yn + A.rk_scale_next * k[q];
+        const double yn = ldg(in, q * ps + o);
+        stg(accb, q * ps + o, yn + A.rk_scale_acc * k[q]);
+        stg(nxt, q * ps + o, yn + A.rk_scale_next * k[q]);
       }
     } else if (mode == MODE_S3) {
       const double* accb = B.buf[BUF_ACC];
       double* out = B.buf[BUF_YN];
 #pragma unroll
-      for (int q = 0; q < 4; ++q) out[q * ps + o] = accb[q * ps + o] + A.rk_scale_acc * k[q];
+      for (int q = 0; q < 4; ++q) stg(out, q * ps + o, ldg(accb, q * ps + o) + A.rk_scale_acc * k[q]);
     } else {
       const double* ynb = B.buf[BUF_YN];
       double* accb = B.buf[BUF_ACC];
       double* nxt = B.buf[mode == MODE_S1 ? BUF_YB : BUF_YA];
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
-        const double yn = ynb[q * ps + o];
-        accb[q * ps + o] = accb[q * ps + o] + A.rk_scale_acc * k[q];
-        nxt[q * ps + o] = yn + A.rk_scale_next * k[q];
+        const double yn = ldg(ynb, q * ps + o);
+        stg(accb, q * ps + o, ldg((const double*)accb, q * ps + o) + A.rk_scale_acc * k[q]);
+        stg(nxt, q * ps + o, yn + A.rk_scale_next * k[q]);
       }
     }
   }
-  if (err) atomicOr(A.error, err);
+  if (err) record_error(A.error, err);
 }
 
 }  // namespace sbp

@@ -2,6 +2,8 @@
 // by the build) and every η-operator, RHS kind and tile shape.
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "sbp_kernels.hpp"
 #include "sbp_stage.hpp"
 
@@ -14,12 +16,36 @@
 
 namespace sbp {
 
+// Kernel configurations (tile TX x TY, RY points per thread, caching
+// variant); the list matches kStageConfigs in sbp_misc.hip.
+template <class OX, class OY, bool UP, int TX, int TY, int RY, int VAR>
+static void launch_one(int mode, const StageArgs& args, int ntiles, hipStream_t stream) {
+  stage_kernel<OX, OY, UP, TX, TY, RY, VAR><<<ntiles, TX * (TY / RY), 0, stream>>>(args, mode);
+}
+
 template <class OX, class OY, bool UP>
-static hipError_t launch_shape(int shape, int mode, const StageArgs& args, int ntiles, hipStream_t stream) {
-  switch (shape) {
-    case 0: stage_kernel<OX, OY, UP, 32, 32, 4><<<ntiles, StageGeom<OX, OY, 32, 32, 4>::NT, 0, stream>>>(args, mode); break;
-    case 1: stage_kernel<OX, OY, UP, 64, 16, 4><<<ntiles, StageGeom<OX, OY, 64, 16, 4>::NT, 0, stream>>>(args, mode); break;
-    case 2: stage_kernel<OX, OY, UP, 32, 16, 2><<<ntiles, StageGeom<OX, OY, 32, 16, 2>::NT, 0, stream>>>(args, mode); break;
+static hipError_t launch_shape(int cfg, int mode, const StageArgs& args, int ntiles, hipStream_t stream) {
+  constexpr bool full = std::is_same<OX, OY>::value;  // every configuration only for equal operator pairs
+  switch (cfg) {
+    case 0: launch_one<OX, OY, UP, 32, 16, 2, 1>(mode, args, ntiles, stream); break;
+    case 1:
+      if constexpr (full) { launch_one<OX, OY, UP, 32, 16, 2, 0>(mode, args, ntiles, stream); break; }
+      return hipErrorNotSupported;
+    case 2:
+      if constexpr (full) { launch_one<OX, OY, UP, 32, 16, 2, 2>(mode, args, ntiles, stream); break; }
+      return hipErrorNotSupported;
+    case 3:
+      if constexpr (full) { launch_one<OX, OY, UP, 32, 32, 2, 1>(mode, args, ntiles, stream); break; }
+      return hipErrorNotSupported;
+    case 4:
+      if constexpr (full) { launch_one<OX, OY, UP, 32, 32, 4, 1>(mode, args, ntiles, stream); break; }
+      return hipErrorNotSupported;
+    case 5:
+      if constexpr (full) { launch_one<OX, OY, UP, 64, 16, 4, 1>(mode, args, ntiles, stream); break; }
+      return hipErrorNotSupported;
+    case 6:
+      if constexpr (full) { launch_one<OX, OY, UP, 64, 16, 2, 1>(mode, args, ntiles, stream); break; }
+      return hipErrorNotSupported;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();

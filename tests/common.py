@@ -51,3 +51,12 @@ def rel_maxnorm(a, b):
     a = np.asarray(a)
     b = np.asarray(b)
     return max(float(np.max(np.abs(a[c] - b[c])) / max(np.max(np.abs(b[c])), 1e-300)) for c in range(a.shape[0]))
+
+
+def rel_global(a, b):
+    """max|a-b| / max|b| over all components. Used for right-hand sides,
+    whose components cancel to very different magnitudes (the density
+    component of a trad RHS is a difference of O(idx) fluxes)."""
+    a = np.asarray(a)
+    b = np.asarray(b)
+    return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))

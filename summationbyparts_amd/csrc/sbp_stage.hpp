@@ -87,16 +87,30 @@ __global__ __launch_bounds__(TX * (TY / RY)) void stage_kernel(const StageArgs A
     const int xw = rx1 - rx0;
     const bool interior = x0 >= OX::M && x0 + TX <= nx - OX::M;
     double cache[G::ITX][NC];
+    // all loads of the phase first (invalid slots read a clamped in-bounds
+    // point), so their latencies overlap; then the arithmetic
+    double v[G::ITX][6];
+#pragma unroll
+    for (int it = 0; it < G::ITX; ++it) {
+      const int l = tid + it * NT;
+      const int r = l / WX, c = l - r * WX;
+      const bool ok = l < G::NPX && c < xw && y0 + r < yend;
+      const uint32_t o = ok ? (uint32_t)((y0 + r) * ld + rx0 + c) : (uint32_t)(y0 * ld + rx0);
+      v[it][0] = ldg(in, o);
+      v[it][1] = ldg(in, ps + o);
+      v[it][2] = ldg(in, 2 * ps + o);
+      v[it][3] = ldg(in, 3 * ps + o);
+      v[it][4] = ldg(met, ps + o);
+      v[it][5] = ldg(met, 2 * ps + o);
+    }
 #pragma unroll
     for (int it = 0; it < G::ITX; ++it) {
       const int l = tid + it * NT;
       const int r = l / WX, c = l - r * WX;
       const int gy = y0 + r;
       if (l < G::NPX && c < xw && gy < yend) {
-        const uint32_t o = (uint32_t)(gy * ld + rx0 + c);
         double f[4], aq[4], al;
-        face_terms<UP>(ldg(in, o), ldg(in, ps + o), ldg(in, 2 * ps + o), ldg(in, 3 * ps + o), ldg(met, ps + o),
-                       ldg(met, 2 * ps + o), gamma, f, aq, al, err);
+        face_terms<UP>(v[it][0], v[it][1], v[it][2], v[it][3], v[it][4], v[it][5], gamma, f, aq, al, err);
 #pragma unroll
         for (int q = 0; q < 4; ++q) lds[q * NP + l] = f[q];
         if constexpr (UP) {
@@ -174,15 +188,27 @@ __global__ __launch_bounds__(TX * (TY / RY)) void stage_kernel(const StageArgs A
     const int xend = min(x0 + TX, nx);
     const bool interior = y0 >= OY::M && y0 + TY <= ny - OY::M;
     double cache[G::ITY][NC];
+    double v[G::ITY][6];
+#pragma unroll
+    for (int it = 0; it < G::ITY; ++it) {
+      const int l = tid + it * NT;
+      const int r = l / TX, c = l - r * TX;
+      const bool ok = l < G::NPY && r < yh && x0 + c < xend;
+      const uint32_t o = ok ? (uint32_t)((ry0 + r) * ld + x0 + c) : (uint32_t)(ry0 * ld + x0);
+      v[it][0] = ldg(in, o);
+      v[it][1] = ldg(in, ps + o);
+      v[it][2] = ldg(in, 2 * ps + o);
+      v[it][3] = ldg(in, 3 * ps + o);
+      v[it][4] = ldg(met, 3 * ps + o);
+      v[it][5] = ldg(met, 4 * ps + o);
+    }
 #pragma unroll
     for (int it = 0; it < G::ITY; ++it) {
       const int l = tid + it * NT;
       const int r = l / TX, c = l - r * TX;
       if (l < G::NPY && r < yh && x0 + c < xend) {
-        const uint32_t o = (uint32_t)((ry0 + r) * ld + x0 + c);
         double f[4], aq[4], al;
-        face_terms<UP>(ldg(in, o), ldg(in, ps + o), ldg(in, 2 * ps + o), ldg(in, 3 * ps + o), ldg(met, 3 * ps + o),
-                       ldg(met, 4 * ps + o), gamma, f, aq, al, err);
+        face_terms<UP>(v[it][0], v[it][1], v[it][2], v[it][3], v[it][4], v[it][5], gamma, f, aq, al, err);
 #pragma unroll
         for (int q = 0; q < 4; ++q) lds[q * NP + l] = f[q];
         if constexpr (UP) {

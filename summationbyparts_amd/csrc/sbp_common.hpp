@@ -43,6 +43,20 @@ template <typename T>
 __device__ __forceinline__ void stg(T* base, uint32_t idx, T v) {
   *reinterpret_cast<T*>(reinterpret_cast<char*>(base) + (idx << (sizeof(T) == 8 ? 3 : 2))) = v;
 }
+// Streaming store/load (non-temporal hint): for data written or read once per
+// stage, so that it neither allocates nor evicts L2 lines holding halos.
+template <typename T>
+__device__ __forceinline__ void stg_nt(T* base, uint32_t idx, T v) {
+  using GP = __attribute__((address_space(1))) T*;
+  using GC = __attribute__((address_space(1))) char*;
+  __builtin_nontemporal_store(v, (GP)((GC)base + (idx << (sizeof(T) == 8 ? 3 : 2))));
+}
+template <typename T>
+__device__ __forceinline__ T ldg_nt(const T* base, uint32_t idx) {
+  using GP = const __attribute__((address_space(1))) T*;
+  using GC = const __attribute__((address_space(1))) char*;
+  return __builtin_nontemporal_load((GP)((GC)base + (idx << (sizeof(T) == 8 ? 3 : 2))));
+}
 
 // The sticky device error word keeps the first error raised (the reference
 // panics at its first failed assertion): a thread with several failures at
@@ -92,11 +106,13 @@ __device__ __forceinline__ T interior_sum(const T* p, int stride) {
 
 // Full operator row i of an axis of length n: start block, interior or end
 // block (algos.rs:101-187). `line` points at axis index r0, stride `stride`.
+// The closure loops are kept rolled: their coefficient rows are selected per
+// lane at run time, and unrolling them would hold a whole row in registers.
 template <class B, typename T>
 __device__ __forceinline__ T row_sum(const T* line, int stride, int i, int n, int r0) {
   if (i < B::M) {
     T s = T(0);
-#pragma unroll
+#pragma unroll 1
     for (int k = 0; k < B::N; ++k) s += T(B::start(i, k)) * line[(k - r0) * stride];
     return s;
   }
@@ -104,7 +120,7 @@ __device__ __forceinline__ T row_sum(const T* line, int stride, int i, int n, in
     const int r = i - (n - B::M);
     const int b0 = n - B::N - r0;
     T s = T(0);
-#pragma unroll
+#pragma unroll 1
     for (int k = 0; k < B::N; ++k) s += T(B::end(r, k)) * line[(b0 + k) * stride];
     return s;
   }

@@ -7,12 +7,12 @@
 namespace sbp {
 
 // Configurations of the fused stage kernel: TX points along ξ, TY along η,
-// each thread owning RY consecutive η-points of one ξ-column, and the
+// each thread owning RX adjacent ξ-columns by RY consecutive η-rows, and the
 // variant of keeping the dissipation terms between passes.
 struct TileShape {
-  int tx, ty, ry, var;
+  int tx, ty, rx, ry, var;
 };
-constexpr int kNumTileShapes = 7;
+constexpr int kNumTileShapes = 9;
 extern const TileShape kTileShapes[kNumTileShapes];
 
 // Operator application along one axis (0 = ξ, 1 = η) of a (ny, nx) array.

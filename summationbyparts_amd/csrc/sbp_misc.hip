@@ -7,8 +7,10 @@
 
 namespace sbp {
 
-const TileShape kTileShapes[kNumTileShapes] = {{32, 16, 2, 1}, {32, 16, 2, 0}, {32, 16, 2, 2}, {32, 32, 2, 1},
-                                               {32, 32, 4, 1}, {64, 16, 4, 1}, {64, 16, 2, 1}};
+const TileShape kTileShapes[kNumTileShapes] = {
+    {32, 16, 1, 2, 1}, {32, 16, 1, 2, 0}, {32, 24, 1, 2, 0}, {32, 20, 1, 2, 0}, {32, 12, 1, 2, 0},
+    {64, 16, 1, 2, 0}, {32, 24, 1, 2, 1}, {32, 16, 1, 2, 3}, {32, 12, 1, 2, 3},
+};
 
 // ---------------------------------------------------------------------------
 // Operator application along one axis (SbpOperator2d::diffxi/diffeta,

@@ -6,6 +6,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <memory>
@@ -39,7 +40,10 @@ hipError_t launch_stage(int op_eta, int op_xi, bool upwind, int shape, int mode,
   }
 }
 
-int stage_threads(int shape) { return kTileShapes[shape].tx * (kTileShapes[shape].ty / kTileShapes[shape].ry); }
+int stage_threads(int shape) {
+  const TileShape& t = kTileShapes[shape];
+  return (t.tx / t.rx) * (t.ty / t.ry);
+}
 }  // namespace sbp
 
 using namespace sbp;
@@ -275,8 +279,10 @@ BlockDev make_dev(const sbp_system_s* s, int i, int shape) {
 
 // Tile list of a group ordered for the 8 XCDs of an MI355X: workgroup g runs
 // on XCD g mod 8, so the natural (block, row, column) order is cut into 8
-// contiguous chunks and interleaved; each XCD then sweeps one compact region
-// and the halo rows shared by neighbouring tiles stay in its L2.
+// contiguous chunks and interleaved; each XCD then sweeps one compact band of
+// rows (measured: 30% faster than the plain order at 4096²). Narrow vertical
+// strips per XCD raise the L2 hit rate of the η-halos from 39% to 51% but run
+// 20% slower; the band order streams whole rows.
 void build_tiles(const sbp_system_s* s, const Group& g, int shape, std::vector<TileDev>& out) {
   const TileShape& ts = kTileShapes[shape];
   std::vector<TileDev> nat;
@@ -288,7 +294,7 @@ void build_tiles(const sbp_system_s* s, const Group& g, int shape, std::vector<T
   }
   const int T = (int)nat.size();
   const int X = 8;
-  std::vector<int> start(X + 1, 0);
+  std::vector<int> start((size_t)X + 1, 0);
   for (int x = 0; x < X; ++x) start[x + 1] = start[x] + T / X + (x < T % X ? 1 : 0);
   const int maxlen = T / X + (T % X ? 1 : 0);
   for (int j = 0; j < maxlen; ++j)

@@ -21,15 +21,14 @@
 // sat_edge_kernel: the stage combination is linear in k, so the SAT term
 // enters as acc += b·dt·sat and next += a·dt·sat. Keeping the 4x4
 // characteristic algebra out of this kernel keeps its register count low.
-// Threads own one ξ-column and RY consecutive η-points of the tile.
 #pragma once
 #include "sbp_common.hpp"
 
 namespace sbp {
 
-template <class OX, class OY, int TX, int TY, int RY, int VAR = 0>
+template <class OX, class OY, int TX, int TY, int RX, int RY, int VAR = 0>
 struct StageGeom {
-  static constexpr int NT = TX * (TY / RY);
+  static constexpr int NT = (TX / RX) * (TY / RY);
   // values per point held in LDS at once: 4 (two passes) or 8 (one pass)
   static constexpr int NV = VAR == 2 ? 8 : 4;
   static constexpr int HX = OX::H, HY = OY::H;
@@ -41,19 +40,82 @@ struct StageGeom {
   static constexpr int ITX = (NPX + NT - 1) / NT;
   static constexpr int ITY = (NPY + NT - 1) / NT;
   static constexpr int LDS_BYTES = NV * NP * 8;
+  static_assert(RX == 1 || RX == 2, "RX is 1 or 2");
+  static_assert(TX % 2 == 0 && NP % 2 == 0, "16-byte LDS alignment of paired reads");
 };
+
+// Interior stencils of R consecutive outputs from one window of R + D - 1
+// values: along ξ the window is contiguous (read in 16-byte pairs when
+// R == 2), along η it is a column of pairs (two adjacent ξ-columns).
+template <class B, int R>
+__device__ __forceinline__ void row_window(const double* p, double out[R]) {
+  constexpr int W = R + B::D - 1;
+  double w[W + 1];
+  if constexpr (R == 2) {
+    const double2* p2 = reinterpret_cast<const double2*>(p);
+#pragma unroll
+    for (int k = 0; k < (W + 1) / 2; ++k) {
+      const double2 v = p2[k];
+      w[2 * k] = v.x;
+      w[2 * k + 1] = v.y;
+    }
+  } else {
+#pragma unroll
+    for (int k = 0; k < W; ++k) w[k] = p[k];
+  }
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    double s = 0.0;
+#pragma unroll
+    for (int d = 0; d < B::D; ++d)
+      if (B::diag(d) != 0.0) s += B::diag(d) * w[r + d];
+    out[r] = s;
+  }
+}
+
+template <class B, int RS, int RX>
+__device__ __forceinline__ void col_window(const double* p, int stride, double out[RS][RX]) {
+  constexpr int W = RS + B::D - 1;
+#pragma unroll
+  for (int i = 0; i < RX; ++i)
+#pragma unroll
+    for (int r = 0; r < RS; ++r) out[r][i] = 0.0;
+#pragma unroll
+  for (int k = 0; k < W; ++k) {
+    double v[RX];
+    if constexpr (RX == 2) {
+      const double2 t = *reinterpret_cast<const double2*>(p + k * stride);
+      v[0] = t.x;
+      v[1] = t.y;
+    } else {
+      v[0] = p[k * stride];
+    }
+#pragma unroll
+    for (int r = 0; r < RS; ++r) {
+      const int d = k - r;
+      if (d >= 0 && d < B::D && B::diag(d) != 0.0) {
+#pragma unroll
+        for (int i = 0; i < RX; ++i) out[r][i] += B::diag(d) * v[i];
+      }
+    }
+  }
+}
 
 // Variants (autotuned): VAR 0 keeps the dissipation terms α·q of a thread's
 // region points in registers between the flux and dissipation passes; VAR 1
 // keeps only α and re-reads q (cache hits); VAR 2 stores flux and α·q
-// together (8 values per point in LDS, one pass per direction).
-template <class OX, class OY, bool UP, int TX, int TY, int RY, int VAR>
-__global__ __launch_bounds__(TX * (TY / RY)) void stage_kernel(const StageArgs A, const int mode) {
-  using G = StageGeom<OX, OY, TX, TY, RY, VAR>;
+// together (8 values per point in LDS, one pass per direction); VAR 3 is
+// VAR 0 with the finish operands loaded before the η-phase.
+// Threads own RX adjacent ξ-columns and RY consecutive η-rows of the tile.
+template <class OX, class OY, bool UP, int TX, int TY, int RX, int RY, int VAR>
+__global__ __launch_bounds__((TX / RX) * (TY / RY)) void stage_kernel(const StageArgs A, const int mode) {
+  using G = StageGeom<OX, OY, TX, TY, RX, RY, VAR>;
   constexpr int NT = G::NT, HX = G::HX, HY = G::HY, WX = G::WX, NP = G::NP;
   constexpr int NV = UP ? G::NV : 4;
-  constexpr int NC = VAR == 0 ? 4 : 1;  // cached values per region point between passes
-  __shared__ double lds[NV * NP];
+  constexpr int NC = (VAR == 0 || VAR == 3) ? 4 : 1;  // cached values per region point between passes
+  constexpr bool PF = VAR == 3;                      // prefetch the finish operands
+  constexpr int LCH = 3;                // region points whose loads are in flight together
+  __shared__ __align__(16) double lds[NV * NP];
 
   const TileDev T = A.tiles[blockIdx.x];
   const BlockDev& B = A.blocks[T.block];
@@ -70,15 +132,24 @@ __global__ __launch_bounds__(TX * (TY / RY)) void stage_kernel(const StageArgs A
   unsigned err = 0;
 
   const int tid = threadIdx.x;
-  const int tcol = tid % TX, trow = tid / TX;
-  const int x = x0 + tcol;
-  const int yb = y0 + trow * RY;
+  const int tc = tid % (TX / RX), tr = tid / (TX / RX);
+  const int xb = x0 + tc * RX;  // first ξ-column of this thread
+  const int yb = y0 + tr * RY;  // first η-row of this thread
   const int yend = min(y0 + TY, ny);
   // Only the points this tile writes are computed: every stencil they need
   // lies inside the loaded regions (see tile_place).
-  const bool xin = x >= wx0 && x < wx1;
+  bool xin[RX], yin[RY];
+#pragma unroll
+  for (int i = 0; i < RX; ++i) xin[i] = xb + i >= wx0 && xb + i < wx1;
+#pragma unroll
+  for (int s = 0; s < RY; ++s) yin[s] = yb + s >= wy0 && yb + s < wy1;
+  bool any = false;
+#pragma unroll
+  for (int i = 0; i < RX; ++i)
+#pragma unroll
+    for (int s = 0; s < RY; ++s) any = any || (xin[i] && yin[s]);
 
-  double acc[RY][4];
+  double acc[RY][RX][4];
 
   // ------------------------------------------------------------ ξ-phase
   {
@@ -87,98 +158,123 @@ __global__ __launch_bounds__(TX * (TY / RY)) void stage_kernel(const StageArgs A
     const int xw = rx1 - rx0;
     const bool interior = x0 >= OX::M && x0 + TX <= nx - OX::M;
     double cache[G::ITX][NC];
-    // all loads of the phase first (invalid slots read a clamped in-bounds
-    // point), so their latencies overlap; then the arithmetic
-    double v[G::ITX][6];
+    // loads of up to LCH region points are issued together (invalid slots
+    // read a clamped in-bounds point) so that their latencies overlap
 #pragma unroll
-    for (int it = 0; it < G::ITX; ++it) {
-      const int l = tid + it * NT;
-      const int r = l / WX, c = l - r * WX;
-      const bool ok = l < G::NPX && c < xw && y0 + r < yend;
-      const uint32_t o = ok ? (uint32_t)((y0 + r) * ld + rx0 + c) : (uint32_t)(y0 * ld + rx0);
-      v[it][0] = ldg(in, o);
-      v[it][1] = ldg(in, ps + o);
-      v[it][2] = ldg(in, 2 * ps + o);
-      v[it][3] = ldg(in, 3 * ps + o);
-      v[it][4] = ldg(met, ps + o);
-      v[it][5] = ldg(met, 2 * ps + o);
-    }
+    for (int i0 = 0; i0 < G::ITX; i0 += LCH) {
+      double v[LCH][6];
 #pragma unroll
-    for (int it = 0; it < G::ITX; ++it) {
-      const int l = tid + it * NT;
-      const int r = l / WX, c = l - r * WX;
-      const int gy = y0 + r;
-      if (l < G::NPX && c < xw && gy < yend) {
-        double f[4], aq[4], al;
-        face_terms<UP>(v[it][0], v[it][1], v[it][2], v[it][3], v[it][4], v[it][5], gamma, f, aq, al, err);
+      for (int j = 0; j < LCH; ++j) {
+        const int l = tid + (i0 + j) * NT;
+        const int r = l / WX, c = l - r * WX;
+        const bool ok = i0 + j < G::ITX && l < G::NPX && c < xw && y0 + r < yend;
+        const uint32_t o = ok ? (uint32_t)((y0 + r) * ld + rx0 + c) : (uint32_t)(y0 * ld + rx0);
+        v[j][0] = ldg(in, o);
+        v[j][1] = ldg(in, ps + o);
+        v[j][2] = ldg(in, 2 * ps + o);
+        v[j][3] = ldg(in, 3 * ps + o);
+        v[j][4] = ldg(met, ps + o);
+        v[j][5] = ldg(met, 2 * ps + o);
+      }
 #pragma unroll
-        for (int q = 0; q < 4; ++q) lds[q * NP + l] = f[q];
-        if constexpr (UP) {
-          if constexpr (VAR == 0) {
+      for (int j = 0; j < LCH; ++j) {
+        const int it = i0 + j;
+        const int l = tid + it * NT;
+        const int r = l / WX, c = l - r * WX;
+        if (it < G::ITX && l < G::NPX && c < xw && y0 + r < yend) {
+          double f[4], aq[4], al;
+          face_terms<UP>(v[j][0], v[j][1], v[j][2], v[j][3], v[j][4], v[j][5], gamma, f, aq, al, err);
 #pragma unroll
-            for (int q = 0; q < 4; ++q) cache[it][q] = aq[q];
-          } else if constexpr (VAR == 1) {
-            cache[it][0] = al;
-          } else {
+          for (int q = 0; q < 4; ++q) lds[q * NP + l] = f[q];
+          if constexpr (UP) {
+            if constexpr (VAR == 0) {
 #pragma unroll
-            for (int q = 0; q < 4; ++q) lds[(4 + q) * NP + l] = aq[q];
+              for (int q = 0; q < 4; ++q) cache[it][q] = aq[q];
+            } else if constexpr (VAR == 1) {
+              cache[it][0] = al;
+            } else {
+#pragma unroll
+              for (int q = 0; q < 4; ++q) lds[(4 + q) * NP + l] = aq[q];
+            }
           }
         }
       }
     }
     __syncthreads();
+    // ξ-stencil of one LDS slot group into acc with sign sg
+    // acc = sg·idx_ξ·Σ (first pass) or acc += sg·idx_ξ·Σ
+    auto xi_pass = [&](auto blk, int slot, double sg, bool first) {
+      using BB = decltype(blk);
 #pragma unroll
-    for (int s = 0; s < RY; ++s) {
-      const int yl = trow * RY + s;
-      if (xin && yb + s >= wy0 && yb + s < wy1) {
+      for (int s = 0; s < RY; ++s) {
+        if (!(yin[s] && any)) continue;
+        const double* line = lds + (size_t)(tr * RY + s) * WX;
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
-          const double* line = lds + q * NP + yl * WX;
-          acc[s][q] = -(interior ? interior_sum<BD>(line + (x - rx0), 1) : row_sum<BD>(line, 1, x, nx, rx0));
+          const double* lq = line + (slot + q) * NP;
+          double o[RX];
+          if (interior) {
+            row_window<BB, RX>(lq + (xb - rx0 - HX), o);
+          } else {
+#pragma unroll
+            for (int i = 0; i < RX; ++i) o[i] = xin[i] ? row_sum<BB>(lq, 1, xb + i, nx, rx0) : 0.0;
+          }
+#pragma unroll
+          for (int i = 0; i < RX; ++i) acc[s][i][q] = first ? sg * o[i] : __builtin_fma(sg, o[i], acc[s][i][q]);
         }
       }
-    }
+    };
+    xi_pass(BD{}, 0, -idx_xi, true);
     if constexpr (UP) {
       using BS = Blk<OX, SBP_DISS>;
-      constexpr int QB = VAR == 2 ? 4 : 0;  // LDS slot of α·q
       if constexpr (VAR != 2) {
         __syncthreads();
 #pragma unroll
         for (int it = 0; it < G::ITX; ++it) {
           const int l = tid + it * NT;
           const int r = l / WX, c = l - r * WX;
-          const int gy = y0 + r;
-          if (l < G::NPX && c < xw && gy < yend) {
-            if constexpr (VAR == 0) {
+          if (l < G::NPX && c < xw && y0 + r < yend) {
+            if constexpr (NC == 4) {
 #pragma unroll
               for (int q = 0; q < 4; ++q) lds[q * NP + l] = cache[it][q];
             } else {
-              const uint32_t o = (uint32_t)(gy * ld + rx0 + c);
+              const uint32_t o = (uint32_t)((y0 + r) * ld + rx0 + c);
 #pragma unroll
               for (int q = 0; q < 4; ++q) lds[q * NP + l] = cache[it][0] * ldg(in, q * ps + o);
             }
           }
         }
         __syncthreads();
-      }
-#pragma unroll
-      for (int s = 0; s < RY; ++s) {
-        const int yl = trow * RY + s;
-        if (xin && yb + s >= wy0 && yb + s < wy1) {
-#pragma unroll
-          for (int q = 0; q < 4; ++q) {
-            const double* line = lds + (QB + q) * NP + yl * WX;
-            acc[s][q] += interior ? interior_sum<BS>(line + (x - rx0), 1) : row_sum<BS>(line, 1, x, nx, rx0);
-          }
-        }
+        xi_pass(BS{}, 0, idx_xi, false);
+      } else {
+        xi_pass(BS{}, 4, idx_xi, false);
       }
     }
+    __syncthreads();
+  }
+
+  // Finish operands at the output points (J, and y_n / accumulator as the
+  // stage combination needs them), loaded ahead of the η-phase when PF so
+  // that their latency overlaps it.
+  double fJ[RY][RX], fa[RY][RX][4], fb[RY][RX][4];
+  auto load_finish = [&]() {
 #pragma unroll
     for (int s = 0; s < RY; ++s)
 #pragma unroll
-      for (int q = 0; q < 4; ++q) acc[s][q] *= idx_xi;
-    __syncthreads();
-  }
+      for (int i = 0; i < RX; ++i) {
+        const uint32_t o = (xin[i] && yin[s]) ? (uint32_t)((yb + s) * ld + xb + i) : (uint32_t)(y0 * ld + x0);
+        fJ[s][i] = ldg(met, o);
+        if (mode == MODE_RHS) continue;
+        const double* pa = mode == MODE_S0 ? in : (mode == MODE_S3 ? B.buf[BUF_ACC] : B.buf[BUF_YN]);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) fa[s][i][q] = ldg_nt(pa, q * ps + o);
+        if (mode == MODE_S1 || mode == MODE_S2) {
+#pragma unroll
+          for (int q = 0; q < 4; ++q) fb[s][i][q] = ldg_nt((const double*)B.buf[BUF_ACC], q * ps + o);
+        }
+      }
+  };
+  if constexpr (PF) load_finish();
 
   // ------------------------------------------------------------ η-phase
   {
@@ -188,58 +284,73 @@ __global__ __launch_bounds__(TX * (TY / RY)) void stage_kernel(const StageArgs A
     const int xend = min(x0 + TX, nx);
     const bool interior = y0 >= OY::M && y0 + TY <= ny - OY::M;
     double cache[G::ITY][NC];
-    double v[G::ITY][6];
 #pragma unroll
-    for (int it = 0; it < G::ITY; ++it) {
-      const int l = tid + it * NT;
-      const int r = l / TX, c = l - r * TX;
-      const bool ok = l < G::NPY && r < yh && x0 + c < xend;
-      const uint32_t o = ok ? (uint32_t)((ry0 + r) * ld + x0 + c) : (uint32_t)(ry0 * ld + x0);
-      v[it][0] = ldg(in, o);
-      v[it][1] = ldg(in, ps + o);
-      v[it][2] = ldg(in, 2 * ps + o);
-      v[it][3] = ldg(in, 3 * ps + o);
-      v[it][4] = ldg(met, 3 * ps + o);
-      v[it][5] = ldg(met, 4 * ps + o);
-    }
+    for (int i0 = 0; i0 < G::ITY; i0 += LCH) {
+      double v[LCH][6];
 #pragma unroll
-    for (int it = 0; it < G::ITY; ++it) {
-      const int l = tid + it * NT;
-      const int r = l / TX, c = l - r * TX;
-      if (l < G::NPY && r < yh && x0 + c < xend) {
-        double f[4], aq[4], al;
-        face_terms<UP>(v[it][0], v[it][1], v[it][2], v[it][3], v[it][4], v[it][5], gamma, f, aq, al, err);
+      for (int j = 0; j < LCH; ++j) {
+        const int l = tid + (i0 + j) * NT;
+        const int r = l / TX, c = l - r * TX;
+        const bool ok = i0 + j < G::ITY && l < G::NPY && r < yh && x0 + c < xend;
+        const uint32_t o = ok ? (uint32_t)((ry0 + r) * ld + x0 + c) : (uint32_t)(ry0 * ld + x0);
+        v[j][0] = ldg(in, o);
+        v[j][1] = ldg(in, ps + o);
+        v[j][2] = ldg(in, 2 * ps + o);
+        v[j][3] = ldg(in, 3 * ps + o);
+        v[j][4] = ldg(met, 3 * ps + o);
+        v[j][5] = ldg(met, 4 * ps + o);
+      }
 #pragma unroll
-        for (int q = 0; q < 4; ++q) lds[q * NP + l] = f[q];
-        if constexpr (UP) {
-          if constexpr (VAR == 0) {
+      for (int j = 0; j < LCH; ++j) {
+        const int it = i0 + j;
+        const int l = tid + it * NT;
+        const int r = l / TX, c = l - r * TX;
+        if (it < G::ITY && l < G::NPY && r < yh && x0 + c < xend) {
+          double f[4], aq[4], al;
+          face_terms<UP>(v[j][0], v[j][1], v[j][2], v[j][3], v[j][4], v[j][5], gamma, f, aq, al, err);
 #pragma unroll
-            for (int q = 0; q < 4; ++q) cache[it][q] = aq[q];
-          } else if constexpr (VAR == 1) {
-            cache[it][0] = al;
-          } else {
+          for (int q = 0; q < 4; ++q) lds[q * NP + l] = f[q];
+          if constexpr (UP) {
+            if constexpr (VAR == 0) {
 #pragma unroll
-            for (int q = 0; q < 4; ++q) lds[(4 + q) * NP + l] = aq[q];
+              for (int q = 0; q < 4; ++q) cache[it][q] = aq[q];
+            } else if constexpr (VAR == 1) {
+              cache[it][0] = al;
+            } else {
+#pragma unroll
+              for (int q = 0; q < 4; ++q) lds[(4 + q) * NP + l] = aq[q];
+            }
           }
         }
       }
     }
     __syncthreads();
-    double sy[RY][4];
+    // acc += sg·idx_η·Σ
+    auto eta_pass = [&](auto blk, int slot, double sg) {
+      using BB = decltype(blk);
+      if (!any) return;
 #pragma unroll
-    for (int s = 0; s < RY; ++s) {
-      const int y = yb + s;
-      if (xin && y >= wy0 && y < wy1) {
+      for (int q = 0; q < 4; ++q) {
+        const double* col = lds + (slot + q) * NP + tc * RX;
+        double o[RY][RX];
+        if (interior) {
+          col_window<BB, RY, RX>(col + (yb - ry0 - HY) * TX, TX, o);
+        } else {
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
-          const double* col = lds + q * NP + tcol;
-          sy[s][q] = -(interior ? interior_sum<BD>(col + (y - ry0) * TX, TX) : row_sum<BD>(col, TX, y, ny, ry0));
+          for (int s = 0; s < RY; ++s)
+#pragma unroll
+            for (int i = 0; i < RX; ++i)
+              o[s][i] = (yin[s] && xin[i]) ? row_sum<BB>(col + i, TX, yb + s, ny, ry0) : 0.0;
         }
+#pragma unroll
+        for (int s = 0; s < RY; ++s)
+#pragma unroll
+          for (int i = 0; i < RX; ++i) acc[s][i][q] = __builtin_fma(sg, o[s][i], acc[s][i][q]);
       }
-    }
+    };
+    eta_pass(BD{}, 0, -idx_eta);
     if constexpr (UP) {
       using BS = Blk<OY, SBP_DISS>;
-      constexpr int QB = VAR == 2 ? 4 : 0;
       if constexpr (VAR != 2) {
         __syncthreads();
 #pragma unroll
@@ -247,7 +358,7 @@ __global__ __launch_bounds__(TX * (TY / RY)) void stage_kernel(const StageArgs A
           const int l = tid + it * NT;
           const int r = l / TX, c = l - r * TX;
           if (l < G::NPY && r < yh && x0 + c < xend) {
-            if constexpr (VAR == 0) {
+            if constexpr (NC == 4) {
 #pragma unroll
               for (int q = 0; q < 4; ++q) lds[q * NP + l] = cache[it][q];
             } else {
@@ -258,63 +369,49 @@ __global__ __launch_bounds__(TX * (TY / RY)) void stage_kernel(const StageArgs A
           }
         }
         __syncthreads();
-      }
-#pragma unroll
-      for (int s = 0; s < RY; ++s) {
-        const int y = yb + s;
-        if (xin && y >= wy0 && y < wy1) {
-#pragma unroll
-          for (int q = 0; q < 4; ++q) {
-            const double* col = lds + (QB + q) * NP + tcol;
-            sy[s][q] += interior ? interior_sum<BS>(col + (y - ry0) * TX, TX) : row_sum<BS>(col, TX, y, ny, ry0);
-          }
-        }
+        eta_pass(BS{}, 0, idx_eta);
+      } else {
+        eta_pass(BS{}, 4, idx_eta);
       }
     }
-#pragma unroll
-    for (int s = 0; s < RY; ++s)
-#pragma unroll
-      for (int q = 0; q < 4; ++q) acc[s][q] += idx_eta * sy[s][q];
   }
 
   // ------------------------------------------------------------ finish
   // (the edge SATs follow in sat_edge_kernel, added through the same linear
   // stage combination)
+  if constexpr (!PF) load_finish();
 #pragma unroll
-  for (int s = 0; s < RY; ++s) {
-    const int y = yb + s;
-    if (!(xin && y >= wy0 && y < wy1)) continue;
-    const uint32_t o = (uint32_t)(y * ld + x);
-    const double ij = frcp(ldg(met, o));
+  for (int s = 0; s < RY; ++s)
+#pragma unroll
+  for (int i = 0; i < RX; ++i) {
+    if (!(xin[i] && yin[s])) continue;
+    const uint32_t o = (uint32_t)((yb + s) * ld + xb + i);
+    const double ij = frcp(fJ[s][i]);
     double k[4];
 #pragma unroll
-    for (int q = 0; q < 4; ++q) k[q] = acc[s][q] * ij;
+    for (int q = 0; q < 4; ++q) k[q] = acc[s][i][q] * ij;
     if (mode == MODE_RHS) {
 #pragma unroll
-      for (int q = 0; q < 4; ++q) stg(B.rhs_out, q * ps + o, k[q]);
+      for (int q = 0; q < 4; ++q) stg_nt(B.rhs_out, q * ps + o, k[q]);
     } else if (mode == MODE_S0) {
       double* accb = B.buf[BUF_ACC];
       double* nxt = B.buf[BUF_YA];
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
-        const double yn = ldg(in, q * ps + o);
-        stg(accb, q * ps + o, yn + A.rk_scale_acc * k[q]);
-        stg(nxt, q * ps + o, yn + A.rk_scale_next * k[q]);
+        stg_nt(accb, q * ps + o, fa[s][i][q] + A.rk_scale_acc * k[q]);
+        stg_nt(nxt, q * ps + o, fa[s][i][q] + A.rk_scale_next * k[q]);
       }
     } else if (mode == MODE_S3) {
-      const double* accb = B.buf[BUF_ACC];
       double* out = B.buf[BUF_YN];
 #pragma unroll
-      for (int q = 0; q < 4; ++q) stg(out, q * ps + o, ldg(accb, q * ps + o) + A.rk_scale_acc * k[q]);
+      for (int q = 0; q < 4; ++q) stg_nt(out, q * ps + o, fa[s][i][q] + A.rk_scale_acc * k[q]);
     } else {
-      const double* ynb = B.buf[BUF_YN];
       double* accb = B.buf[BUF_ACC];
       double* nxt = B.buf[mode == MODE_S1 ? BUF_YB : BUF_YA];
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
-        const double yn = ldg(ynb, q * ps + o);
-        stg(accb, q * ps + o, ldg((const double*)accb, q * ps + o) + A.rk_scale_acc * k[q]);
-        stg(nxt, q * ps + o, yn + A.rk_scale_next * k[q]);
+        stg_nt(accb, q * ps + o, fb[s][i][q] + A.rk_scale_acc * k[q]);
+        stg_nt(nxt, q * ps + o, fa[s][i][q] + A.rk_scale_next * k[q]);
       }
     }
   }

@@ -18,34 +18,32 @@ namespace sbp {
 
 // Kernel configurations (tile TX x TY, RY points per thread, caching
 // variant); the list matches kStageConfigs in sbp_misc.hip.
-template <class OX, class OY, bool UP, int TX, int TY, int RY, int VAR>
+template <class OX, class OY, bool UP, int TX, int TY, int RX, int RY, int VAR>
 static void launch_one(int mode, const StageArgs& args, int ntiles, hipStream_t stream) {
-  stage_kernel<OX, OY, UP, TX, TY, RY, VAR><<<ntiles, TX * (TY / RY), 0, stream>>>(args, mode);
+  stage_kernel<OX, OY, UP, TX, TY, RX, RY, VAR><<<ntiles, (TX / RX) * (TY / RY), 0, stream>>>(args, mode);
 }
 
 template <class OX, class OY, bool UP>
 static hipError_t launch_shape(int cfg, int mode, const StageArgs& args, int ntiles, hipStream_t stream) {
   constexpr bool full = std::is_same<OX, OY>::value;  // every configuration only for equal operator pairs
   switch (cfg) {
-    case 0: launch_one<OX, OY, UP, 32, 16, 2, 1>(mode, args, ntiles, stream); break;
-    case 1:
-      if constexpr (full) { launch_one<OX, OY, UP, 32, 16, 2, 0>(mode, args, ntiles, stream); break; }
-      return hipErrorNotSupported;
-    case 2:
-      if constexpr (full) { launch_one<OX, OY, UP, 32, 16, 2, 2>(mode, args, ntiles, stream); break; }
-      return hipErrorNotSupported;
-    case 3:
-      if constexpr (full) { launch_one<OX, OY, UP, 32, 32, 2, 1>(mode, args, ntiles, stream); break; }
-      return hipErrorNotSupported;
-    case 4:
-      if constexpr (full) { launch_one<OX, OY, UP, 32, 32, 4, 1>(mode, args, ntiles, stream); break; }
-      return hipErrorNotSupported;
-    case 5:
-      if constexpr (full) { launch_one<OX, OY, UP, 64, 16, 4, 1>(mode, args, ntiles, stream); break; }
-      return hipErrorNotSupported;
-    case 6:
-      if constexpr (full) { launch_one<OX, OY, UP, 64, 16, 2, 1>(mode, args, ntiles, stream); break; }
-      return hipErrorNotSupported;
+    case 0: launch_one<OX, OY, UP, 32, 16, 1, 2, 1>(mode, args, ntiles, stream); break;
+#define SBP_CFG(N, TX, TY, RX, RY, VAR)                                                            \
+  case N:                                                                                        \
+    if constexpr (full) {                                                                        \
+      launch_one<OX, OY, UP, TX, TY, RX, RY, VAR>(mode, args, ntiles, stream);                   \
+      break;                                                                                     \
+    }                                                                                            \
+    return hipErrorNotSupported;
+    SBP_CFG(1, 32, 16, 1, 2, 0)
+    SBP_CFG(2, 32, 24, 1, 2, 0)
+    SBP_CFG(3, 32, 20, 1, 2, 0)
+    SBP_CFG(4, 32, 12, 1, 2, 0)
+    SBP_CFG(5, 64, 16, 1, 2, 0)
+    SBP_CFG(6, 32, 24, 1, 2, 1)
+    SBP_CFG(7, 32, 16, 1, 2, 3)
+    SBP_CFG(8, 32, 12, 1, 2, 3)
+#undef SBP_CFG
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();

@@ -356,9 +356,9 @@ int rebuild(sbp_system_s* s) {
     return SBP_OK;
   };
   if (need_tune) {
-    // Autotuner: time MODE_S0 launches (they read y_n and write only the
-    // accumulator and the next stage input, so the state is untouched) for
-    // every tile shape and keep the fastest per group.
+    // Autotuner: time the stage-1..3 kernels (modes S0, S1, S2 read y_n and
+    // write only the accumulator and the stage inputs, so the state is
+    // untouched) for every configuration and keep the fastest per group.
     double gamma;
     gamma_get(gamma);
     std::vector<int> best(s->groups.size(), 0);
@@ -390,9 +390,11 @@ int rebuild(sbp_system_s* s) {
         const hipError_t w = launch_stage(g.op_eta, g.op_xi, g.upwind, sh, MODE_S0, a, g.ntiles, st);  // warm-up
         if (w == hipErrorNotSupported) continue;  // configuration not built for this operator pair
         SBP_TRY_HIP(w);
-        const int reps = 5;
+        const int reps = 3;
         SBP_TRY_HIP(hipEventRecord(e0, st));
-        for (int r = 0; r < reps; ++r) SBP_TRY_HIP(launch_stage(g.op_eta, g.op_xi, g.upwind, sh, MODE_S0, a, g.ntiles, st));
+        for (int r = 0; r < reps; ++r)
+          for (int m = MODE_S0; m <= MODE_S2; ++m)
+            SBP_TRY_HIP(launch_stage(g.op_eta, g.op_xi, g.upwind, sh, m, a, g.ntiles, st));
         SBP_TRY_HIP(hipEventRecord(e1, st));
         SBP_TRY_HIP(hipEventSynchronize(e1));
         float ms = 0.f;
@@ -1157,14 +1159,16 @@ int sbp_system_profile(sbp_system s, double dt, int64_t nsteps, double* ms_total
   float ms = 0.f;
   SBP_TRY_HIP(hipEventElapsedTime(&ms, e0, e1));
   *ms_total = ms;
-  // per-launch durations of the stage kernels: one more step with every stage
-  // launch bracketed by its own events
+  // per-launch durations of the stage kernels: up to 8 more steps with every
+  // stage launch bracketed by its own events
   static const double A_next[3] = {0.5, 0.5, 1.0};
   static const double Bw[4] = {1.0 / 6.0, 1.0 / 3.0, 1.0 / 3.0, 1.0 / 6.0};
   static const double C[3] = {0.5, 0.5, 1.0};
   std::vector<hipEvent_t> ev;
   double sum = 0.0;
   int nl = 0;
+  const int64_t tsteps = std::min<int64_t>(nsteps, 8);
+  for (int64_t step = 0; step < tsteps; ++step) {
   for (int stage = 0; stage < 4; ++stage)
     for (const Group& g : s->groups) {
       hipEvent_t a0, a1;
@@ -1191,6 +1195,7 @@ int sbp_system_profile(sbp_system s, double dt, int64_t nsteps, double* ms_total
       }
     }
   SBP_TRY_HIP(launch_advance_time(s->d_time, dt, st));
+  }
   SBP_TRY_HIP(hipStreamSynchronize(st));
   for (size_t i = 0; i < ev.size(); i += 2) {
     float m = 0.f;

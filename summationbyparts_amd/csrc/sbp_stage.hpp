@@ -104,16 +104,14 @@ __device__ __forceinline__ void col_window(const double* p, int stride, double o
 // Variants (autotuned): VAR 0 keeps the dissipation terms α·q of a thread's
 // region points in registers between the flux and dissipation passes; VAR 1
 // keeps only α and re-reads q (cache hits); VAR 2 stores flux and α·q
-// together (8 values per point in LDS, one pass per direction); VAR 3 is
-// VAR 0 with the finish operands loaded before the η-phase.
+// together (8 values per point in LDS, one pass per direction).
 // Threads own RX adjacent ξ-columns and RY consecutive η-rows of the tile.
 template <class OX, class OY, bool UP, int TX, int TY, int RX, int RY, int VAR>
 __global__ __launch_bounds__((TX / RX) * (TY / RY)) void stage_kernel(const StageArgs A, const int mode) {
   using G = StageGeom<OX, OY, TX, TY, RX, RY, VAR>;
   constexpr int NT = G::NT, HX = G::HX, HY = G::HY, WX = G::WX, NP = G::NP;
   constexpr int NV = UP ? G::NV : 4;
-  constexpr int NC = (VAR == 0 || VAR == 3) ? 4 : 1;  // cached values per region point between passes
-  constexpr bool PF = VAR == 3;                      // prefetch the finish operands
+  constexpr int NC = VAR == 0 ? 4 : 1;  // cached values per region point between passes
   constexpr int LCH = 3;                // region points whose loads are in flight together
   __shared__ __align__(16) double lds[NV * NP];
 
@@ -254,8 +252,7 @@ __global__ __launch_bounds__((TX / RX) * (TY / RY)) void stage_kernel(const Stag
   }
 
   // Finish operands at the output points (J, and y_n / accumulator as the
-  // stage combination needs them), loaded ahead of the η-phase when PF so
-  // that their latency overlaps it.
+  // stage combination needs them), all loaded before any store.
   double fJ[RY][RX], fa[RY][RX][4], fb[RY][RX][4];
   auto load_finish = [&]() {
 #pragma unroll
@@ -274,7 +271,6 @@ __global__ __launch_bounds__((TX / RX) * (TY / RY)) void stage_kernel(const Stag
         }
       }
   };
-  if constexpr (PF) load_finish();
 
   // ------------------------------------------------------------ η-phase
   {
@@ -379,7 +375,7 @@ __global__ __launch_bounds__((TX / RX) * (TY / RY)) void stage_kernel(const Stag
   // ------------------------------------------------------------ finish
   // (the edge SATs follow in sat_edge_kernel, added through the same linear
   // stage combination)
-  if constexpr (!PF) load_finish();
+  load_finish();
 #pragma unroll
   for (int s = 0; s < RY; ++s)
 #pragma unroll

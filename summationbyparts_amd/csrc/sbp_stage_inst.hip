@@ -41,8 +41,8 @@ static hipError_t launch_shape(int cfg, int mode, const StageArgs& args, int nti
     SBP_CFG(4, 32, 12, 1, 2, 0)
     SBP_CFG(5, 64, 16, 1, 2, 0)
     SBP_CFG(6, 32, 24, 1, 2, 1)
-    SBP_CFG(7, 32, 16, 1, 2, 3)
-    SBP_CFG(8, 32, 12, 1, 2, 3)
+    SBP_CFG(7, 16, 16, 1, 2, 0)
+    SBP_CFG(8, 16, 24, 1, 2, 0)
 #undef SBP_CFG
     default: return hipErrorInvalidValue;
   }

@@ -127,11 +127,13 @@ def test_rhs_periodic(gpu, names, shape, rhs_kind):
     assert rel_global(k, k_ref) < RHS_TOL
 
 
-@pytest.mark.parametrize("shape_index", [0, 1, 2])
+@pytest.mark.parametrize("shape_index", list(range(9)))
 def test_rhs_all_tile_shapes(gpu, shape_index):
-    """Every tile shape, on sizes that leave partial last tiles in both axes."""
-    from summationbyparts_amd import euler
-    for nx, ny in [(131, 77), (70, 40), (33, 97)]:
+    """Every kernel configuration, on sizes that leave partial last tiles in
+    both axes."""
+    from summationbyparts_amd import _lib, euler
+    assert _lib.lib().sbp_tile_shape_count() == 9
+    for nx, ny in [(131, 77), (70, 40), (33, 97), (80, 64)]:
         X, Y = curvilinear(nx, ny)
         q = _c2_state(X, Y, seed=3)
         ref_sys = O.System([(X, Y)], [(O.UPWIND9, O.UPWIND9)])
@@ -140,6 +142,11 @@ def test_rhs_all_tile_shapes(gpu, shape_index):
         sys.set_tile_shape(shape_index)
         sys.set_state(0, q)
         assert rel_global(sys.rhs(0.0)[0], ref_sys.rhs(0.0)[0]) < RHS_TOL, (nx, ny)
+        # one fused step (all four stage modes) against the oracle
+        dt = 0.2 / (nx - 1)
+        sys.advance_steps(dt, 1)
+        ref_sys.advance(dt, 1)
+        assert rel_maxnorm(sys.get_state(0), ref_sys.get_state(0)) < 1e-12, ("step", nx, ny)
 
 
 def test_rhs_device_buffers(gpu):

@@ -7,7 +7,7 @@ sign or index, or a transposed operand fails.
 import numpy as np
 import pytest
 
-from common import GAMMA, MACH, VORTEX, cartesian, curvilinear
+from common import GAMMA, MACH, VORTEX, cartesian, curvilinear, rel_maxnorm
 from oracle import oracle as O
 
 ALL = list(O.OPERATORS.items())
@@ -308,3 +308,160 @@ def test_max_dt_formula():
     # û = J·ξx·u = 10·1.5, v̂ = J·ηy·v = 10·0.5
     expect = 1.0 / max(15.0 * 30, 5.0 * 20)
     assert abs(S.max_dt() - expect) < 1e-15
+
+
+# --- flux, SAT and RK4 pins -------------------------------------------------
+
+def _euler_fluxes(q, gamma=GAMMA):
+    """Textbook 2-D Euler fluxes in conservative variables (works on complex
+    input for complex-step differentiation)."""
+    rho, m, n, e = q
+    p = (gamma - 1.0) * (e - (m * m + n * n) / (2.0 * rho))
+    E = np.array([m, m * m / rho + p, m * n / rho, m * (e + p) / rho])
+    F = np.array([n, m * n / rho, n * n / rho + p, n * (e + p) / rho])
+    return E, F
+
+
+def _flux_jacobian_times(q, d, kx, ky):
+    """(kx·∂E/∂q + ky·∂F/∂q)·d at state q by complex-step differentiation
+    (exact to rounding: the fluxes are rational in q)."""
+    h = 1e-30
+    E, F = _euler_fluxes(q.astype(complex) + 1j * h * d)
+    return (kx * E.imag + ky * F.imag) / h
+
+
+def test_fluxes_interior_polynomial_state():
+    """RHS_trad with SBP8 on a Cartesian grid with ρ = 1 and momentum /
+    energy of degree ≤ 2: every flux is a polynomial of degree ≤ 3, which the
+    SBP8 closure and interior differentiate exactly, so away from the SAT
+    edges k = -(∂E/∂x + ∂F/∂y) with the textbook fluxes. A dropped pressure
+    term, a swapped metric or a wrong sign fails this."""
+    nx, ny = 37, 29
+    x = np.linspace(-5.0, 5.0, nx)
+    y = np.linspace(-2.0, 3.0, ny)
+    X, Y = np.meshgrid(x, y)
+    q = np.empty((4, ny, nx))
+    q[0] = 1.0
+    q[1] = 0.5 + 0.1 * X
+    q[2] = 0.2 - 0.05 * Y
+    q[3] = 3.0 + 0.02 * X * X + 0.01 * X * Y
+    S = O.System([(X, Y)], [(O.SBP8, O.SBP8)])
+    S.set_rhs_kind(O.RHS_TRAD)
+    S.set_state(0, q)
+    k = S.rhs(0.0)[0]
+    # analytic divergence of the textbook fluxes by complex step along x and y
+    h = 1e-30
+    dEdx = _euler_fluxes(np.array([q[0], q[1] + 1j * h * 0.1, q[2], q[3] + 1j * h * (0.04 * X + 0.01 * Y)]))[0].imag / h
+    dFdy = _euler_fluxes(np.array([q[0], q[1], q[2] + 1j * h * (-0.05), q[3] + 1j * h * (0.01 * X)]))[1].imag / h
+    expect = -(dEdx + dFdy)
+    inner = (slice(None), slice(1, ny - 1), slice(1, nx - 1))
+    scale = np.max(np.abs(expect[inner]))
+    assert np.max(np.abs(k[inner] - expect[inner])) < 1e-9 * scale
+
+
+def _characteristic_penalty(q, d, kx_, ky_, sigma, gamma=GAMMA):
+    """Σ_i w_i (λ_i + σ|λ_i|) r_i (l_i · d) over the characteristic fields of
+    the normal flux Jacobian A_n = kx_·∂E/∂q + ky_·∂F/∂q at state q, with
+    the textbook right eigenvectors r_i (entropy, shear, two acoustic waves;
+    unit normal n = k/|k|) and l_i the rows of R⁻¹. The reference's TI
+    (euler/src/lib.rs:1213-1220) is diag(1, 1/ρ, 1, 1)·R⁻¹, and its T
+    (:1194-1199) is R·diag(1, 1, α, α), α = ρ/(√2 c), except that the
+    energy entry of the shear column is ρ(ky·u - kx·v) where the eigenvector
+    has (ky·u - kx·v). So the fields carry weights (1, 1/ρ, α, α), the shear
+    field feeds the energy equation ρ times too strongly, and T·TI is not the
+    identity unless ρ = 1 and α = 1; the reference's SAT is reproduced as it
+    is."""
+    rho, m, n_, e = q
+    u, v = m / rho, n_ / rho
+    p = (gamma - 1.0) * (e - (m * m + n_ * n_) / (2.0 * rho))
+    c = np.sqrt(gamma * p / rho)
+    kn = np.hypot(kx_, ky_)
+    kx, ky = kx_ / kn, ky_ / kn
+    theta = kx * u + ky * v
+    H = (e + p) / rho
+    R = np.array([[1.0, 0.0, 1.0, 1.0],
+                  [u, ky, u + kx * c, u - kx * c],
+                  [v, -kx, v + ky * c, v - ky * c],
+                  [(u * u + v * v) / 2.0, ky * u - kx * v, H + c * theta, H - c * theta]])
+    U = kx_ * u + ky_ * v
+    lam = np.array([U, U, U + c * kn, U - c * kn])
+    # the eigenpairs are those of the exact Jacobian (complex step)
+    An = np.stack([_flux_jacobian_times(q, np.eye(4)[i], kx_, ky_) for i in range(4)], axis=1)
+    assert np.allclose(An @ R, R * lam[None, :], rtol=0, atol=1e-12 * np.max(np.abs(An)))
+    alpha = rho / (np.sqrt(2.0) * c)
+    w = np.array([1.0, 1.0 / rho, alpha, alpha])
+    T = R.copy()
+    T[3, 1] *= rho
+    return T @ (w * (lam + sigma * np.abs(lam)) * np.linalg.solve(R, d))
+
+
+@pytest.mark.parametrize("edge", ["north", "south", "east", "west"])
+def test_sat_characteristic_supersonic(edge):
+    """Characteristic SAT of a uniform supersonic state against far-field
+    vortex data g: at the inflow edge every characteristic enters, so the
+    penalty is hi·τ·Σ_i w_i·2λ_i·r_i·(l_i·(q-g)) with hi = (n-1)/H[0], and
+    at the outflow edge every λ + σ|λ| vanishes. Pins the sign σ, τ, hi and
+    the eigenvector tables T and TI of euler/src/lib.rs:1150-1252."""
+    nx, ny = 31, 27
+    x = np.linspace(-5.0, 5.0, nx)
+    y = np.linspace(-2.0, 3.0, ny)
+    X, Y = np.meshgrid(x, y)
+    speed = 3.0  # sound speed of q is √(1.4/1.3) ≈ 1.04
+    vel = {"north": (0.3, -speed), "south": (0.3, speed), "east": (-speed, 0.2), "west": (speed, 0.2)}[edge]
+    rho, p = 1.3, 1.0
+    qv = np.array([rho, rho * vel[0], rho * vel[1], p / (GAMMA - 1) + 0.5 * rho * (vel[0] ** 2 + vel[1] ** 2)])
+    q = np.broadcast_to(qv[:, None, None], (4, ny, nx)).copy()
+    d = {"north": O.NORTH, "south": O.SOUTH, "east": O.EAST, "west": O.WEST}[edge]
+    S = O.System([(X, Y)], [(O.SBP8, O.UPWIND4)])
+    S.set_rhs_kind(O.RHS_TRAD)
+    vp = O.VortexParams.make(list(VORTEX), MACH)
+    S.set_bc(0, d, O.BC_VORTEX, 0, vp)
+    S.set_state(0, q)
+    k = S.rhs(0.0)[0]
+    g = O.vortex(vp, 0.0, X, Y)  # far-field data at the edge points
+    if edge in ("north", "south"):
+        n_a, h0, kx, ky = ny, O.op_h(O.SBP8)[0], 0.0, 1.0 / (y[1] - y[0]) / (ny - 1)  # ∇η = (0, dη/dy)
+        sl = (slice(None), ny - 1 if edge == "north" else 0, slice(None))
+        sigma, tau = (-1.0, 1.0) if edge == "north" else (1.0, -1.0)
+        other = (slice(None), 0 if edge == "north" else ny - 1, slice(1, nx - 1))
+    else:
+        n_a, h0, kx, ky = nx, O.op_h(O.UPWIND4)[0], 1.0 / (x[1] - x[0]) / (nx - 1), 0.0
+        sl = (slice(None), slice(None), nx - 1 if edge == "east" else 0)
+        sigma, tau = (-1.0, 1.0) if edge == "east" else (1.0, -1.0)
+        other = (slice(None), slice(1, ny - 1), 0 if edge == "east" else nx - 1)
+    hi = (n_a - 1) / h0
+    ge = g[sl]
+    expect = np.stack([hi * tau * _characteristic_penalty(qv, qv - ge[:, i], kx, ky, sigma)
+                       for i in range(ge.shape[1])], axis=1)
+    got = k[sl]
+    assert np.max(np.abs(got - expect)) < 1e-11 * np.max(np.abs(expect))
+    # the opposite edge is outflow for this state: no penalty there (periodic
+    # self-coupling of a uniform state adds none either)
+    assert np.max(np.abs(k[other])) < 1e-11 * np.max(np.abs(expect))
+
+
+def test_rk4_step_is_classical_rk4():
+    """One step of the oracle's integrate equals the textbook classical RK4
+    composed from its own RHS, with stage times t, t+dt/2, t+dt/2, t+dt (the
+    far-field vortex BC makes the RHS time dependent, so a wrong stage time
+    fails)."""
+    vp = O.VortexParams.make(list(VORTEX), MACH)
+    n = 24
+    X, Y = cartesian(n, n, -3.0, 3.0)
+    S = O.System([(X, Y)], [(O.UPWIND4, O.UPWIND9)])
+    for d in range(4):
+        S.set_bc(0, d, O.BC_VORTEX, 0, vp)
+    S.vortex(0.0, vp)
+    t0, dt = 0.3, 0.01
+    q0 = S.get_state(0)
+    stages = []
+    for c, a in ((0.0, 0.0), (0.5, 0.5), (0.5, 0.5), (1.0, 1.0)):
+        S.set_state(0, q0 + (a * dt) * stages[-1] if stages else q0)
+        stages.append(S.rhs(t0 + c * dt)[0])
+    expect = q0 + dt * (stages[0] + 2 * stages[1] + 2 * stages[2] + stages[3]) / 6.0
+    S.set_state(0, q0)
+    S.time = t0
+    S.advance(dt, 1)
+    got = S.get_state(0)
+    assert rel_maxnorm(got - q0, expect - q0) < 1e-12
+    assert abs(S.time - (t0 + dt)) < 1e-15

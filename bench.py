@@ -111,6 +111,12 @@ def main():
     sys_.advance_steps(dt, args.steps)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
+    # host -> device upload of one state through the boundary (not part of
+    # the timed region: a run pays it once)
+    t1 = time.perf_counter()
+    sys_.set_state(0, q0)
+    torch.cuda.synchronize()
+    upload_ms = (time.perf_counter() - t1) * 1e3
     # after the timed region: HIP-event time of 8 graph-replayed steps, then
     # the mean duration of one stage-kernel launch over 8 more steps (events
     # on the library's stream around each launch)
@@ -142,7 +148,8 @@ def main():
         "config": {"workload": f"single curvilinear block {n}x{n}, upwind9 + dissipation, vortex + 1e-3 density "
                                f"noise, periodic SATs, RK4 dt=0.2/{n - 1}", "nx": n, "ny": n,
                    "operator": "upwind9", "precision": args.precision, "tile": list(tile),
-                   "event_ms_per_step": ms_events / 8, "stage_launch_ms": ms_stage},
+                   "event_ms_per_step": ms_events / 8, "stage_launch_ms": ms_stage,
+                   "state_upload_ms": upload_ms},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic},
     }

@@ -33,6 +33,9 @@ __device__ __forceinline__ double fsqrt(double x) {
   return __builtin_fma(d, h, g);
 }
 
+__device__ __forceinline__ double fma_t(double a, double b, double c) { return __builtin_fma(a, b, c); }
+__device__ __forceinline__ float fma_t(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+
 // Load through a uniform base pointer and a 32-bit element offset (selects
 // the SGPR-base + VGPR-offset addressing of global loads).
 template <typename T>
@@ -179,6 +182,102 @@ __device__ __forceinline__ void face_terms(double rho, double ru, double rv, dou
   }
 }
 
+// ---------------------------------------------------------------------------
+// Mixed form (SBP_MIXED): the face terms are evaluated in FP32 as deviations
+// from a reference state q_C that is constant over one tile (its centre
+// point), so FP32 rounding acts on local variations, not on the magnitudes
+// of the state. The stencils of both operators have zero row sums, so
+//   D(α q)                    = D(α δq + δα q_C)          (α = α_C + δα)
+// exactly, and for the fluxes Ê = J·ξx E + J·ξy F
+//   D_ξ Ê + D_η F̂ = D_ξ(J·ξx δE + J·ξy δF) + D_η(J·ηx δE + J·ηy δF)
+//                   + E_C (D_ξ J·ξx + D_η J·ηx) + F_C (D_ξ J·ξy + D_η J·ηy),
+// where the last two terms vanish identically for metrics computed with the
+// same operators (J·ξx = D_η y, J·ηx = -D_ξ y, and D_ξ, D_η commute), so the
+// mixed kernel differentiates only the deviation terms. δq is formed in FP64
+// and rounded once; δE, δF, δα follow from δq by the algebra below without
+// subtracting large FP32 numbers.
+struct MixRef {
+  double rho, m, n, e;  // q_C (FP64, subtracted before rounding)
+  double jx, jy;        // metric pair of the phase at the centre
+  float rhof, mf, nf, ef;
+  float u, v, p, ep, pr, c;  // velocities, pressure, e+p, p/ρ, sound speed at q_C
+  float uh, g, alpha;        // û_C, |∇|_C and α_C with the centre metric pair
+};
+
+__device__ __forceinline__ MixRef mix_ref(const double q[4], double jx, double jy, double gamma) {
+  MixRef R;
+  R.rho = q[0];
+  R.m = q[1];
+  R.n = q[2];
+  R.e = q[3];
+  R.jx = jx;
+  R.jy = jy;
+  const double u = q[1] / q[0], v = q[2] / q[0];
+  const double p = (gamma - 1.0) * (q[3] - 0.5 * (q[1] * u + q[2] * v));
+  const double pr = p / q[0];
+  const double c = sqrt(fmax(gamma * pr, 0.0));
+  const double uh = jx * u + jy * v;
+  const double g = sqrt(jx * jx + jy * jy);
+  R.rhof = (float)q[0];
+  R.mf = (float)q[1];
+  R.nf = (float)q[2];
+  R.ef = (float)q[3];
+  R.u = (float)u;
+  R.v = (float)v;
+  R.p = (float)p;
+  R.ep = (float)(q[3] + p);
+  R.pr = (float)pr;
+  R.c = (float)c;
+  R.uh = (float)uh;
+  R.g = (float)g;
+  R.alpha = (float)(fabs(uh) + c * g);
+  return R;
+}
+
+__device__ __forceinline__ float frcpf(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ float fsqrtf(float x) { return __builtin_amdgcn_sqrtf(x); }
+
+// Deviation face terms of one point: f = Ê - Ê_C(jx, jy) and
+// aq = α q - α_C q_C, both in FP32 (Ê_C(jx, jy) = jx E(q_C) + jy F(q_C)).
+template <bool UP>
+__device__ __forceinline__ void face_terms_mix(const double q[4], double jx, double jy, const MixRef& R, float gamma,
+                                               float f[4], float aq[4], unsigned& err) {
+  const float d0 = (float)(q[0] - R.rho), d1 = (float)(q[1] - R.m), d2 = (float)(q[2] - R.n), d3 = (float)(q[3] - R.e);
+  const float jxf = (float)jx, jyf = (float)jy;
+  const float ir = frcpf((float)q[0]);
+  const float du = (d1 - R.u * d0) * ir, dv = (d2 - R.v * d0) * ir;  // u - u_C, v - v_C
+  const float u = R.u + du, v = R.v + dv;
+  const float dk = 0.5f * (d1 * u + R.mf * du + d2 * v + R.nf * dv);  // δ(ρ(u²+v²)/2)
+  const float dp = (gamma - 1.0f) * (d3 - dk);
+  const float duh = jxf * du + jyf * dv;     // û - (jx u_C + jy v_C)
+  const float uh = jxf * R.u + jyf * R.v + duh;  // û
+  f[0] = d0 * uh + R.rhof * duh;
+  f[1] = d1 * uh + R.mf * duh + jxf * dp;
+  f[2] = d2 * uh + R.nf * duh + jyf * dp;
+  f[3] = (d3 + dp) * uh + R.ep * duh;
+  if constexpr (UP) {
+    if (!(q[0] > 0.0)) err |= ERRBIT_RHO;
+    if (!(R.p + dp > 0.0f)) err |= ERRBIT_PRESSURE;
+    const float djx = (float)(jx - R.jx), djy = (float)(jy - R.jy);
+    const float dU = duh + djx * R.u + djy * R.v;  // û - û_C
+    const float un = R.uh + dU;
+    const bool pos = R.uh >= 0.0f;
+    // |û| - |û_C| without cancellation: ±δû on one side of zero, else ∓(2û_C + δû)
+    const float dabs = (pos == (un >= 0.0f)) ? (pos ? dU : -dU) : (pos ? -(2.0f * R.uh + dU) : 2.0f * R.uh + dU);
+    const float g = fsqrtf(jxf * jxf + jyf * jyf);
+    const float dg = (djx * (jxf + (float)R.jx) + djy * (jyf + (float)R.jy)) * frcpf(g + R.g);
+    const float dpr = (dp - R.pr * d0) * ir;  // p/ρ - p_C/ρ_C
+    const float c = fsqrtf(fmaxf(gamma * (R.pr + dpr), 0.0f));
+    const float dc = gamma * dpr * frcpf(c + R.c);
+    const float dal = dabs + dc * g + R.c * dg;  // α - α_C
+    const float al = R.alpha + dal;
+    aq[0] = al * d0 + dal * R.rhof;
+    aq[1] = al * d1 + dal * R.mf;
+    aq[2] = al * d2 + dal * R.nf;
+    aq[3] = al * d3 + dal * R.ef;
+  }
+}
+
 // Characteristic SAT of one edge point (SAT_characteristic, lib.rs:1150-1252):
 // k += hi·tau·T((L + sign|L|)∘(TI(q - g))).
 __device__ __forceinline__ void sat_point(double k[4], const double q[4], const double g[4], double hi, double sign, double tau,
@@ -287,13 +386,47 @@ __device__ __forceinline__ const double* stage_input(const BlockDev& B, int mode
   }
 }
 
+// Buffers of the mixed form (see BufRole).
+__device__ __forceinline__ double* yn_buf(const BlockDev& B, const StageArgs& A) {
+  return A.swap ? B.buf[BUF_ACC] : B.buf[BUF_YN];
+}
+__device__ __forceinline__ double* acc_buf(const BlockDev& B, const StageArgs& A) {
+  return A.swap ? B.buf[BUF_YN] : B.buf[BUF_ACC];
+}
+// Stage-increment set read by stage `mode` (S1, S3: set 0; S2: set 1) and
+// written by it (S0, S2: set 0; S1: set 1).
+__device__ __forceinline__ const float* delta_in(const BlockDev& B, int mode) {
+  return reinterpret_cast<const float*>(B.buf[BUF_YA]) + (mode == MODE_S2 ? 4 * B.ps : 0);
+}
+__device__ __forceinline__ float* delta_out(const BlockDev& B, int mode) {
+  return reinterpret_cast<float*>(B.buf[BUF_YA]) + (mode == MODE_S1 ? 4 * B.ps : 0);
+}
+
+// The state stage `mode` reads at element offset o.
+__device__ __forceinline__ void load_stage_state(const StageArgs& A, const BlockDev& B, int mode, uint32_t o, double q[4]) {
+  const uint32_t ps = (uint32_t)B.ps;
+  if (A.mixed && mode >= MODE_S0) {
+    const double* yn = yn_buf(B, A);
+#pragma unroll
+    for (int c = 0; c < 4; ++c) q[c] = ldg(yn, c * ps + o);
+    if (mode >= MODE_S1) {
+      const float* d = delta_in(B, mode);
+#pragma unroll
+      for (int c = 0; c < 4; ++c) q[c] += (double)ldg(d, c * ps + o);
+    }
+  } else {
+    const double* in = stage_input(B, mode);
+#pragma unroll
+    for (int c = 0; c < 4; ++c) q[c] = ldg(in, c * ps + o);
+  }
+}
+
 // Boundary data g of edge `dir` at edge index t (boundary_extracts, lib.rs:869-915).
 __device__ __forceinline__ void edge_data(const StageArgs& A, const BlockDev& B, int dir, int t, int x, int y, int mode,
                                           double time, double g[4], unsigned& err) {
   const int src = B.edge_src[dir];
   if (src == EDGE_THIS || src == EDGE_GRID) {
     const BlockDev& S = src == EDGE_THIS ? B : A.blocks[B.edge_nb[dir]];
-    const double* in = stage_input(S, mode);
     int64_t o;
     switch (dir) {
       case SBP_NORTH: o = t; break;                                  // source south row
@@ -301,8 +434,7 @@ __device__ __forceinline__ void edge_data(const StageArgs& A, const BlockDev& B,
       case SBP_EAST: o = (int64_t)t * S.ld; break;                   // source west column
       default: o = (int64_t)t * S.ld + (S.nx - 1); break;             // source east column
     }
-#pragma unroll
-    for (int c = 0; c < 4; ++c) g[c] = in[c * S.ps + o];
+    load_stage_state(A, S, mode, (uint32_t)o, g);
   } else if (src == EDGE_VORTEX) {
     const int64_t o = (int64_t)y * B.ld + x;
     g[2] = 0.0;

@@ -7,6 +7,10 @@
 namespace sbp {
 
 // Roles of the four state buffers of a block during one fused RK4 step.
+// FP64 form: y_n, the accumulator and two stage inputs Y_s. Mixed form: YA
+// holds the FP32 stage increments Δ_s = Y_s - y_n (two sets of 4 planes, the
+// second at float offset 4·ps), and the last stage writes y_{n+1} over the
+// accumulator, after which y_n and the accumulator swap roles (StageArgs::swap).
 enum BufRole : int { BUF_YN = 0, BUF_ACC = 1, BUF_YA = 2, BUF_YB = 3 };
 
 // Boundary-data source of one block edge inside a kernel.
@@ -64,6 +68,8 @@ struct StageArgs {
   double rk_scale_acc;  // b_s·dt
   double rk_scale_next;  // a_s·dt (stage input of the next stage)
   double time_offset;  // c_s·dt added to *time for boundary data
+  int mixed;  // FP32 right-hand side with FP64 state (SBP_MIXED)
+  int swap;   // mixed form, odd steps: y_n lives in BUF_ACC and the accumulator in BUF_YN
 };
 
 }  // namespace sbp

@@ -307,14 +307,12 @@ __global__ __launch_bounds__(256) void sat_edge_kernel(const StageArgs A, const 
   }
   const uint32_t ps = (uint32_t)B.ps;
   const uint32_t o = (uint32_t)(y * B.ld + x);
-  const double* in = stage_input(B, mode);
   const double* met = B.metrics;
   const double gamma = A.gamma;
   const double time = (A.time ? *A.time : 0.0) + A.time_offset;
   unsigned err = 0;
   double q[4], g[4], sat[4] = {0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-  for (int c = 0; c < 4; ++c) q[c] = ldg(in, c * ps + o);
+  load_stage_state(A, B, mode, o, q);
   const double dj = ldg(met, o);
   if (y == ny - 1) {
     edge_data(A, B, SBP_NORTH, x, x, y, mode, time, g, err);
@@ -335,6 +333,17 @@ __global__ __launch_bounds__(256) void sat_edge_kernel(const StageArgs A, const 
   if (mode == MODE_RHS) {
 #pragma unroll
     for (int c = 0; c < 4; ++c) stg(B.rhs_out, c * ps + o, ldg((const double*)B.rhs_out, c * ps + o) + sat[c]);
+  } else if (A.mixed) {
+    // y_{n+1} (stage 4) or the accumulator, and the FP32 stage increment
+    double* accb = acc_buf(B, A);
+#pragma unroll
+    for (int c = 0; c < 4; ++c) stg(accb, c * ps + o, ldg((const double*)accb, c * ps + o) + A.rk_scale_acc * sat[c]);
+    if (mode != MODE_S3) {
+      float* dn = delta_out(B, mode);
+#pragma unroll
+      for (int c = 0; c < 4; ++c)
+        stg(dn, c * ps + o, (float)((double)ldg((const float*)dn, c * ps + o) + A.rk_scale_next * sat[c]));
+    }
   } else if (mode == MODE_S3) {
     double* out = B.buf[BUF_YN];
 #pragma unroll

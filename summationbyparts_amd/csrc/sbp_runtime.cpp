@@ -199,6 +199,8 @@ struct sbp_system_s {
   int partials_cap = 0;
   int rhs_kind = SBP_RHS_AUTO;
   int precision = SBP_FP64;
+  // mixed form: 1 while y_n lives in the accumulator buffer (odd steps)
+  int parity = 0;
   int forced_shape = -1;
   bool dirty = true;  // device descriptors / tiles need rebuilding
   // captured steps: graph of `graph_steps` RK4 steps for step size graph_dt
@@ -277,6 +279,9 @@ BlockDev make_dev(const sbp_system_s* s, int i, int shape) {
   return b;
 }
 
+// Autotuner cache key part: RHS kind and precision of a group.
+int tune_kind(const sbp_system_s* s, const Group& g) { return (int)g.upwind + 2 * (s->precision == SBP_MIXED); }
+
 // Tile list of a group ordered for the 8 XCDs of an MI355X: workgroup g runs
 // on XCD g mod 8, so the natural (block, row, column) order is cut into 8
 // contiguous chunks and interleaved; each XCD then sweeps one compact band of
@@ -302,8 +307,43 @@ void build_tiles(const sbp_system_s* s, const Group& g, int shape, std::vector<T
       if (start[x] + j < start[x + 1]) out.push_back(nat[start[x] + j]);
 }
 
+// Arguments of stage `stage` (0..3) of an RK4 step of size dt; the tile
+// pointer is set per group by the caller. `swap`: the mixed form's buffer
+// roles of the step (y_n in the accumulator buffer on odd steps).
+StageArgs stage_args(const sbp_system_s* s, double dt, int stage, int swap) {
+  static const double A_next[3] = {0.5, 0.5, 1.0};                        // Rk4::A nonzero entries
+  static const double Bw[4] = {1.0 / 6.0, 1.0 / 3.0, 1.0 / 3.0, 1.0 / 6.0};  // Rk4::B
+  static const double C[3] = {0.5, 0.5, 1.0};                             // Rk4::C
+  double gamma;
+  gamma_get(gamma);
+  StageArgs a{};
+  a.blocks = s->d_blocks;
+  a.tiles = s->d_tiles;
+  a.time = s->d_time;
+  a.error = s->d_err;
+  a.dt = dt;
+  a.gamma = gamma;
+  a.rk_scale_acc = Bw[stage] * dt;
+  a.rk_scale_next = stage < 3 ? A_next[stage] * dt : 0.0;
+  a.time_offset = stage == 0 ? 0.0 : dt * C[stage - 1];
+  a.mixed = s->precision == SBP_MIXED;
+  a.swap = a.mixed ? swap : 0;
+  return a;
+}
+
+// Mixed form: bring y_n back into its own buffer after an odd number of
+// steps, so that every host-side access finds the state in BUF_YN.
+int normalize(sbp_system_s* s) {
+  if (!s->parity) return SBP_OK;
+  for (auto& h : s->blocks)
+    SBP_TRY_HIP(launch_copy_planes(h.d_buf[BUF_ACC], h.ps, h.ld, h.d_buf[BUF_YN], h.ps, h.ld, 4, h.ny, h.nx, s->ctx->stream));
+  s->parity = 0;
+  return SBP_OK;
+}
+
 int rebuild(sbp_system_s* s) {
   if (!s->dirty) return SBP_OK;
+  SBP_TRY(normalize(s));
   hipStream_t st = s->ctx->stream;
   // groups by operator pair and RHS kind
   s->groups.clear();
@@ -325,7 +365,7 @@ int rebuild(sbp_system_s* s) {
   for (auto& g : s->groups) {
     int64_t pts = 0;
     for (int bi : g.blocks) pts += (int64_t)s->blocks[bi].nx * s->blocks[bi].ny;
-    auto key = std::make_tuple(g.op_eta, g.op_xi, (int)g.upwind, pts, (int)g.blocks.size());
+    auto key = std::make_tuple(g.op_eta, g.op_xi, tune_kind(s, g), pts, (int)g.blocks.size());
     if (s->forced_shape >= 0) {
       g.shape = s->forced_shape;
     } else {
@@ -359,8 +399,6 @@ int rebuild(sbp_system_s* s) {
     // Autotuner: time the stage-1..3 kernels (modes S0, S1, S2 read y_n and
     // write only the accumulator and the stage inputs, so the state is
     // untouched) for every configuration and keep the fastest per group.
-    double gamma;
-    gamma_get(gamma);
     std::vector<int> best(s->groups.size(), 0);
     std::vector<float> best_ms(s->groups.size(), 1e30f);
     hipEvent_t e0, e1;
@@ -377,16 +415,8 @@ int rebuild(sbp_system_s* s) {
       for (size_t gi = 0; gi < s->groups.size(); ++gi) {
         Group& g = s->groups[gi];
         if (saved[gi] >= 0) continue;
-        StageArgs a{};
-        a.blocks = s->d_blocks;
+        StageArgs a = stage_args(s, 0.0, 0, 0);
         a.tiles = s->d_tiles + g.tile_offset;
-        a.time = s->d_time;
-        a.error = s->d_err;
-        a.dt = 0.0;
-        a.gamma = gamma;
-        a.rk_scale_acc = 0.0;
-        a.rk_scale_next = 0.0;
-        a.time_offset = 0.0;
         const hipError_t w = launch_stage(g.op_eta, g.op_xi, g.upwind, sh, MODE_S0, a, g.ntiles, st);  // warm-up
         if (w == hipErrorNotSupported) continue;  // configuration not built for this operator pair
         SBP_TRY_HIP(w);
@@ -416,7 +446,7 @@ int rebuild(sbp_system_s* s) {
       g.shape = best[gi];
       int64_t pts = 0;
       for (int bi : g.blocks) pts += (int64_t)s->blocks[bi].nx * s->blocks[bi].ny;
-      s->ctx->tuned[std::make_tuple(g.op_eta, g.op_xi, (int)g.upwind, pts, (int)g.blocks.size())] = g.shape;
+      s->ctx->tuned[std::make_tuple(g.op_eta, g.op_xi, tune_kind(s, g), pts, (int)g.blocks.size())] = g.shape;
     }
   }
   SBP_TRY(upload());
@@ -433,50 +463,16 @@ int check_err(sbp_system_s* s) {
   return status_from_errword(w);
 }
 
-// Arguments of stage `stage` (0..3) of an RK4 step of size dt; the tile
-// pointer is set per group by the caller.
-StageArgs stage_args(const sbp_system_s* s, double dt, int stage) {
-  static const double A_next[3] = {0.5, 0.5, 1.0};                        // Rk4::A nonzero entries
-  static const double Bw[4] = {1.0 / 6.0, 1.0 / 3.0, 1.0 / 3.0, 1.0 / 6.0};  // Rk4::B
-  static const double C[3] = {0.5, 0.5, 1.0};                             // Rk4::C
-  double gamma;
-  gamma_get(gamma);
-  StageArgs a{};
-  a.blocks = s->d_blocks;
-  a.tiles = s->d_tiles;
-  a.time = s->d_time;
-  a.error = s->d_err;
-  a.dt = dt;
-  a.gamma = gamma;
-  a.rk_scale_acc = Bw[stage] * dt;
-  a.rk_scale_next = stage < 3 ? A_next[stage] * dt : 0.0;
-  a.time_offset = stage == 0 ? 0.0 : dt * C[stage - 1];
-  return a;
-}
-
-// One fused RK4 step: four stage launches per group and the time update
-// (integrate::<Rk4>, utils/integrate/src/lib.rs:176-222).
-int enqueue_step(sbp_system_s* s, double dt, hipStream_t st) {
-  static const double A_next[3] = {0.5, 0.5, 1.0};                  // Rk4::A nonzero entries
-  static const double Bw[4] = {1.0 / 6.0, 1.0 / 3.0, 1.0 / 3.0, 1.0 / 6.0};  // Rk4::B
-  static const double C[3] = {0.5, 0.5, 1.0};                       // Rk4::C
-  double gamma;
-  gamma_get(gamma);
+// One fused RK4 step: four stage launches per group, the edge SATs of each
+// stage and the time update (integrate::<Rk4>, utils/integrate/src/lib.rs:176-222).
+int enqueue_step(sbp_system_s* s, double dt, int swap, hipStream_t st) {
   for (int stage = 0; stage < 4; ++stage) {
+    StageArgs a = stage_args(s, dt, stage, swap);
     for (const Group& g : s->groups) {
-      StageArgs a{};
-      a.blocks = s->d_blocks;
-      a.tiles = s->d_tiles + g.tile_offset;
-      a.time = s->d_time;
-      a.error = s->d_err;
-      a.dt = dt;
-      a.gamma = gamma;
-      a.rk_scale_acc = Bw[stage] * dt;
-      a.rk_scale_next = stage < 3 ? A_next[stage] * dt : 0.0;
-      a.time_offset = stage == 0 ? 0.0 : dt * C[stage - 1];
-      SBP_TRY_HIP(launch_stage(g.op_eta, g.op_xi, g.upwind, g.shape, stage, a, g.ntiles, st));
+      StageArgs ag = a;
+      ag.tiles = s->d_tiles + g.tile_offset;
+      SBP_TRY_HIP(launch_stage(g.op_eta, g.op_xi, g.upwind, g.shape, stage, ag, g.ntiles, st));
     }
-    StageArgs a = stage_args(s, dt, stage);
     SBP_TRY_HIP(launch_sat_edges(stage, a, (int)s->blocks.size(), max_perimeter(s), st));
   }
   SBP_TRY_HIP(launch_advance_time(s->d_time, dt, st));
@@ -489,7 +485,8 @@ int ensure_graph(sbp_system_s* s, double dt, int steps) {
   hipStream_t st = s->ctx->stream;
   SBP_TRY_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
   int rc = SBP_OK;
-  for (int i = 0; i < steps && rc == SBP_OK; ++i) rc = enqueue_step(s, dt, st);
+  // the mixed form alternates buffer roles; its graphs hold an even number of steps
+  for (int i = 0; i < steps && rc == SBP_OK; ++i) rc = enqueue_step(s, dt, i & 1, st);
   hipGraph_t g = nullptr;
   hipError_t e = hipStreamEndCapture(st, &g);
   if (rc != SBP_OK) {
@@ -509,20 +506,30 @@ int ensure_graph(sbp_system_s* s, double dt, int steps) {
 int steps_per_graph(const sbp_system_s* s) {
   int64_t pts = 0;
   for (auto& b : s->blocks) pts += (int64_t)b.nx * b.ny;
-  if (pts >= (int64_t)1 << 22) return 1;
+  if (pts >= (int64_t)1 << 22) return s->precision == SBP_MIXED ? 2 : 1;
   if (pts >= (int64_t)1 << 18) return 4;
   return 16;
 }
 
 int advance_impl(sbp_system_s* s, double dt, int64_t nsteps) {
   SBP_TRY(rebuild(s));
+  const bool mixed = s->precision == SBP_MIXED;
   const int G = steps_per_graph(s);
   int64_t done = 0;
-  if (nsteps >= G) {
+  // graphs start with y_n in its own buffer
+  if (s->parity && nsteps > 0) {
+    SBP_TRY(enqueue_step(s, dt, s->parity, s->ctx->stream));
+    s->parity ^= 1;
+    ++done;
+  }
+  if (nsteps - done >= G) {
     SBP_TRY(ensure_graph(s, dt, G));
     for (; done + G <= nsteps; done += G) SBP_TRY_HIP(hipGraphLaunch(s->graph, s->ctx->stream));
   }
-  for (; done < nsteps; ++done) SBP_TRY(enqueue_step(s, dt, s->ctx->stream));
+  for (; done < nsteps; ++done) {
+    SBP_TRY(enqueue_step(s, dt, s->parity, s->ctx->stream));
+    if (mixed) s->parity ^= 1;
+  }
   return SBP_OK;
 }
 
@@ -536,6 +543,7 @@ int ensure_partials(sbp_system_s* s, int n) {
 }
 
 int h2_sum(sbp_system_s* s, int bi, const double* b, const VortexDev* vp_host, double t, long double& sum) {
+  SBP_TRY(normalize(s));
   HostBlock& h = s->blocks[bi];
   const int nwg = reduce_h2_workgroups(h.ny, h.nx);
   SBP_TRY(ensure_partials(s, nwg + 64));
@@ -945,7 +953,8 @@ int sbp_system_set_rhs_kind(sbp_system s, int kind) {
 
 int sbp_system_set_precision(sbp_system s, int precision) {
   if (!s || (precision != SBP_FP64 && precision != SBP_MIXED)) return SBP_ERR_INVALID;
-  if (precision == SBP_MIXED) return SBP_ERR_INVALID;  // FP32-RHS kernels not built yet
+  SBP_TRY_HIP(hipSetDevice(s->ctx->device));
+  SBP_TRY(normalize(s));
   s->precision = precision;
   s->dirty = true;
   return SBP_OK;
@@ -955,6 +964,7 @@ int sbp_system_set_state(sbp_system s, int block, const double* host) {
   if (!s || !host || block < 0 || block >= (int)s->blocks.size()) return SBP_ERR_INVALID;
   HostBlock& h = s->blocks[block];
   SBP_TRY_HIP(hipSetDevice(s->ctx->device));
+  SBP_TRY(normalize(s));
   for (int c = 0; c < 4; ++c)
     SBP_TRY_HIP(hipMemcpy2DAsync(h.d_buf[BUF_YN] + c * h.ps, h.ld * 8, host + (size_t)c * h.nx * h.ny, (size_t)h.nx * 8,
                                  (size_t)h.nx * 8, h.ny, hipMemcpyHostToDevice, s->ctx->stream));
@@ -966,6 +976,7 @@ int sbp_system_get_state(sbp_system s, int block, double* host) {
   if (!s || !host || block < 0 || block >= (int)s->blocks.size()) return SBP_ERR_INVALID;
   HostBlock& h = s->blocks[block];
   SBP_TRY_HIP(hipSetDevice(s->ctx->device));
+  SBP_TRY(normalize(s));
   for (int c = 0; c < 4; ++c)
     SBP_TRY_HIP(hipMemcpy2DAsync(host + (size_t)c * h.nx * h.ny, (size_t)h.nx * 8, h.d_buf[BUF_YN] + c * h.ps, h.ld * 8,
                                  (size_t)h.nx * 8, h.ny, hipMemcpyDeviceToHost, s->ctx->stream));
@@ -976,6 +987,8 @@ int sbp_system_get_state(sbp_system s, int block, double* host) {
 int sbp_system_state_device(sbp_system s, int block, double** dev, int64_t* ld, int64_t* plane_stride) {
   if (!s || !dev || block < 0 || block >= (int)s->blocks.size()) return SBP_ERR_INVALID;
   HostBlock& h = s->blocks[block];
+  SBP_TRY_HIP(hipSetDevice(s->ctx->device));
+  SBP_TRY(normalize(s));
   *dev = h.d_buf[BUF_YN];
   if (ld) *ld = h.ld;
   if (plane_stride) *plane_stride = h.ps;
@@ -998,6 +1011,7 @@ int sbp_system_vortex(sbp_system s, double t, const sbp_vortex_params* params) {
   double gamma;
   if (!gamma_get(gamma)) return SBP_ERR_GAMMA;
   SBP_TRY_HIP(hipSetDevice(s->ctx->device));
+  s->parity = 0;  // every block's y_n is overwritten
   const VortexDev vp = to_dev(*params);
   for (auto& h : s->blocks)
     SBP_TRY_HIP(launch_vortex(vp, gamma, t, h.d_x, h.d_y, h.d_buf[BUF_YN], h.ps, h.ld, h.ny, h.nx, s->d_err, s->ctx->stream));
@@ -1018,25 +1032,18 @@ int sbp_system_rhs(sbp_system s, double t, double* k_host) {
       alloc = true;
     }
   if (alloc) s->dirty = true;
+  SBP_TRY(normalize(s));
   SBP_TRY(rebuild(s));
   SBP_TRY_HIP(hipMemcpyAsync(s->d_time_rhs, &t, sizeof(double), hipMemcpyHostToDevice, st));
+  StageArgs a = stage_args(s, 0.0, 0, 0);
+  a.time = s->d_time_rhs;
+  a.rk_scale_acc = a.rk_scale_next = 0.0;
   for (const Group& g : s->groups) {
-    StageArgs a{};
-    a.blocks = s->d_blocks;
-    a.tiles = s->d_tiles + g.tile_offset;
-    a.time = s->d_time_rhs;
-    a.error = s->d_err;
-    a.gamma = gamma;
-    SBP_TRY_HIP(launch_stage(g.op_eta, g.op_xi, g.upwind, g.shape, MODE_RHS, a, g.ntiles, st));
+    StageArgs ag = a;
+    ag.tiles = s->d_tiles + g.tile_offset;
+    SBP_TRY_HIP(launch_stage(g.op_eta, g.op_xi, g.upwind, g.shape, MODE_RHS, ag, g.ntiles, st));
   }
-  {
-    StageArgs a{};
-    a.blocks = s->d_blocks;
-    a.time = s->d_time_rhs;
-    a.error = s->d_err;
-    a.gamma = gamma;
-    SBP_TRY_HIP(launch_sat_edges(MODE_RHS, a, (int)s->blocks.size(), max_perimeter(s), st));
-  }
+  SBP_TRY_HIP(launch_sat_edges(MODE_RHS, a, (int)s->blocks.size(), max_perimeter(s), st));
   size_t off = 0;
   for (auto& h : s->blocks) {
     for (int c = 0; c < 4; ++c)
@@ -1082,6 +1089,7 @@ int sbp_system_max_dt(sbp_system s, double* out) {
   }
   const double c_max = is_h2 ? 0.5 : 1.0;
   double max_dt = INFINITY;
+  SBP_TRY(normalize(s));
   hipStream_t st = s->ctx->stream;
   for (auto& h : s->blocks) {
     const int nwg = reduce_max_workgroups(h.ny, h.nx);
@@ -1161,40 +1169,29 @@ int sbp_system_profile(sbp_system s, double dt, int64_t nsteps, double* ms_total
   *ms_total = ms;
   // per-launch durations of the stage kernels: up to 8 more steps with every
   // stage launch bracketed by its own events
-  static const double A_next[3] = {0.5, 0.5, 1.0};
-  static const double Bw[4] = {1.0 / 6.0, 1.0 / 3.0, 1.0 / 3.0, 1.0 / 6.0};
-  static const double C[3] = {0.5, 0.5, 1.0};
   std::vector<hipEvent_t> ev;
   double sum = 0.0;
   int nl = 0;
   const int64_t tsteps = std::min<int64_t>(nsteps, 8);
   for (int64_t step = 0; step < tsteps; ++step) {
-  for (int stage = 0; stage < 4; ++stage)
-    for (const Group& g : s->groups) {
-      hipEvent_t a0, a1;
-      SBP_TRY_HIP(hipEventCreate(&a0));
-      SBP_TRY_HIP(hipEventCreate(&a1));
-      StageArgs a{};
-      a.blocks = s->d_blocks;
-      a.tiles = s->d_tiles + g.tile_offset;
-      a.time = s->d_time;
-      a.error = s->d_err;
-      a.dt = dt;
-      a.gamma = gamma;
-      a.rk_scale_acc = Bw[stage] * dt;
-      a.rk_scale_next = stage < 3 ? A_next[stage] * dt : 0.0;
-      a.time_offset = stage == 0 ? 0.0 : dt * C[stage - 1];
-      SBP_TRY_HIP(hipEventRecord(a0, st));
-      SBP_TRY_HIP(launch_stage(g.op_eta, g.op_xi, g.upwind, g.shape, stage, a, g.ntiles, st));
-      SBP_TRY_HIP(hipEventRecord(a1, st));
-      ev.push_back(a0);
-      ev.push_back(a1);
-      if (&g == &s->groups.back()) {
-        StageArgs sa = stage_args(s, dt, stage);
-        SBP_TRY_HIP(launch_sat_edges(stage, sa, (int)s->blocks.size(), max_perimeter(s), st));
+    for (int stage = 0; stage < 4; ++stage) {
+      const StageArgs a = stage_args(s, dt, stage, s->parity);
+      for (const Group& g : s->groups) {
+        hipEvent_t a0, a1;
+        SBP_TRY_HIP(hipEventCreate(&a0));
+        SBP_TRY_HIP(hipEventCreate(&a1));
+        StageArgs ag = a;
+        ag.tiles = s->d_tiles + g.tile_offset;
+        SBP_TRY_HIP(hipEventRecord(a0, st));
+        SBP_TRY_HIP(launch_stage(g.op_eta, g.op_xi, g.upwind, g.shape, stage, ag, g.ntiles, st));
+        SBP_TRY_HIP(hipEventRecord(a1, st));
+        ev.push_back(a0);
+        ev.push_back(a1);
       }
+      SBP_TRY_HIP(launch_sat_edges(stage, a, (int)s->blocks.size(), max_perimeter(s), st));
     }
-  SBP_TRY_HIP(launch_advance_time(s->d_time, dt, st));
+    SBP_TRY_HIP(launch_advance_time(s->d_time, dt, st));
+    if (s->precision == SBP_MIXED) s->parity ^= 1;
   }
   SBP_TRY_HIP(hipStreamSynchronize(st));
   for (size_t i = 0; i < ev.size(); i += 2) {

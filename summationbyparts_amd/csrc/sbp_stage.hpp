@@ -1,4 +1,4 @@
-// The fused Euler RK4 stage kernel (FP64).
+// The fused Euler RK4 stage kernel (FP64, and the mixed FP32-RHS form).
 //
 // One workgroup computes the right-hand side of one TX x TY tile of one block
 // and folds it straight into the RK4 stage combination, replacing in a single
@@ -17,16 +17,22 @@
 //   η-phase: the same over the tile columns widened by the η halo, with
 //     (J·ηx, J·ηy), and η-stencils.
 //   finish: k = (idx_ξ·Sξ + idx_η·Sη)/J, RK4 combination, store.
+// Mixed form (MIX): the face terms are FP32 deviations from a per-tile
+// reference state (face_terms_mix), LDS and stencil sums are FP32, and the
+// stage combination, accumulator and y_n stay FP64; the next stage input is
+// stored as the FP32 increment Δ = a·dt·k.
 // The characteristic SATs of the edge points run afterwards in
 // sat_edge_kernel: the stage combination is linear in k, so the SAT term
 // enters as acc += b·dt·sat and next += a·dt·sat. Keeping the 4x4
 // characteristic algebra out of this kernel keeps its register count low.
 #pragma once
+#include <type_traits>
+
 #include "sbp_common.hpp"
 
 namespace sbp {
 
-template <class OX, class OY, int TX, int TY, int RX, int RY, int VAR = 0>
+template <class OX, class OY, int TX, int TY, int RX, int RY, int VAR = 0, int VBYTES = 8>
 struct StageGeom {
   static constexpr int NT = (TX / RX) * (TY / RY);
   // values per point held in LDS at once: 4 (two passes) or 8 (one pass)
@@ -39,23 +45,24 @@ struct StageGeom {
   static constexpr int NP = NPX > NPY ? NPX : NPY;
   static constexpr int ITX = (NPX + NT - 1) / NT;
   static constexpr int ITY = (NPY + NT - 1) / NT;
-  static constexpr int LDS_BYTES = NV * NP * 8;
+  static constexpr int LDS_BYTES = NV * NP * VBYTES;
   static_assert(RX == 1 || RX == 2, "RX is 1 or 2");
   static_assert(TX % 2 == 0 && NP % 2 == 0, "16-byte LDS alignment of paired reads");
 };
 
 // Interior stencils of R consecutive outputs from one window of R + D - 1
-// values: along ξ the window is contiguous (read in 16-byte pairs when
-// R == 2), along η it is a column of pairs (two adjacent ξ-columns).
-template <class B, int R>
-__device__ __forceinline__ void row_window(const double* p, double out[R]) {
+// values: along ξ the window is contiguous (read in pairs when R == 2),
+// along η it is a column of pairs (two adjacent ξ-columns).
+template <class B, int R, typename T>
+__device__ __forceinline__ void row_window(const T* p, T out[R]) {
   constexpr int W = R + B::D - 1;
-  double w[W + 1];
+  T w[W + 1];
   if constexpr (R == 2) {
-    const double2* p2 = reinterpret_cast<const double2*>(p);
+    using T2 = std::conditional_t<sizeof(T) == 8, double2, float2>;
+    const T2* p2 = reinterpret_cast<const T2*>(p);
 #pragma unroll
     for (int k = 0; k < (W + 1) / 2; ++k) {
-      const double2 v = p2[k];
+      const T2 v = p2[k];
       w[2 * k] = v.x;
       w[2 * k + 1] = v.y;
     }
@@ -65,26 +72,27 @@ __device__ __forceinline__ void row_window(const double* p, double out[R]) {
   }
 #pragma unroll
   for (int r = 0; r < R; ++r) {
-    double s = 0.0;
+    T s = T(0);
 #pragma unroll
     for (int d = 0; d < B::D; ++d)
-      if (B::diag(d) != 0.0) s += B::diag(d) * w[r + d];
+      if (B::diag(d) != 0.0) s += T(B::diag(d)) * w[r + d];
     out[r] = s;
   }
 }
 
-template <class B, int RS, int RX>
-__device__ __forceinline__ void col_window(const double* p, int stride, double out[RS][RX]) {
+template <class B, int RS, int RX, typename T>
+__device__ __forceinline__ void col_window(const T* p, int stride, T out[RS][RX]) {
   constexpr int W = RS + B::D - 1;
 #pragma unroll
   for (int i = 0; i < RX; ++i)
 #pragma unroll
-    for (int r = 0; r < RS; ++r) out[r][i] = 0.0;
+    for (int r = 0; r < RS; ++r) out[r][i] = T(0);
 #pragma unroll
   for (int k = 0; k < W; ++k) {
-    double v[RX];
+    T v[RX];
     if constexpr (RX == 2) {
-      const double2 t = *reinterpret_cast<const double2*>(p + k * stride);
+      using T2 = std::conditional_t<sizeof(T) == 8, double2, float2>;
+      const T2 t = *reinterpret_cast<const T2*>(p + k * stride);
       v[0] = t.x;
       v[1] = t.y;
     } else {
@@ -95,7 +103,7 @@ __device__ __forceinline__ void col_window(const double* p, int stride, double o
       const int d = k - r;
       if (d >= 0 && d < B::D && B::diag(d) != 0.0) {
 #pragma unroll
-        for (int i = 0; i < RX; ++i) out[r][i] += B::diag(d) * v[i];
+        for (int i = 0; i < RX; ++i) out[r][i] += T(B::diag(d)) * v[i];
       }
     }
   }
@@ -104,16 +112,19 @@ __device__ __forceinline__ void col_window(const double* p, int stride, double o
 // Variants (autotuned): VAR 0 keeps the dissipation terms α·q of a thread's
 // region points in registers between the flux and dissipation passes; VAR 1
 // keeps only α and re-reads q (cache hits); VAR 2 stores flux and α·q
-// together (8 values per point in LDS, one pass per direction).
+// together (8 values per point in LDS, one pass per direction). The mixed
+// form uses VAR 0.
 // Threads own RX adjacent ξ-columns and RY consecutive η-rows of the tile.
-template <class OX, class OY, bool UP, int TX, int TY, int RX, int RY, int VAR>
+template <class OX, class OY, bool UP, int TX, int TY, int RX, int RY, int VAR, bool MIX = false>
 __global__ __launch_bounds__((TX / RX) * (TY / RY)) void stage_kernel(const StageArgs A, const int mode) {
-  using G = StageGeom<OX, OY, TX, TY, RX, RY, VAR>;
+  static_assert(!MIX || VAR == 0, "the mixed form caches α·q in registers");
+  using CT = std::conditional_t<MIX, float, double>;  // face terms, LDS, stencil sums
+  using G = StageGeom<OX, OY, TX, TY, RX, RY, VAR, sizeof(CT)>;
   constexpr int NT = G::NT, HX = G::HX, HY = G::HY, WX = G::WX, NP = G::NP;
   constexpr int NV = UP ? G::NV : 4;
   constexpr int NC = VAR == 0 ? 4 : 1;  // cached values per region point between passes
   constexpr int LCH = 3;                // region points whose loads are in flight together
-  __shared__ __align__(16) double lds[NV * NP];
+  __shared__ __align__(16) CT lds[NV * NP];
 
   const TileDev T = A.tiles[blockIdx.x];
   const BlockDev& B = A.blocks[T.block];
@@ -123,7 +134,10 @@ __global__ __launch_bounds__((TX / RX) * (TY / RY)) void stage_kernel(const Stag
   int x0, wx0, wx1, y0, wy0, wy1;
   tile_place(T.tx, B.ntx, nx, TX, OX::M, x0, wx0, wx1);
   tile_place(T.ty, B.nty, ny, TY, OY::M, y0, wy0, wy1);
-  const double* __restrict__ in = stage_input(B, mode);
+  // stage input: FP64 planes, plus the FP32 increment planes in the mixed form
+  const bool mixs = MIX && mode >= MODE_S0;
+  const double* __restrict__ in = mixs ? yn_buf(B, A) : stage_input(B, mode);
+  const float* __restrict__ din = (MIX && mode >= MODE_S1) ? delta_in(B, mode) : nullptr;
   const double* __restrict__ met = B.metrics;
   const double gamma = A.gamma;
   const double idx_xi = B.idx_xi, idx_eta = B.idx_eta;
@@ -147,7 +161,21 @@ __global__ __launch_bounds__((TX / RX) * (TY / RY)) void stage_kernel(const Stag
 #pragma unroll
     for (int s = 0; s < RY; ++s) any = any || (xin[i] && yin[s]);
 
-  double acc[RY][RX][4];
+  // state of the stage input at offset o (FP64; the mixed form adds Δ)
+  auto load_q = [&](uint32_t o, double q[4]) {
+#pragma unroll
+    for (int c = 0; c < 4; ++c) q[c] = ldg(in, c * ps + o);
+    if constexpr (MIX) {
+      if (din) {
+#pragma unroll
+        for (int c = 0; c < 4; ++c) q[c] += (double)ldg(din, c * ps + o);
+      }
+    }
+  };
+  // reference state of the mixed form: the tile's centre point
+  const uint32_t oc = (uint32_t)(min(y0 + TY / 2, ny - 1) * ld + min(x0 + TX / 2, nx - 1));
+
+  CT acc[RY][RX][4];
 
   // ------------------------------------------------------------ ξ-phase
   {
@@ -155,7 +183,13 @@ __global__ __launch_bounds__((TX / RX) * (TY / RY)) void stage_kernel(const Stag
     const int rx0 = max(0, x0 - HX), rx1 = min(nx, x0 + TX + HX);
     const int xw = rx1 - rx0;
     const bool interior = x0 >= OX::M && x0 + TX <= nx - OX::M;
-    double cache[G::ITX][NC];
+    MixRef R;
+    if constexpr (MIX) {
+      double qc[4];
+      load_q(oc, qc);
+      R = mix_ref(qc, ldg(met, ps + oc), ldg(met, 2 * ps + oc), gamma);
+    }
+    CT cache[G::ITX][NC];
     // loads of up to LCH region points are issued together (invalid slots
     // read a clamped in-bounds point) so that their latencies overlap
 #pragma unroll
@@ -167,10 +201,7 @@ __global__ __launch_bounds__((TX / RX) * (TY / RY)) void stage_kernel(const Stag
         const int r = l / WX, c = l - r * WX;
         const bool ok = i0 + j < G::ITX && l < G::NPX && c < xw && y0 + r < yend;
         const uint32_t o = ok ? (uint32_t)((y0 + r) * ld + rx0 + c) : (uint32_t)(y0 * ld + rx0);
-        v[j][0] = ldg(in, o);
-        v[j][1] = ldg(in, ps + o);
-        v[j][2] = ldg(in, 2 * ps + o);
-        v[j][3] = ldg(in, 3 * ps + o);
+        load_q(o, v[j]);
         v[j][4] = ldg(met, ps + o);
         v[j][5] = ldg(met, 2 * ps + o);
       }
@@ -180,8 +211,13 @@ __global__ __launch_bounds__((TX / RX) * (TY / RY)) void stage_kernel(const Stag
         const int l = tid + it * NT;
         const int r = l / WX, c = l - r * WX;
         if (it < G::ITX && l < G::NPX && c < xw && y0 + r < yend) {
-          double f[4], aq[4], al;
-          face_terms<UP>(v[j][0], v[j][1], v[j][2], v[j][3], v[j][4], v[j][5], gamma, f, aq, al, err);
+          CT f[4], aq[4];
+          double al = 0.0;
+          if constexpr (MIX) {
+            face_terms_mix<UP>(v[j], v[j][4], v[j][5], R, (float)gamma, f, aq, err);
+          } else {
+            face_terms<UP>(v[j][0], v[j][1], v[j][2], v[j][3], v[j][4], v[j][5], gamma, f, aq, al, err);
+          }
 #pragma unroll
           for (int q = 0; q < 4; ++q) lds[q * NP + l] = f[q];
           if constexpr (UP) {
@@ -201,28 +237,28 @@ __global__ __launch_bounds__((TX / RX) * (TY / RY)) void stage_kernel(const Stag
     __syncthreads();
     // ξ-stencil of one LDS slot group into acc with sign sg
     // acc = sg·idx_ξ·Σ (first pass) or acc += sg·idx_ξ·Σ
-    auto xi_pass = [&](auto blk, int slot, double sg, bool first) {
+    auto xi_pass = [&](auto blk, int slot, CT sg, bool first) {
       using BB = decltype(blk);
 #pragma unroll
       for (int s = 0; s < RY; ++s) {
         if (!(yin[s] && any)) continue;
-        const double* line = lds + (size_t)(tr * RY + s) * WX;
+        const CT* line = lds + (size_t)(tr * RY + s) * WX;
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
-          const double* lq = line + (slot + q) * NP;
-          double o[RX];
+          const CT* lq = line + (slot + q) * NP;
+          CT o[RX];
           if (interior) {
             row_window<BB, RX>(lq + (xb - rx0 - HX), o);
           } else {
 #pragma unroll
-            for (int i = 0; i < RX; ++i) o[i] = xin[i] ? row_sum<BB>(lq, 1, xb + i, nx, rx0) : 0.0;
+            for (int i = 0; i < RX; ++i) o[i] = xin[i] ? row_sum<BB>(lq, 1, xb + i, nx, rx0) : CT(0);
           }
 #pragma unroll
-          for (int i = 0; i < RX; ++i) acc[s][i][q] = first ? sg * o[i] : __builtin_fma(sg, o[i], acc[s][i][q]);
+          for (int i = 0; i < RX; ++i) acc[s][i][q] = first ? sg * o[i] : fma_t(sg, o[i], acc[s][i][q]);
         }
       }
     };
-    xi_pass(BD{}, 0, -idx_xi, true);
+    xi_pass(BD{}, 0, CT(-idx_xi), true);
     if constexpr (UP) {
       using BS = Blk<OX, SBP_DISS>;
       if constexpr (VAR != 2) {
@@ -243,9 +279,9 @@ __global__ __launch_bounds__((TX / RX) * (TY / RY)) void stage_kernel(const Stag
           }
         }
         __syncthreads();
-        xi_pass(BS{}, 0, idx_xi, false);
+        xi_pass(BS{}, 0, CT(idx_xi), false);
       } else {
-        xi_pass(BS{}, 4, idx_xi, false);
+        xi_pass(BS{}, 4, CT(idx_xi), false);
       }
     }
     __syncthreads();
@@ -253,8 +289,11 @@ __global__ __launch_bounds__((TX / RX) * (TY / RY)) void stage_kernel(const Stag
 
   // Finish operands at the output points (J, and y_n / accumulator as the
   // stage combination needs them), all loaded before any store.
+  //   FP64:  S0 fa = y_n;  S1, S2 fa = y_n, fb = acc;  S3 fa = acc
+  //   mixed: S0 fa = y_n;  S1, S2 fb = acc;            S3 fa = acc
   double fJ[RY][RX], fa[RY][RX][4], fb[RY][RX][4];
   auto load_finish = [&]() {
+    const double* pacc = MIX ? acc_buf(B, A) : B.buf[BUF_ACC];
 #pragma unroll
     for (int s = 0; s < RY; ++s)
 #pragma unroll
@@ -262,12 +301,14 @@ __global__ __launch_bounds__((TX / RX) * (TY / RY)) void stage_kernel(const Stag
         const uint32_t o = (xin[i] && yin[s]) ? (uint32_t)((yb + s) * ld + xb + i) : (uint32_t)(y0 * ld + x0);
         fJ[s][i] = ldg(met, o);
         if (mode == MODE_RHS) continue;
-        const double* pa = mode == MODE_S0 ? in : (mode == MODE_S3 ? B.buf[BUF_ACC] : B.buf[BUF_YN]);
+        if (mode == MODE_S0 || mode == MODE_S3 || !MIX) {
+          const double* pa = mode == MODE_S0 ? in : (mode == MODE_S3 ? pacc : B.buf[BUF_YN]);
 #pragma unroll
-        for (int q = 0; q < 4; ++q) fa[s][i][q] = ldg_nt(pa, q * ps + o);
+          for (int q = 0; q < 4; ++q) fa[s][i][q] = ldg_nt(pa, q * ps + o);
+        }
         if (mode == MODE_S1 || mode == MODE_S2) {
 #pragma unroll
-          for (int q = 0; q < 4; ++q) fb[s][i][q] = ldg_nt((const double*)B.buf[BUF_ACC], q * ps + o);
+          for (int q = 0; q < 4; ++q) fb[s][i][q] = ldg_nt(pacc, q * ps + o);
         }
       }
   };
@@ -279,7 +320,13 @@ __global__ __launch_bounds__((TX / RX) * (TY / RY)) void stage_kernel(const Stag
     const int yh = ry1 - ry0;
     const int xend = min(x0 + TX, nx);
     const bool interior = y0 >= OY::M && y0 + TY <= ny - OY::M;
-    double cache[G::ITY][NC];
+    MixRef R;
+    if constexpr (MIX) {
+      double qc[4];
+      load_q(oc, qc);
+      R = mix_ref(qc, ldg(met, 3 * ps + oc), ldg(met, 4 * ps + oc), gamma);
+    }
+    CT cache[G::ITY][NC];
 #pragma unroll
     for (int i0 = 0; i0 < G::ITY; i0 += LCH) {
       double v[LCH][6];
@@ -289,10 +336,7 @@ __global__ __launch_bounds__((TX / RX) * (TY / RY)) void stage_kernel(const Stag
         const int r = l / TX, c = l - r * TX;
         const bool ok = i0 + j < G::ITY && l < G::NPY && r < yh && x0 + c < xend;
         const uint32_t o = ok ? (uint32_t)((ry0 + r) * ld + x0 + c) : (uint32_t)(ry0 * ld + x0);
-        v[j][0] = ldg(in, o);
-        v[j][1] = ldg(in, ps + o);
-        v[j][2] = ldg(in, 2 * ps + o);
-        v[j][3] = ldg(in, 3 * ps + o);
+        load_q(o, v[j]);
         v[j][4] = ldg(met, 3 * ps + o);
         v[j][5] = ldg(met, 4 * ps + o);
       }
@@ -302,8 +346,13 @@ __global__ __launch_bounds__((TX / RX) * (TY / RY)) void stage_kernel(const Stag
         const int l = tid + it * NT;
         const int r = l / TX, c = l - r * TX;
         if (it < G::ITY && l < G::NPY && r < yh && x0 + c < xend) {
-          double f[4], aq[4], al;
-          face_terms<UP>(v[j][0], v[j][1], v[j][2], v[j][3], v[j][4], v[j][5], gamma, f, aq, al, err);
+          CT f[4], aq[4];
+          double al = 0.0;
+          if constexpr (MIX) {
+            face_terms_mix<UP>(v[j], v[j][4], v[j][5], R, (float)gamma, f, aq, err);
+          } else {
+            face_terms<UP>(v[j][0], v[j][1], v[j][2], v[j][3], v[j][4], v[j][5], gamma, f, aq, al, err);
+          }
 #pragma unroll
           for (int q = 0; q < 4; ++q) lds[q * NP + l] = f[q];
           if constexpr (UP) {
@@ -322,13 +371,13 @@ __global__ __launch_bounds__((TX / RX) * (TY / RY)) void stage_kernel(const Stag
     }
     __syncthreads();
     // acc += sg·idx_η·Σ
-    auto eta_pass = [&](auto blk, int slot, double sg) {
+    auto eta_pass = [&](auto blk, int slot, CT sg) {
       using BB = decltype(blk);
       if (!any) return;
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
-        const double* col = lds + (slot + q) * NP + tc * RX;
-        double o[RY][RX];
+        const CT* col = lds + (slot + q) * NP + tc * RX;
+        CT o[RY][RX];
         if (interior) {
           col_window<BB, RY, RX>(col + (yb - ry0 - HY) * TX, TX, o);
         } else {
@@ -336,15 +385,15 @@ __global__ __launch_bounds__((TX / RX) * (TY / RY)) void stage_kernel(const Stag
           for (int s = 0; s < RY; ++s)
 #pragma unroll
             for (int i = 0; i < RX; ++i)
-              o[s][i] = (yin[s] && xin[i]) ? row_sum<BB>(col + i, TX, yb + s, ny, ry0) : 0.0;
+              o[s][i] = (yin[s] && xin[i]) ? row_sum<BB>(col + i, TX, yb + s, ny, ry0) : CT(0);
         }
 #pragma unroll
         for (int s = 0; s < RY; ++s)
 #pragma unroll
-          for (int i = 0; i < RX; ++i) acc[s][i][q] = __builtin_fma(sg, o[s][i], acc[s][i][q]);
+          for (int i = 0; i < RX; ++i) acc[s][i][q] = fma_t(sg, o[s][i], acc[s][i][q]);
       }
     };
-    eta_pass(BD{}, 0, -idx_eta);
+    eta_pass(BD{}, 0, CT(-idx_eta));
     if constexpr (UP) {
       using BS = Blk<OY, SBP_DISS>;
       if constexpr (VAR != 2) {
@@ -365,9 +414,9 @@ __global__ __launch_bounds__((TX / RX) * (TY / RY)) void stage_kernel(const Stag
           }
         }
         __syncthreads();
-        eta_pass(BS{}, 0, idx_eta);
+        eta_pass(BS{}, 0, CT(idx_eta));
       } else {
-        eta_pass(BS{}, 4, idx_eta);
+        eta_pass(BS{}, 4, CT(idx_eta));
       }
     }
   }
@@ -385,10 +434,25 @@ __global__ __launch_bounds__((TX / RX) * (TY / RY)) void stage_kernel(const Stag
     const double ij = frcp(fJ[s][i]);
     double k[4];
 #pragma unroll
-    for (int q = 0; q < 4; ++q) k[q] = acc[s][i][q] * ij;
+    for (int q = 0; q < 4; ++q) k[q] = (double)acc[s][i][q] * ij;
     if (mode == MODE_RHS) {
 #pragma unroll
       for (int q = 0; q < 4; ++q) stg_nt(B.rhs_out, q * ps + o, k[q]);
+    } else if constexpr (MIX) {
+      double* accb = acc_buf(B, A);
+      if (mode == MODE_S3) {
+        // y_{n+1} over the accumulator (the roles swap after the step)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) stg_nt(accb, q * ps + o, fa[s][i][q] + A.rk_scale_acc * k[q]);
+      } else {
+        float* dn = delta_out(B, mode);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const double a0 = mode == MODE_S0 ? fa[s][i][q] : fb[s][i][q];
+          stg_nt(accb, q * ps + o, a0 + A.rk_scale_acc * k[q]);
+          stg_nt(dn, q * ps + o, (float)(A.rk_scale_next * k[q]));
+        }
+      }
     } else if (mode == MODE_S0) {
       double* accb = B.buf[BUF_ACC];
       double* nxt = B.buf[BUF_YA];

@@ -18,20 +18,22 @@ namespace sbp {
 
 // Kernel configurations (tile TX x TY, RY points per thread, caching
 // variant); the list matches kStageConfigs in sbp_misc.hip.
-template <class OX, class OY, bool UP, int TX, int TY, int RX, int RY, int VAR>
+// The mixed form always caches α·q in registers (VAR 0).
+template <class OX, class OY, bool UP, bool MIX, int TX, int TY, int RX, int RY, int VAR>
 static void launch_one(int mode, const StageArgs& args, int ntiles, hipStream_t stream) {
-  stage_kernel<OX, OY, UP, TX, TY, RX, RY, VAR><<<ntiles, (TX / RX) * (TY / RY), 0, stream>>>(args, mode);
+  stage_kernel<OX, OY, UP, TX, TY, RX, RY, (MIX ? 0 : VAR), MIX>
+      <<<ntiles, (TX / RX) * (TY / RY), 0, stream>>>(args, mode);
 }
 
-template <class OX, class OY, bool UP>
+template <class OX, class OY, bool UP, bool MIX>
 static hipError_t launch_shape(int cfg, int mode, const StageArgs& args, int ntiles, hipStream_t stream) {
   constexpr bool full = std::is_same<OX, OY>::value;  // every configuration only for equal operator pairs
   switch (cfg) {
-    case 0: launch_one<OX, OY, UP, 32, 16, 1, 2, 1>(mode, args, ntiles, stream); break;
+    case 0: launch_one<OX, OY, UP, MIX, 32, 16, 1, 2, 1>(mode, args, ntiles, stream); break;
 #define SBP_CFG(N, TX, TY, RX, RY, VAR)                                                            \
   case N:                                                                                        \
     if constexpr (full) {                                                                        \
-      launch_one<OX, OY, UP, TX, TY, RX, RY, VAR>(mode, args, ntiles, stream);                   \
+      launch_one<OX, OY, UP, MIX, TX, TY, RX, RY, VAR>(mode, args, ntiles, stream);              \
       break;                                                                                     \
     }                                                                                            \
     return hipErrorNotSupported;
@@ -49,13 +51,19 @@ static hipError_t launch_shape(int cfg, int mode, const StageArgs& args, int nti
   return hipGetLastError();
 }
 
-template <class OX, class OY>
-static hipError_t launch_pair(bool up, int shape, int mode, const StageArgs& args, int ntiles, hipStream_t stream) {
+template <class OX, class OY, bool MIX>
+static hipError_t launch_pair_p(bool up, int shape, int mode, const StageArgs& args, int ntiles, hipStream_t stream) {
   if (up) {
-    if constexpr (OX::is_upwind && OY::is_upwind) return launch_shape<OX, OY, true>(shape, mode, args, ntiles, stream);
+    if constexpr (OX::is_upwind && OY::is_upwind) return launch_shape<OX, OY, true, MIX>(shape, mode, args, ntiles, stream);
     return hipErrorInvalidValue;
   }
-  return launch_shape<OX, OY, false>(shape, mode, args, ntiles, stream);
+  return launch_shape<OX, OY, false, MIX>(shape, mode, args, ntiles, stream);
+}
+
+template <class OX, class OY>
+static hipError_t launch_pair(bool up, int shape, int mode, const StageArgs& args, int ntiles, hipStream_t stream) {
+  if (args.mixed) return launch_pair_p<OX, OY, true>(up, shape, mode, args, ntiles, stream);
+  return launch_pair_p<OX, OY, false>(up, shape, mode, args, ntiles, stream);
 }
 
 hipError_t SBP_CAT(launch_stage_opxi_, SBP_OPX_ID)(int op_eta, bool up, int shape, int mode, const StageArgs& args, int ntiles,

@@ -1,0 +1,168 @@
+"""The mixed form (SBP_MIXED): FP32 fluxes, stencils and dissipation with FP64
+state, stage accumulation and metrics, checked against the FP64 path of the
+same library (itself pinned to the oracle at 1e-12 per step) and against the
+oracle directly.
+
+Tolerances (relative to the FP64 result):
+- right-hand side, relative to max|k|: MIX_RHS_TOL;
+- states after many steps: relative L2 of the whole state (all four
+  components) below 1e-8; per-component relative L2 and max errors are
+  printed (ρv, which vanishes outside the vortex, has the largest relative
+  error) and bounded by MIX_COMPONENT_TOL.
+"""
+import numpy as np
+import pytest
+
+from common import GAMMA, MACH, VORTEX, cartesian, curvilinear, density_noise, rel_global
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+MIX_RHS_TOL = 2e-5
+MIX_STATE_TOL = 1e-8
+MIX_COMPONENT_TOL = 5e-7
+
+
+def _vp_o(vortices=VORTEX, mach=MACH):
+    return O.VortexParams.make(list(vortices), mach)
+
+
+def _vp_g(vortices=VORTEX, mach=MACH):
+    from summationbyparts_amd.euler import VortexParameters, Vortice
+    return VortexParameters([Vortice(*v) for v in vortices], mach)
+
+
+def _state(X, Y, seed=1):
+    q = O.vortex(_vp_o(), 0.0, X, Y)
+    q[0] += density_noise(seed, 0, *X.shape)
+    return q
+
+
+def rel_l2(a, b):
+    """||a_c - b_c||_2 / ||b_c||_2 per component, as a list."""
+    return [float(np.linalg.norm(a[c] - b[c]) / np.linalg.norm(b[c])) for c in range(a.shape[0])]
+
+
+def rel_l2_all(a, b):
+    """||a - b||_2 / ||b||_2 over the whole state."""
+    return float(np.linalg.norm((a - b).ravel()) / np.linalg.norm(b.ravel()))
+
+
+def _system(X, Y, name, precision):
+    from summationbyparts_amd import _lib, euler, operators
+    s = euler.System(X, Y, operators.BY_NAME[name]())
+    s.set_precision(_lib.MIXED if precision == "mixed" else _lib.FP64)
+    return s
+
+
+@pytest.mark.parametrize("name,shape,kind", [
+    ("upwind9", (131, 77), O.RHS_UPWIND),
+    ("upwind9", (70, 96), O.RHS_TRAD),
+    ("upwind4", (64, 80), O.RHS_UPWIND),
+    ("sbp8", (60, 50), O.RHS_TRAD),
+    ("upwind9h2", (50, 45), O.RHS_UPWIND),
+])
+def test_mixed_rhs(gpu, name, shape, kind):
+    ny, nx = shape
+    X, Y = curvilinear(nx, ny)
+    q = _state(X, Y)
+    ref = O.System([(X, Y)], [(O.OPERATORS[name], O.OPERATORS[name])])
+    ref.set_rhs_kind(kind)
+    ref.set_state(0, q)
+    k_ref = ref.rhs(0.0)[0]
+    s = _system(X, Y, name, "mixed")
+    s.set_rhs_kind(kind)
+    s.set_state(0, q)
+    k = s.rhs(0.0)[0]
+    err = rel_global(k, k_ref)
+    print(f"mixed rhs {name} {shape} kind={kind}: {err:.3e}")
+    assert err < MIX_RHS_TOL
+
+
+def test_mixed_steps_in_chunks(gpu):
+    """Odd step counts leave y_n in the accumulator buffer; every way of
+    splitting 11 steps over calls gives the same bits."""
+    X, Y = curvilinear(96, 80)
+    q = _state(X, Y)
+    dt = 0.2 / 95
+    runs = []
+    for chunks in ([11], [1, 3, 2, 5], [2, 2, 2, 2, 2, 1], [5, 6]):
+        s = _system(X, Y, "upwind9", "mixed")
+        s.set_state(0, q)
+        for c in chunks:
+            s.advance_n(dt, c)
+        runs.append(s.field())
+        assert abs(s.time - 11 * dt) < 1e-14
+    for r in runs[1:]:
+        assert np.array_equal(r, runs[0])
+
+
+def test_mixed_state_vs_fp64(gpu):
+    """200 steps of a 512² curvilinear upwind9 vortex with density noise:
+    mixed vs FP64 per component, relative L2 and max."""
+    n = 512
+    X, Y = curvilinear(n, n)
+    q = _state(X, Y)
+    dt = 0.2 / (n - 1)
+    out = {}
+    for prec in ("fp64", "mixed"):
+        s = _system(X, Y, "upwind9", prec)
+        s.set_state(0, q)
+        s.advance_n(dt, 200)
+        out[prec] = s.field()
+    l2 = rel_l2(out["mixed"], out["fp64"])
+    mx = [float(np.max(np.abs(out["mixed"][c] - out["fp64"][c])) / np.max(np.abs(out["fp64"][c]))) for c in range(4)]
+    tot = rel_l2_all(out["mixed"], out["fp64"])
+    print("mixed vs fp64 512^2 200 steps: rel L2", tot, "per component", l2, "rel max", mx)
+    assert tot < MIX_STATE_TOL
+    assert max(l2) < MIX_COMPONENT_TOL
+
+
+def test_mixed_farfield_c1(gpu):
+    """C1 (101², SBP8, vortex far field, 100 steps) in the mixed form: the
+    analytic error agrees with the FP64 run to 3 significant digits."""
+    from summationbyparts_amd import _lib
+    n = 101
+    X, Y = cartesian(n, n)
+    errs = {}
+    for prec in ("fp64", "mixed"):
+        s = _system(X, Y, "sbp8", prec)
+        for d in range(4):
+            s.set_bc(0, d, _lib.BC_VORTEX, 0, _vp_g())
+        s.vortex(0.0, _vp_g())
+        s.advance_n(0.002, 100, upwind=False)
+        errs[prec] = s.error_vortex(_vp_g())
+    print("C1 analytic error fp64 / mixed:", errs)
+    assert abs(errs["mixed"] - errs["fp64"]) < 5e-4 * errs["fp64"]
+
+
+def test_c4_mixed_full(gpu):
+    """Config C4: 4096² curvilinear upwind9 vortex + noise, 1000 RK4 steps,
+    mixed vs FP64: relative L2 error of the state at most 1e-8, and the
+    H-norm energy drift of both runs. The curvilinear grid is not periodic
+    in ξ (x(1, η) - x(0, η) = 1 - 0.2 sin 2πη), so a periodic run of it
+    turns unstable after about 800 steps in either precision; this run uses
+    the analytic vortex as far-field data on all four edges."""
+    from summationbyparts_amd import _lib
+    n = 4096
+    X, Y = curvilinear(n, n)
+    q = _state(X, Y)
+    dt = 0.2 / (n - 1)
+    out, drift = {}, {}
+    for prec in ("fp64", "mixed"):
+        s = _system(X, Y, "upwind9", prec)
+        for d in range(4):
+            s.set_bc(0, d, _lib.BC_VORTEX, 0, _vp_g())
+        s.set_state(0, q)
+        e0 = s.energy()
+        s.advance_n(dt, 1000)
+        drift[prec] = (s.energy() - e0) / e0
+        out[prec] = s.field()
+        s.close()
+    l2 = rel_l2(out["mixed"], out["fp64"])
+    mx = [float(np.max(np.abs(out["mixed"][c] - out["fp64"][c])) / np.max(np.abs(out["fp64"][c]))) for c in range(4)]
+    tot = rel_l2_all(out["mixed"], out["fp64"])
+    print("C4 mixed vs fp64: rel L2", tot, "per component", l2, "rel max", mx, "energy drift", drift)
+    assert tot < MIX_STATE_TOL
+    assert max(l2) < MIX_COMPONENT_TOL
+    assert abs(drift["mixed"] - drift["fp64"]) < 1e-8

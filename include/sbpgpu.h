@@ -158,6 +158,14 @@ int sbp_system_state_device(sbp_system sys, int block, double** dev, int64_t* ld
 int sbp_system_get_metrics(sbp_system sys, int block, double* host);
 /* Field::vortex on every block (euler/src/lib.rs:319-328, vortex.rs:23-113). */
 int sbp_system_vortex(sbp_system sys, double t, const sbp_vortex_params* params);
+/* Field::vortex on one block: independent solves batched in one system each
+ * start from their own vortex (the reference builds one euler::System per
+ * solve, euler/src/lib.rs:29-48,76-80). */
+int sbp_system_vortex_block(sbp_system sys, int block, double t, const sbp_vortex_params* params);
+/* Block `block` steps with dt*scale and its time is t*scale (independent
+ * solves with their own step size in one batched system; each reference
+ * System::advance takes its own dt, euler/src/lib.rs:50-74). Default 1. */
+int sbp_system_set_dt_scale(sbp_system sys, int block, double scale);
 /* One RHS evaluation of the current state at time t (the closure passed to
  * integrate, multigrid/src/system.rs:508-527), k copied to k_host:
  * concatenation of each block's (4, ny, nx). */
@@ -173,6 +181,9 @@ int sbp_system_max_dt(sbp_system sys, double* out);
  * Field::h2_err (multigrid/src/system.rs:618-635; euler/src/lib.rs:377-432),
  * compensated FP64 sums on the device. */
 int sbp_system_error_vortex(sbp_system sys, const sbp_vortex_params* params, double* out);
+/* Field::h2_err of one block against the vortex at the block's time
+ * (euler/src/lib.rs:377-432; euler/tests/convergence.rs:48-51). */
+int sbp_system_error_vortex_block(sbp_system sys, int block, const sbp_vortex_params* params, double* out);
 /* ||q||_H^2 summed over blocks and components (the H-norm energy used for
  * drift reports), compensated FP64 sums on the device. */
 int sbp_system_energy(sbp_system sys, double* out);

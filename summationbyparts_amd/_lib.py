@@ -87,12 +87,15 @@ def _declare(L):
                                      ctypes.POINTER(c_int64)], c_int),
         "sbp_system_get_metrics": ([c_void_p, c_int, dp], c_int),
         "sbp_system_vortex": ([c_void_p, c_double, ctypes.POINTER(VortexParamsC)], c_int),
+        "sbp_system_vortex_block": ([c_void_p, c_int, c_double, ctypes.POINTER(VortexParamsC)], c_int),
+        "sbp_system_set_dt_scale": ([c_void_p, c_int, c_double], c_int),
         "sbp_system_rhs": ([c_void_p, c_double, dp], c_int),
         "sbp_system_advance": ([c_void_p, c_double, c_int64], c_int),
         "sbp_system_get_time": ([c_void_p, dp], c_int),
         "sbp_system_set_time": ([c_void_p, c_double], c_int),
         "sbp_system_max_dt": ([c_void_p, dp], c_int),
         "sbp_system_error_vortex": ([c_void_p, ctypes.POINTER(VortexParamsC), dp], c_int),
+        "sbp_system_error_vortex_block": ([c_void_p, c_int, ctypes.POINTER(VortexParamsC), dp], c_int),
         "sbp_system_energy": ([c_void_p, dp], c_int),
         "sbp_system_check": ([c_void_p], c_int),
         "sbp_system_profile": ([c_void_p, c_double, c_int64, dp, dp], c_int),
@@ -113,7 +116,8 @@ EXPORTED = [
     "sbp_context_synchronize", "sbp_diffxi", "sbp_diffeta", "sbp_dissxi", "sbp_disseta", "sbp_rhs", "sbp_metrics",
     "sbp_system_create", "sbp_system_destroy", "sbp_system_set_bc", "sbp_system_set_rhs_kind",
     "sbp_system_set_precision", "sbp_system_set_state", "sbp_system_get_state", "sbp_system_state_device",
-    "sbp_system_get_metrics", "sbp_system_vortex", "sbp_system_rhs", "sbp_system_advance", "sbp_system_get_time",
+    "sbp_system_get_metrics", "sbp_system_vortex", "sbp_system_vortex_block", "sbp_system_set_dt_scale",
+    "sbp_system_error_vortex_block", "sbp_system_rhs", "sbp_system_advance", "sbp_system_get_time",
     "sbp_system_set_time", "sbp_system_max_dt", "sbp_system_error_vortex", "sbp_system_energy", "sbp_system_check",
     "sbp_system_profile", "sbp_system_tile_shape", "sbp_system_set_tile_shape", "sbp_tile_shape_count",
     "sbp_tile_shape_get",

@@ -46,6 +46,7 @@ struct BlockDev {
   double* rhs_out;  // MODE_RHS output k (4 planes)
   double idx_xi, idx_eta;  // 1/dξ and 1/dη (algos.rs:125-130)
   double hi_xi, hi_eta;    // SAT factors (n-1)/H0 (lib.rs:1034-1038,1095-1099)
+  double dt_scale;         // this block steps with dt·dt_scale; its time is t·dt_scale
   int edge_src[4];         // EdgeSrc per sbp_direction
   int edge_nb[4];          // neighbour block for EDGE_GRID
   const double* edge_ptr[4];  // EDGE_EXPLICIT data
@@ -65,8 +66,8 @@ struct StageArgs {
   unsigned int* error;  // sticky error word (bit per sbp_status)
   double dt;
   double gamma;
-  double rk_scale_acc;  // b_s·dt
-  double rk_scale_next;  // a_s·dt (stage input of the next stage)
+  double rk_b;  // b_s: the stage's weight in the accumulator (b_s·dt_block)
+  double rk_a;  // a_s: weight of the next stage input (a_s·dt_block)
   double time_offset;  // c_s·dt added to *time for boundary data
   int mixed;  // FP32 right-hand side with FP64 state (SBP_MIXED)
   int swap;   // mixed form, odd steps: y_n lives in BUF_ACC and the accumulator in BUF_YN

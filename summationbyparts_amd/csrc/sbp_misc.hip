@@ -309,7 +309,9 @@ __global__ __launch_bounds__(256) void sat_edge_kernel(const StageArgs A, const 
   const uint32_t o = (uint32_t)(y * B.ld + x);
   const double* met = B.metrics;
   const double gamma = A.gamma;
-  const double time = (A.time ? *A.time : 0.0) + A.time_offset;
+  const double time = ((A.time ? *A.time : 0.0) + A.time_offset) * B.dt_scale;
+  const double dtb = A.dt * B.dt_scale;
+  const double sacc = A.rk_b * dtb, snext = A.rk_a * dtb;
   unsigned err = 0;
   double q[4], g[4], sat[4] = {0.0, 0.0, 0.0, 0.0};
   load_stage_state(A, B, mode, o, q);
@@ -337,24 +339,24 @@ __global__ __launch_bounds__(256) void sat_edge_kernel(const StageArgs A, const 
     // y_{n+1} (stage 4) or the accumulator, and the FP32 stage increment
     double* accb = acc_buf(B, A);
 #pragma unroll
-    for (int c = 0; c < 4; ++c) stg(accb, c * ps + o, ldg((const double*)accb, c * ps + o) + A.rk_scale_acc * sat[c]);
+    for (int c = 0; c < 4; ++c) stg(accb, c * ps + o, ldg((const double*)accb, c * ps + o) + sacc * sat[c]);
     if (mode != MODE_S3) {
       float* dn = delta_out(B, mode);
 #pragma unroll
       for (int c = 0; c < 4; ++c)
-        stg(dn, c * ps + o, (float)((double)ldg((const float*)dn, c * ps + o) + A.rk_scale_next * sat[c]));
+        stg(dn, c * ps + o, (float)((double)ldg((const float*)dn, c * ps + o) + snext * sat[c]));
     }
   } else if (mode == MODE_S3) {
     double* out = B.buf[BUF_YN];
 #pragma unroll
-    for (int c = 0; c < 4; ++c) stg(out, c * ps + o, ldg((const double*)out, c * ps + o) + A.rk_scale_acc * sat[c]);
+    for (int c = 0; c < 4; ++c) stg(out, c * ps + o, ldg((const double*)out, c * ps + o) + sacc * sat[c]);
   } else {
     double* accb = B.buf[BUF_ACC];
     double* nxt = B.buf[mode == MODE_S1 ? BUF_YB : BUF_YA];
 #pragma unroll
     for (int c = 0; c < 4; ++c) {
-      stg(accb, c * ps + o, ldg((const double*)accb, c * ps + o) + A.rk_scale_acc * sat[c]);
-      stg(nxt, c * ps + o, ldg((const double*)nxt, c * ps + o) + A.rk_scale_next * sat[c]);
+      stg(accb, c * ps + o, ldg((const double*)accb, c * ps + o) + sacc * sat[c]);
+      stg(nxt, c * ps + o, ldg((const double*)nxt, c * ps + o) + snext * sat[c]);
     }
   }
   if (err) record_error(A.error, err);

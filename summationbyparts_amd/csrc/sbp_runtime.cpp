@@ -171,6 +171,7 @@ struct HostBlock {
   double* d_y = nullptr;
   double* d_buf[4] = {nullptr, nullptr, nullptr, nullptr};
   double* d_k = nullptr;
+  double dt_scale = 1.0;  // the block steps with dt·dt_scale (independent solves in one batch)
   int bc_kind[4] = {SBP_BC_THIS, SBP_BC_THIS, SBP_BC_THIS, SBP_BC_THIS};
   int bc_nb[4] = {0, 0, 0, 0};
   sbp_vortex_params bc_vortex[4];
@@ -269,6 +270,7 @@ BlockDev make_dev(const sbp_system_s* s, int i, int shape) {
   b.idx_eta = 1.0 / (oe.h2 ? 1.0 / (double)(h.ny - 2) : 1.0 / (double)(h.ny - 1));
   b.hi_xi = (ox.h2 ? (double)(h.nx - 2) : (double)(h.nx - 1)) / ox.h[0];
   b.hi_eta = (oe.h2 ? (double)(h.ny - 2) : (double)(h.ny - 1)) / oe.h[0];
+  b.dt_scale = h.dt_scale;
   int e;
   b.upwind = block_upwind(s, h, &e);
   for (int d = 0; d < 4; ++d) {
@@ -323,8 +325,8 @@ StageArgs stage_args(const sbp_system_s* s, double dt, int stage, int swap) {
   a.error = s->d_err;
   a.dt = dt;
   a.gamma = gamma;
-  a.rk_scale_acc = Bw[stage] * dt;
-  a.rk_scale_next = stage < 3 ? A_next[stage] * dt : 0.0;
+  a.rk_b = Bw[stage];
+  a.rk_a = stage < 3 ? A_next[stage] : 0.0;
   a.time_offset = stage == 0 ? 0.0 : dt * C[stage - 1];
   a.mixed = s->precision == SBP_MIXED;
   a.swap = a.mixed ? swap : 0;
@@ -782,6 +784,7 @@ int sbp_rhs(sbp_context ctx, int kind, int op_eta, int op_xi, int ny, int nx, do
   b.op_xi = op_xi;
   b.op_eta = op_eta;
   b.upwind = up;
+  b.dt_scale = 1.0;
   b.ntx = ntiles_along(nx, ts.tx);
   b.nty = ntiles_along(ny, ts.ty);
   b.metrics = metrics;
@@ -1018,6 +1021,19 @@ int sbp_system_vortex(sbp_system s, double t, const sbp_vortex_params* params) {
   return check_err(s);
 }
 
+int sbp_system_vortex_block(sbp_system s, int block, double t, const sbp_vortex_params* params) {
+  if (!s || !params || block < 0 || block >= (int)s->blocks.size() || params->nvortices < 0 || params->nvortices > 5)
+    return SBP_ERR_INVALID;
+  double gamma;
+  if (!gamma_get(gamma)) return SBP_ERR_GAMMA;
+  SBP_TRY_HIP(hipSetDevice(s->ctx->device));
+  SBP_TRY(normalize(s));
+  const VortexDev vp = to_dev(*params);
+  HostBlock& h = s->blocks[block];
+  SBP_TRY_HIP(launch_vortex(vp, gamma, t, h.d_x, h.d_y, h.d_buf[BUF_YN], h.ps, h.ld, h.ny, h.nx, s->d_err, s->ctx->stream));
+  return check_err(s);
+}
+
 int sbp_system_rhs(sbp_system s, double t, double* k_host) {
   if (!s || !k_host) return SBP_ERR_INVALID;
   double gamma;
@@ -1037,7 +1053,7 @@ int sbp_system_rhs(sbp_system s, double t, double* k_host) {
   SBP_TRY_HIP(hipMemcpyAsync(s->d_time_rhs, &t, sizeof(double), hipMemcpyHostToDevice, st));
   StageArgs a = stage_args(s, 0.0, 0, 0);
   a.time = s->d_time_rhs;
-  a.rk_scale_acc = a.rk_scale_next = 0.0;
+  a.rk_b = a.rk_a = 0.0;
   for (const Group& g : s->groups) {
     StageArgs ag = a;
     ag.tiles = s->d_tiles + g.tile_offset;
@@ -1121,10 +1137,32 @@ int sbp_system_error_vortex(sbp_system s, const sbp_vortex_params* params, doubl
   double e = 0.0;
   for (int i = 0; i < (int)s->blocks.size(); ++i) {
     long double sum;
-    SBP_TRY(h2_sum(s, i, nullptr, &vp, t, sum));
+    SBP_TRY(h2_sum(s, i, nullptr, &vp, t * s->blocks[i].dt_scale, sum));
     e += std::sqrt((double)sum);
   }
   *out = e;
+  return SBP_OK;
+}
+
+int sbp_system_error_vortex_block(sbp_system s, int block, const sbp_vortex_params* params, double* out) {
+  if (!s || !params || !out || block < 0 || block >= (int)s->blocks.size() || params->nvortices < 0 ||
+      params->nvortices > 5)
+    return SBP_ERR_INVALID;
+  if (double g; !gamma_get(g)) return SBP_ERR_GAMMA;
+  SBP_TRY_HIP(hipSetDevice(s->ctx->device));
+  double t;
+  SBP_TRY(sbp_system_get_time(s, &t));
+  const VortexDev vp = to_dev(*params);
+  long double sum;
+  SBP_TRY(h2_sum(s, block, nullptr, &vp, t * s->blocks[block].dt_scale, sum));
+  *out = std::sqrt((double)sum);
+  return SBP_OK;
+}
+
+int sbp_system_set_dt_scale(sbp_system s, int block, double scale) {
+  if (!s || block < 0 || block >= (int)s->blocks.size() || !(scale > 0.0) || !std::isfinite(scale)) return SBP_ERR_INVALID;
+  s->blocks[block].dt_scale = scale;
+  s->dirty = true;
   return SBP_OK;
 }
 

@@ -425,6 +425,9 @@ __global__ __launch_bounds__((TX / RX) * (TY / RY)) void stage_kernel(const Stag
   // (the edge SATs follow in sat_edge_kernel, added through the same linear
   // stage combination)
   load_finish();
+  // (b_s·dt)·k and (a_s·dt)·k with this block's step (integrate, lib.rs:188-211)
+  const double dtb = A.dt * B.dt_scale;
+  const double sacc = A.rk_b * dtb, snext = A.rk_a * dtb;
 #pragma unroll
   for (int s = 0; s < RY; ++s)
 #pragma unroll
@@ -443,14 +446,14 @@ __global__ __launch_bounds__((TX / RX) * (TY / RY)) void stage_kernel(const Stag
       if (mode == MODE_S3) {
         // y_{n+1} over the accumulator (the roles swap after the step)
 #pragma unroll
-        for (int q = 0; q < 4; ++q) stg_nt(accb, q * ps + o, fa[s][i][q] + A.rk_scale_acc * k[q]);
+        for (int q = 0; q < 4; ++q) stg_nt(accb, q * ps + o, fa[s][i][q] + sacc * k[q]);
       } else {
         float* dn = delta_out(B, mode);
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
           const double a0 = mode == MODE_S0 ? fa[s][i][q] : fb[s][i][q];
-          stg_nt(accb, q * ps + o, a0 + A.rk_scale_acc * k[q]);
-          stg_nt(dn, q * ps + o, (float)(A.rk_scale_next * k[q]));
+          stg_nt(accb, q * ps + o, a0 + sacc * k[q]);
+          stg_nt(dn, q * ps + o, (float)(snext * k[q]));
         }
       }
     } else if (mode == MODE_S0) {
@@ -458,20 +461,20 @@ __global__ __launch_bounds__((TX / RX) * (TY / RY)) void stage_kernel(const Stag
       double* nxt = B.buf[BUF_YA];
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
-        stg_nt(accb, q * ps + o, fa[s][i][q] + A.rk_scale_acc * k[q]);
-        stg_nt(nxt, q * ps + o, fa[s][i][q] + A.rk_scale_next * k[q]);
+        stg_nt(accb, q * ps + o, fa[s][i][q] + sacc * k[q]);
+        stg_nt(nxt, q * ps + o, fa[s][i][q] + snext * k[q]);
       }
     } else if (mode == MODE_S3) {
       double* out = B.buf[BUF_YN];
 #pragma unroll
-      for (int q = 0; q < 4; ++q) stg_nt(out, q * ps + o, fa[s][i][q] + A.rk_scale_acc * k[q]);
+      for (int q = 0; q < 4; ++q) stg_nt(out, q * ps + o, fa[s][i][q] + sacc * k[q]);
     } else {
       double* accb = B.buf[BUF_ACC];
       double* nxt = B.buf[mode == MODE_S1 ? BUF_YB : BUF_YA];
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
-        stg_nt(accb, q * ps + o, fb[s][i][q] + A.rk_scale_acc * k[q]);
-        stg_nt(nxt, q * ps + o, fa[s][i][q] + A.rk_scale_next * k[q]);
+        stg_nt(accb, q * ps + o, fb[s][i][q] + sacc * k[q]);
+        stg_nt(nxt, q * ps + o, fa[s][i][q] + snext * k[q]);
       }
     }
   }

@@ -134,9 +134,17 @@ class BlockSystem:
                    "sbp_system_get_metrics")
         return out
 
-    def vortex(self, t, params):
+    def vortex(self, t, params, block=None):
+        """Field::vortex on every block, or on one block."""
         p = params.c()
-        _lib.check(_lib.lib().sbp_system_vortex(self._h, t, ctypes.byref(p)), "sbp_system_vortex")
+        if block is None:
+            _lib.check(_lib.lib().sbp_system_vortex(self._h, t, ctypes.byref(p)), "sbp_system_vortex")
+        else:
+            _lib.check(_lib.lib().sbp_system_vortex_block(self._h, block, t, ctypes.byref(p)), "sbp_system_vortex_block")
+
+    def set_dt_scale(self, block, scale):
+        """Block `block` steps with dt·scale; its time is t·scale."""
+        _lib.check(_lib.lib().sbp_system_set_dt_scale(self._h, block, scale), "sbp_system_set_dt_scale")
 
     # ---- stepping
     def rhs(self, t):
@@ -168,11 +176,16 @@ class BlockSystem:
         _lib.check(_lib.lib().sbp_system_max_dt(self._h, ctypes.byref(out)), "sbp_system_max_dt")
         return out.value
 
-    def error_vortex(self, params):
+    def error_vortex(self, params, block=None):
+        """Σ over blocks (or one block) of Field::h2_err against the vortex."""
         out = ctypes.c_double()
         p = params.c()
-        _lib.check(_lib.lib().sbp_system_error_vortex(self._h, ctypes.byref(p), ctypes.byref(out)),
-                   "sbp_system_error_vortex")
+        if block is None:
+            _lib.check(_lib.lib().sbp_system_error_vortex(self._h, ctypes.byref(p), ctypes.byref(out)),
+                       "sbp_system_error_vortex")
+        else:
+            _lib.check(_lib.lib().sbp_system_error_vortex_block(self._h, block, ctypes.byref(p), ctypes.byref(out)),
+                       "sbp_system_error_vortex_block")
         return out.value
 
     def energy(self):

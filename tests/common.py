@@ -53,6 +53,19 @@ def rel_maxnorm(a, b):
     return max(float(np.max(np.abs(a[c] - b[c])) / max(np.max(np.abs(b[c])), 1e-300)) for c in range(a.shape[0]))
 
 
+def rel_maxnorm_blocks(a_blocks, b_blocks):
+    """Multi-block rel_maxnorm: max|a - b| per component over all blocks,
+    relative to that component's largest magnitude over all blocks (a block
+    far from the vortex holds ρv of ~1e-70 to ~1e-16, so per-block scaling
+    would compare rounding noise with itself)."""
+    err = 0.0
+    for c in range(4):
+        scale = max(float(np.max(np.abs(b[c]))) for b in b_blocks)
+        d = max(float(np.max(np.abs(a[c] - b[c]))) for a, b in zip(a_blocks, b_blocks))
+        err = max(err, d / max(scale, 1e-300))
+    return err
+
+
 def rel_global(a, b):
     """max|a-b| / max|b| over all components. Used for right-hand sides,
     whose components cancel to very different magnitudes (the density
@@ -60,3 +73,57 @@ def rel_global(a, b):
     a = np.asarray(a)
     b = np.asarray(b)
     return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
+
+
+def c3_layout(seed=3, lo=64, hi=1024, cols=6, rows=4, extent=10.0):
+    """Batched multi-block workload C3: a cols x rows periodic layout of
+    Cartesian blocks tiling [-extent, extent]^2. Column widths nx_c and row
+    heights ny_r are drawn from `seed` in [lo, hi], so every shared edge has
+    the same number of points on both sides (Grid(j) couplings). Returns
+    (grids, neighbours) with neighbours[i] = {direction: block}."""
+    rng = np.random.default_rng(seed)
+    widths = rng.integers(lo, hi + 1, size=cols)
+    heights = rng.integers(lo, hi + 1, size=rows)
+    xs = np.linspace(-extent, extent, cols + 1)
+    ys = np.linspace(-extent, extent, rows + 1)
+    grids, index = [], {}
+    for r in range(rows):
+        for c in range(cols):
+            nx, ny = int(widths[c]), int(heights[r])
+            x = np.linspace(xs[c], xs[c + 1], nx)
+            y = np.linspace(ys[r], ys[r + 1], ny)
+            index[(r, c)] = len(grids)
+            grids.append((np.broadcast_to(x[None, :], (ny, nx)).copy(), np.broadcast_to(y[:, None], (ny, nx)).copy()))
+    # directions as in the C ABI: north, south, east, west
+    nbs = []
+    for r in range(rows):
+        for c in range(cols):
+            nbs.append({0: index[((r + 1) % rows, c)], 1: index[((r - 1) % rows, c)],
+                        2: index[(r, (c + 1) % cols)], 3: index[(r, (c - 1) % cols)]})
+    return grids, nbs
+
+
+def c3_dt(grids):
+    """0.2 * min over blocks of min(1/(nx-1), 1/(ny-1))."""
+    return 0.2 * min(min(1.0 / (g[0].shape[1] - 1), 1.0 / (g[0].shape[0] - 1)) for g in grids)
+
+
+def c5_batch(seed=5, count=256, lo=48, hi=512):
+    """Batched independent solves C5: per solve (ny, nx) in [lo, hi]^2, vortex
+    Mach in [0.3, 0.7], eps in [0.5, 2.0] and curvilinear amplitude in
+    [0, 0.15], all drawn from `seed`; centre (-1, 0) on [-5, 5]^2 with
+    r* = 1: with r* = 0.5 the strongest draws (eps 2, Mach 0.7) have negative
+    density in the core (1 - eps²(γ-1)γM⁴/(8π²r*²)·e^(1/r*²) < 0), and r* = 1
+    also keeps the core five cells wide on the coarsest grids.
+    Returns a list of dicts (nx, ny, mach, eps, amp, rstar, dt) with the
+    solve's step dt = 0.2 * min(1/(nx-1), 1/(ny-1))."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(count):
+        nx, ny = (int(v) for v in rng.integers(lo, hi + 1, size=2))
+        mach = float(rng.uniform(0.3, 0.7))
+        eps = float(rng.uniform(0.5, 2.0))
+        amp = float(rng.uniform(0.0, 0.15))
+        out.append(dict(nx=nx, ny=ny, mach=mach, eps=eps, amp=amp, rstar=1.0,
+                        dt=0.2 * min(1.0 / (nx - 1), 1.0 / (ny - 1))))
+    return out

@@ -9,6 +9,8 @@ state resident in HBM.
 Prints one JSON line: throughput in Gpoint-RK4-steps/s, the HBM roofline of
 the stage kernel (algorithmic bytes per launch / mean launch duration from HIP
 events), and the CPU oracle timed on this host on a bounded sample.
+`--precision mixed` runs the FP32-RHS/FP64-state form; `--workload c3|c5`
+prints the same line for the 24-block Grid layout and the 256-solve batch.
 """
 import argparse
 import json
@@ -37,11 +39,24 @@ BYTES_PER_POINT_STEP = {"fp64": 672.0, "mixed": 608.0}
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "stage_kernel_upwind9_4096_traffic.json")
 
 
-def build_c2(n, precision):
+class Workload:
+    """A system ready to step, its step size, point count, description and a
+    bounded CPU-oracle sample."""
+
+    def __init__(self, sys_, dt, points, desc, cfg, cpu):
+        self.sys, self.dt, self.points, self.desc, self.cfg, self.cpu = sys_, dt, points, desc, cfg, cpu
+
+
+def _baseline(points, sec, cores, sample):
+    return {"value": points / sec / 1e9, "unit": UNIT, "cores": cores, "kind": "port", "sample": sample}
+
+
+def build_c2(n, precision, cpu_steps):
+    """One curvilinear n x n block, upwind9 + dissipation, vortex + 1e-3
+    density noise, periodic SATs, dt = 0.2/(n-1)."""
     from common import curvilinear, density_noise
     from summationbyparts_amd import _lib, euler, operators
 
-    euler.set_gamma(1.4)
     X, Y = curvilinear(n, n)
     sys_ = euler.System(X, Y, operators.Upwind9())
     if precision == "mixed":
@@ -51,24 +66,105 @@ def build_c2(n, precision):
     q[0] += density_noise(1, 0, n, n)
     sys_.set_state(0, q)
     sys_.set_rhs_kind(_lib.RHS_UPWIND)
-    return sys_, X, Y, q
+
+    def cpu():
+        # the oracle, one thread: the reference's one-thread-per-block semantics for one block
+        from oracle import oracle as O
+        ref = O.System([(X, Y)], [(O.UPWIND9, O.UPWIND9)])
+        ref.set_state(0, q)
+        t0 = time.perf_counter()
+        ref.advance(0.2 / (n - 1), cpu_steps, 1)
+        sec = time.perf_counter() - t0
+        return _baseline(n * n * cpu_steps, sec, 1,
+                         f"{cpu_steps} RK4 steps of the {n}x{n} upwind9 curvilinear block, oracle (C++ restatement), "
+                         f"1 thread, {os.cpu_count()} host cores visible")
+
+    desc = (f"single curvilinear block {n}x{n}, upwind9 + dissipation, vortex + 1e-3 density noise, periodic SATs, "
+            f"RK4 dt=0.2/{n - 1}")
+    return Workload(sys_, 0.2 / (n - 1), n * n, desc, {"nx": n, "ny": n, "operator": "upwind9"}, cpu)
 
 
-def cpu_baseline(X, Y, q, steps):
-    """The oracle (single thread, the reference's one-thread-per-block
-    semantics for one block) on `steps` RK4 steps of the same block."""
-    from oracle import oracle as O
+def build_c3(precision, cpu_steps):
+    """24 Cartesian blocks (6 x 4 periodic layout over [-10, 10]^2, sizes
+    drawn from seed 3 in [64, 1024]) coupled by Grid(j) interfaces, upwind4,
+    vortex initial state, dt = 0.2·min over blocks of min(1/(nx-1), 1/(ny-1))."""
+    from common import c3_dt, c3_layout
+    from summationbyparts_amd import _lib, euler, multigrid, operators
 
-    n = X.shape[0]
-    ref = O.System([(X, Y)], [(O.UPWIND9, O.UPWIND9)])
-    ref.set_state(0, q)
-    dt = 0.2 / (n - 1)
-    t0 = time.perf_counter()
-    ref.advance(dt, steps, 1)
-    sec = time.perf_counter() - t0
-    return {"value": n * n * steps / sec / 1e9, "unit": UNIT, "cores": 1, "kind": "port",
-            "sample": f"{steps} RK4 steps of the {n}x{n} upwind9 curvilinear block, oracle (C++ restatement), "
-                      f"1 thread, {os.cpu_count()} host cores visible"}
+    grids, nbs = c3_layout(seed=3)
+    sys_ = multigrid.System(grids, [operators.Upwind4()] * len(grids))
+    if precision == "mixed":
+        sys_.set_precision(_lib.MIXED)
+    for i, nb in enumerate(nbs):
+        for d, j in nb.items():
+            sys_.set_boundary(i, d, multigrid.Grid(j))
+    vp = euler.VortexParameters([euler.Vortice(-1.0, 0.0, 0.5, 1.0)], 0.5)
+    sys_.vortex(0.0, vp)
+    dt = c3_dt(grids)
+    pts = sum(g[0].size for g in grids)
+
+    def cpu():
+        # the oracle with one thread per block (the reference's distributed system)
+        from oracle import oracle as O
+        ref = O.System(grids, [(O.UPWIND4, O.UPWIND4)] * len(grids))
+        for i, nb in enumerate(nbs):
+            for d, j in nb.items():
+                ref.set_bc(i, d, O.BC_GRID, j)
+        ref.vortex(0.0, O.VortexParams.make([(-1.0, 0.0, 0.5, 1.0)], 0.5))
+        threads = len(grids)
+        t0 = time.perf_counter()
+        ref.advance(dt, cpu_steps, threads)
+        sec = time.perf_counter() - t0
+        return _baseline(pts * cpu_steps, sec, min(threads, os.cpu_count() or 1),
+                         f"{cpu_steps} RK4 steps of the 24-block layout, oracle, one thread per block "
+                         f"({threads} threads on {os.cpu_count()} host cores)")
+
+    desc = f"24 Cartesian blocks 6x4 over [-10,10]^2, sizes 64..1024 (seed 3), Grid interfaces, upwind4, vortex, dt={dt:.6g}"
+    return Workload(sys_, dt, pts, desc, {"blocks": len(grids), "operator": "upwind4"}, cpu)
+
+
+def build_c5(precision, cpu_steps):
+    """256 independent curvilinear vortex solves (seed 5: sizes in [48, 512]^2,
+    Mach, eps, grid amplitude per solve, r* = 1, far-field vortex edges),
+    upwind9, each with its own dt = 0.2·min(1/(nx-1), 1/(ny-1)) (dt scale per
+    block, system step 1)."""
+    from common import c5_batch, curvilinear
+    from summationbyparts_amd import _lib, euler, operators
+
+    batch = c5_batch()
+    grids = [curvilinear(b["nx"], b["ny"], amp=b["amp"]) for b in batch]
+    sys_ = euler.BlockSystem(grids, [operators.Upwind9()] * len(batch))
+    if precision == "mixed":
+        sys_.set_precision(_lib.MIXED)
+    sys_.set_rhs_kind(_lib.RHS_UPWIND)
+    params = []
+    for i, b in enumerate(batch):
+        vp = euler.VortexParameters([euler.Vortice(-1.0, 0.0, b["rstar"], b["eps"])], b["mach"])
+        params.append(vp)
+        for d in range(4):
+            sys_.set_bc(i, d, _lib.BC_VORTEX, 0, vp)
+        sys_.vortex(0.0, vp, block=i)
+        sys_.set_dt_scale(i, b["dt"])
+    pts = sum(b["nx"] * b["ny"] for b in batch)
+
+    def cpu():
+        # the oracle on the first 16 solves, each its own system, one thread
+        from oracle import oracle as O
+        npts, sec = 0, 0.0
+        for b, (X, Y) in list(zip(batch, grids))[:16]:
+            vo = O.VortexParams.make([(-1.0, 0.0, b["rstar"], b["eps"])], b["mach"])
+            ref = O.System([(X, Y)], [(O.UPWIND9, O.UPWIND9)])
+            for d in range(4):
+                ref.set_bc(0, d, O.BC_VORTEX, 0, vo)
+            ref.vortex(0.0, vo)
+            t0 = time.perf_counter()
+            ref.advance(b["dt"], cpu_steps, 1)
+            sec += time.perf_counter() - t0
+            npts += b["nx"] * b["ny"] * cpu_steps
+        return _baseline(npts, sec, 1, f"{cpu_steps} RK4 steps of the first 16 of the 256 solves, oracle, 1 thread")
+
+    desc = "256 independent curvilinear vortex solves, sizes 48..512 (seed 5), own dt/Mach/eps each, far-field edges, upwind9"
+    return Workload(sys_, 1.0, pts, desc, {"blocks": len(batch), "operator": "upwind9"}, cpu)
 
 
 def main():
@@ -78,6 +174,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--n", type=int, default=4096)
     ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
+    ap.add_argument("--workload", choices=["c2", "c3", "c5"], default="c2",
+                    help="c2: one 4096^2 curvilinear block (default); c3: 24-block Grid layout; c5: 256-solve batch")
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--tile", type=int, default=-1, help="force a tile shape index (-1: autotune)")
@@ -104,38 +202,53 @@ def main():
     if not torch.cuda.is_available():
         print("bench.py needs a GPU", file=sys.stderr)
         sys.exit(2)
-    n = args.n
-    sys_, X, Y, q0 = build_c2(n, args.precision)
+    from summationbyparts_amd import euler
+
+    euler.set_gamma(1.4)
+    if args.workload == "c2":
+        w = build_c2(args.n, args.precision, args.cpu_steps)
+    elif args.workload == "c3":
+        w = build_c3(args.precision, args.cpu_steps)
+    else:
+        w = build_c5(args.precision, args.cpu_steps)
+    sys_ = w.sys
     if args.tile >= 0:
         sys_.set_tile_shape(args.tile)
-    dt = 0.2 / (n - 1)
-    sys_.advance_n(dt, args.warmup)
+    dt = w.dt
+    sys_.advance_steps(dt, args.warmup)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     sys_.advance_steps(dt, args.steps)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
-    # host -> device upload of one state through the boundary (not part of
-    # the timed region: a run pays it once)
-    t1 = time.perf_counter()
-    sys_.set_state(0, q0)
-    torch.cuda.synchronize()
-    upload_ms = (time.perf_counter() - t1) * 1e3
+    upload_ms = None
+    if args.workload == "c2":
+        # host -> device upload of one state through the boundary (not part of
+        # the timed region: a run pays it once)
+        q = sys_.field()
+        t1 = time.perf_counter()
+        sys_.set_state(0, q)
+        torch.cuda.synchronize()
+        upload_ms = (time.perf_counter() - t1) * 1e3
     # after the timed region: HIP-event time of 8 graph-replayed steps, then
     # the mean duration of one stage-kernel launch over 8 more steps (events
     # on the library's stream around each launch)
     ms_events, ms_stage = sys_.profile(dt, 8)
     tile = sys_.tile_shape()
-    pts = n * n
+    pts = w.points
     value = pts * args.steps / wall / 1e9
     bytes_per_launch = BYTES_PER_POINT_STEP[args.precision] / 4.0 * pts
     achieved = bytes_per_launch / (ms_stage * 1e-3) / 1e9
     traffic = None
-    if os.path.exists(TRAFFIC_FILE):
+    if os.path.exists(TRAFFIC_FILE) and args.workload == "c2":
         with open(TRAFFIC_FILE) as f:
             tr = json.load(f)
-        if tr.get("n") == n and tr.get("precision", "fp64") == args.precision:
+        if tr.get("n") == args.n and tr.get("precision", "fp64") == args.precision:
             traffic = tr.get("hbm_bytes_per_launch")
+    cfg = {"workload": w.desc, **w.cfg, "precision": args.precision, "tile": list(tile),
+           "event_ms_per_step": ms_events / 8, "stage_launch_ms": ms_stage}
+    if upload_ms is not None:
+        cfg["state_upload_ms"] = upload_ms
     out = {
         "metric": METRIC,
         "value": value,
@@ -149,16 +262,12 @@ def main():
         "vs_baseline": None,
         "dtype": "f64" if args.precision == "fp64" else "f32-rhs/f64-state",
         "data": "synthetic",
-        "config": {"workload": f"single curvilinear block {n}x{n}, upwind9 + dissipation, vortex + 1e-3 density "
-                               f"noise, periodic SATs, RK4 dt=0.2/{n - 1}", "nx": n, "ny": n,
-                   "operator": "upwind9", "precision": args.precision, "tile": list(tile),
-                   "event_ms_per_step": ms_events / 8, "stage_launch_ms": ms_stage,
-                   "state_upload_ms": upload_ms},
+        "config": cfg,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic},
     }
     if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(X, Y, q0, args.cpu_steps)
+        out["cpu_baseline"] = w.cpu()
     print(json.dumps(out), flush=True)
 
 

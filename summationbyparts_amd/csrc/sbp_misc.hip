@@ -351,11 +351,13 @@ __global__ __launch_bounds__(256) void sat_edge_kernel(const StageArgs A, const 
 #pragma unroll
     for (int c = 0; c < 4; ++c) stg(out, c * ps + o, ldg((const double*)out, c * ps + o) + sacc * sat[c]);
   } else {
+    // FP64 stages 1-3: the accumulator (stage 3 keeps none: its k enters
+    // y_{n+1} through Y4) and the next stage input
     double* accb = B.buf[BUF_ACC];
     double* nxt = B.buf[mode == MODE_S1 ? BUF_YB : BUF_YA];
 #pragma unroll
     for (int c = 0; c < 4; ++c) {
-      stg(accb, c * ps + o, ldg((const double*)accb, c * ps + o) + sacc * sat[c]);
+      if (mode != MODE_S2) stg(accb, c * ps + o, ldg((const double*)accb, c * ps + o) + sacc * sat[c]);
       stg(nxt, c * ps + o, ldg((const double*)nxt, c * ps + o) + snext * sat[c]);
     }
   }

@@ -287,11 +287,12 @@ __global__ __launch_bounds__((TX / RX) * (TY / RY)) void stage_kernel(const Stag
     __syncthreads();
   }
 
-  // Finish operands at the output points (J, and y_n / accumulator as the
-  // stage combination needs them), all loaded before any store.
-  //   FP64:  S0 fa = y_n;  S1, S2 fa = y_n, fb = acc;  S3 fa = acc
-  //   mixed: S0 fa = y_n;  S1, S2 fb = acc;            S3 fa = acc
-  double fJ[RY][RX], fa[RY][RX][4], fb[RY][RX][4];
+  // Finish operands at the output points, all loaded before any store.
+  // FP64 (W form of the classical RK4 combination, see the finish):
+  //   S0 fa = y_n;  S1 fa = Y2, fb = acc;  S2 fa = Y2, fb = W, fc = Y3;  S3 fa = W, fc = Y4
+  // mixed: S0 fa = y_n;  S1, S2 fb = acc;  S3 fa = acc
+  constexpr int FC = MIX ? 1 : RY;  // the stage input at the output points (FP64 form only)
+  double fJ[RY][RX], fa[RY][RX][4], fb[RY][RX][4], fc[FC][RX][4];
   auto load_finish = [&]() {
     const double* pacc = MIX ? acc_buf(B, A) : B.buf[BUF_ACC];
 #pragma unroll
@@ -301,14 +302,29 @@ __global__ __launch_bounds__((TX / RX) * (TY / RY)) void stage_kernel(const Stag
         const uint32_t o = (xin[i] && yin[s]) ? (uint32_t)((yb + s) * ld + xb + i) : (uint32_t)(y0 * ld + x0);
         fJ[s][i] = ldg(met, o);
         if (mode == MODE_RHS) continue;
-        if (mode == MODE_S0 || mode == MODE_S3 || !MIX) {
-          const double* pa = mode == MODE_S0 ? in : (mode == MODE_S3 ? pacc : B.buf[BUF_YN]);
+        if constexpr (MIX) {
+          if (mode == MODE_S0 || mode == MODE_S3) {
+            const double* pa = mode == MODE_S0 ? in : pacc;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) fa[s][i][q] = ldg_nt(pa, q * ps + o);
+          }
+          if (mode == MODE_S1 || mode == MODE_S2) {
+#pragma unroll
+            for (int q = 0; q < 4; ++q) fb[s][i][q] = ldg_nt(pacc, q * ps + o);
+          }
+        } else {
+          const double* pa = mode == MODE_S0 ? in : (mode == MODE_S3 ? pacc : B.buf[BUF_YA]);
 #pragma unroll
           for (int q = 0; q < 4; ++q) fa[s][i][q] = ldg_nt(pa, q * ps + o);
-        }
-        if (mode == MODE_S1 || mode == MODE_S2) {
+          if (mode == MODE_S1 || mode == MODE_S2) {
 #pragma unroll
-          for (int q = 0; q < 4; ++q) fb[s][i][q] = ldg_nt(pacc, q * ps + o);
+            for (int q = 0; q < 4; ++q) fb[s][i][q] = ldg_nt(pacc, q * ps + o);
+          }
+          if (mode == MODE_S2 || mode == MODE_S3) {
+            // the stage input at the output point (read moments ago by the phases)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) fc[s][i][q] = ldg(in, q * ps + o);
+          }
         }
       }
   };
@@ -456,25 +472,42 @@ __global__ __launch_bounds__((TX / RX) * (TY / RY)) void stage_kernel(const Stag
           stg_nt(dn, q * ps + o, (float)(snext * k[q]));
         }
       }
-    } else if (mode == MODE_S0) {
-      double* accb = B.buf[BUF_ACC];
-      double* nxt = B.buf[BUF_YA];
-#pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        stg_nt(accb, q * ps + o, fa[s][i][q] + sacc * k[q]);
-        stg_nt(nxt, q * ps + o, fa[s][i][q] + snext * k[q]);
-      }
-    } else if (mode == MODE_S3) {
-      double* out = B.buf[BUF_YN];
-#pragma unroll
-      for (int q = 0; q < 4; ++q) stg_nt(out, q * ps + o, fa[s][i][q] + sacc * k[q]);
     } else {
+      // FP64 stage combination, classical RK4 (utils/integrate/src/lib.rs:30-35,176-222)
+      // with y_n read once per step: Y2 = y + dt/2 k1, Y3 = y + dt/2 k2,
+      // Y4 = y + dt k3, y_{n+1} = y + dt/6 (k1 + 2k2 + 2k3 + k4). Stage 1
+      // keeps acc = y + dt/6 k1; stage 2 recovers y = (3 acc - Y2)/2 and
+      // keeps W = acc + dt/3 k2 - y/3; stage 3 recovers y = Y2 + 2 Y3 - 3 W;
+      // stage 4 forms y_{n+1} = W + Y4/3 + dt/6 k4. Identities of the
+      // tableau; the rounding differs from the reference's running sum by a
+      // few ulps of y.
       double* accb = B.buf[BUF_ACC];
-      double* nxt = B.buf[mode == MODE_S1 ? BUF_YB : BUF_YA];
+      if (mode == MODE_S0) {
+        double* nxt = B.buf[BUF_YA];
 #pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        stg_nt(accb, q * ps + o, fb[s][i][q] + sacc * k[q]);
-        stg_nt(nxt, q * ps + o, fa[s][i][q] + snext * k[q]);
+        for (int q = 0; q < 4; ++q) {
+          stg_nt(accb, q * ps + o, fa[s][i][q] + sacc * k[q]);
+          stg_nt(nxt, q * ps + o, fa[s][i][q] + snext * k[q]);
+        }
+      } else if (mode == MODE_S1) {
+        double* nxt = B.buf[BUF_YB];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const double yv = (3.0 * fb[s][i][q] - fa[s][i][q]) * 0.5;
+          stg_nt(nxt, q * ps + o, yv + snext * k[q]);
+          stg_nt(accb, q * ps + o, fb[s][i][q] + sacc * k[q] - yv * (1.0 / 3.0));
+        }
+      } else if (mode == MODE_S2) {
+        double* nxt = B.buf[BUF_YA];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const double yv = fa[s][i][q] + 2.0 * fc[s][i][q] - 3.0 * fb[s][i][q];
+          stg_nt(nxt, q * ps + o, yv + snext * k[q]);
+        }
+      } else {
+        double* out = B.buf[BUF_YN];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) stg_nt(out, q * ps + o, fa[s][i][q] + fc[s][i][q] * (1.0 / 3.0) + sacc * k[q]);
       }
     }
   }

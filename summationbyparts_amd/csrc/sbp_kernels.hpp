@@ -6,13 +6,15 @@
 
 namespace sbp {
 
-// Configurations of the fused stage kernel: TX points along ξ, TY along η,
-// each thread owning RX adjacent ξ-columns by RY consecutive η-rows, and the
-// variant of keeping the dissipation terms between passes.
+// Configurations of the fused stage kernel. Tiled (var >= 0): TX points
+// along ξ, TY along η, each thread owning RX adjacent ξ-columns by RY
+// consecutive η-rows, and the variant of keeping the dissipation terms
+// between passes. Marching (var == -1): strips of TX columns over segments
+// of TY rows, walked in chunks of RY rows.
 struct TileShape {
   int tx, ty, rx, ry, var;
 };
-constexpr int kNumTileShapes = 9;
+constexpr int kNumTileShapes = 12;
 extern const TileShape kTileShapes[kNumTileShapes];
 
 // Operator application along one axis (0 = ξ, 1 = η) of a (ny, nx) array.

@@ -8,8 +8,9 @@
 namespace sbp {
 
 const TileShape kTileShapes[kNumTileShapes] = {
-    {32, 16, 1, 2, 1}, {32, 16, 1, 2, 0}, {32, 24, 1, 2, 0}, {32, 20, 1, 2, 0}, {32, 12, 1, 2, 0},
-    {64, 16, 1, 2, 0}, {32, 24, 1, 2, 1}, {16, 16, 1, 2, 0}, {16, 24, 1, 2, 0},
+    {32, 16, 1, 2, 1},  {32, 16, 1, 2, 0},  {32, 24, 1, 2, 0},   {32, 20, 1, 2, 0},
+    {32, 12, 1, 2, 0},  {64, 16, 1, 2, 0},  {32, 24, 1, 2, 1},   {16, 16, 1, 2, 0},
+    {16, 24, 1, 2, 0},  {32, 256, 1, 8, -1}, {64, 256, 1, 8, -1}, {32, 512, 1, 8, -1},
 };
 
 // ---------------------------------------------------------------------------

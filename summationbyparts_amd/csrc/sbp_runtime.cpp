@@ -287,9 +287,10 @@ int tune_kind(const sbp_system_s* s, const Group& g) { return (int)g.upwind + 2 
 // Tile list of a group ordered for the 8 XCDs of an MI355X: workgroup g runs
 // on XCD g mod 8, so the natural (block, row, column) order is cut into 8
 // contiguous chunks and interleaved; each XCD then sweeps one compact band of
-// rows (measured: 30% faster than the plain order at 4096²). Narrow vertical
-// strips per XCD raise the L2 hit rate of the η-halos from 39% to 51% but run
-// 20% slower; the band order streams whole rows.
+// rows (measured: 30% faster than the plain order at 4096²). Orders that
+// visit 2-8 tile rows of a column before the next column, or narrow vertical
+// strips per XCD, read the shared η halos closer together (the strips raise
+// their L2 hit rate from 39% to 51%) but run 1-20% slower at 4096².
 void build_tiles(const sbp_system_s* s, const Group& g, int shape, std::vector<TileDev>& out) {
   const TileShape& ts = kTileShapes[shape];
   std::vector<TileDev> nat;

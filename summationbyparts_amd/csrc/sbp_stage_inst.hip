@@ -5,6 +5,7 @@
 #include <type_traits>
 
 #include "sbp_kernels.hpp"
+#include "sbp_march.hpp"
 #include "sbp_stage.hpp"
 
 #ifndef SBP_OPX
@@ -16,8 +17,9 @@
 
 namespace sbp {
 
-// Kernel configurations (tile TX x TY, RY points per thread, caching
-// variant); the list matches kStageConfigs in sbp_misc.hip.
+// Kernel configurations: tiled (tile TX x TY, RY points per thread, caching
+// variant) and marching (strip TX, chunk R rows, segment SL rows, FP64 only);
+// the list matches kTileShapes in sbp_misc.hip.
 // The mixed form always caches α·q in registers (VAR 0).
 template <class OX, class OY, bool UP, bool MIX, int TX, int TY, int RX, int RY, int VAR>
 static void launch_one(int mode, const StageArgs& args, int ntiles, hipStream_t stream) {
@@ -46,6 +48,17 @@ static hipError_t launch_shape(int cfg, int mode, const StageArgs& args, int nti
     SBP_CFG(7, 16, 16, 1, 2, 0)
     SBP_CFG(8, 16, 24, 1, 2, 0)
 #undef SBP_CFG
+#define SBP_MARCH(N, TX, R, SL)                                                                    \
+  case N:                                                                                        \
+    if constexpr (full && !MIX) {                                                                \
+      march_kernel<OX, OY, UP, TX, R, SL><<<ntiles, TX * R, 0, stream>>>(args, mode);            \
+      break;                                                                                     \
+    }                                                                                            \
+    return hipErrorNotSupported;
+    SBP_MARCH(9, 32, 8, 256)
+    SBP_MARCH(10, 64, 8, 256)
+    SBP_MARCH(11, 32, 8, 512)
+#undef SBP_MARCH
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();

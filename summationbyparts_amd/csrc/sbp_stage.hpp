@@ -124,6 +124,10 @@ __global__ __launch_bounds__((TX / RX) * (TY / RY)) void stage_kernel(const Stag
   constexpr int NV = UP ? G::NV : 4;
   constexpr int NC = VAR == 0 ? 4 : 1;  // cached values per region point between passes
   constexpr int LCH = 3;                // region points whose loads are in flight together
+  // FP64: the η phase's first RY points of a thread are its own output points,
+  // whose stage input then serves the stage combination without a reload
+  constexpr bool OWNQ = !MIX && RX == 1 && RY <= LCH;
+  constexpr int ITYE = OWNQ ? RY + (2 * G::HY * TX + NT - 1) / NT : G::ITY;
   __shared__ __align__(16) CT lds[NV * NP];
 
   const TileDev T = A.tiles[blockIdx.x];
@@ -288,11 +292,12 @@ __global__ __launch_bounds__((TX / RX) * (TY / RY)) void stage_kernel(const Stag
   }
 
   // Finish operands at the output points, all loaded before any store.
-  // FP64 (W form of the classical RK4 combination, see the finish):
-  //   S0 fa = y_n;  S1 fa = Y2, fb = acc;  S2 fa = Y2, fb = W, fc = Y3;  S3 fa = W, fc = Y4
+  // FP64 (W form of the classical RK4 combination, see the finish), with
+  // q the stage input held from the η phase:
+  //   S0 q = y_n;  S1 q = Y2, fb = acc;  S2 fa = Y2, fb = W, q = Y3;  S3 fa = W, q = Y4
   // mixed: S0 fa = y_n;  S1, S2 fb = acc;  S3 fa = acc
-  constexpr int FC = MIX ? 1 : RY;  // the stage input at the output points (FP64 form only)
-  double fJ[RY][RX], fa[RY][RX][4], fb[RY][RX][4], fc[FC][RX][4];
+  double fJ[RY][RX], fa[RY][RX][4], fb[RY][RX][4];
+  static_assert(MIX || OWNQ, "the FP64 stage combination takes the stage input from the η phase");
   auto load_finish = [&]() {
     const double* pacc = MIX ? acc_buf(B, A) : B.buf[BUF_ACC];
 #pragma unroll
@@ -313,21 +318,21 @@ __global__ __launch_bounds__((TX / RX) * (TY / RY)) void stage_kernel(const Stag
             for (int q = 0; q < 4; ++q) fb[s][i][q] = ldg_nt(pacc, q * ps + o);
           }
         } else {
-          const double* pa = mode == MODE_S0 ? in : (mode == MODE_S3 ? pacc : B.buf[BUF_YA]);
+          // S0, S1: fa is the stage input, S2, S3: fc is (held from the η phase)
+          if (mode == MODE_S2 || mode == MODE_S3) {
+            const double* pa = mode == MODE_S3 ? pacc : B.buf[BUF_YA];
 #pragma unroll
-          for (int q = 0; q < 4; ++q) fa[s][i][q] = ldg_nt(pa, q * ps + o);
+            for (int q = 0; q < 4; ++q) fa[s][i][q] = ldg_nt(pa, q * ps + o);
+          }
           if (mode == MODE_S1 || mode == MODE_S2) {
 #pragma unroll
             for (int q = 0; q < 4; ++q) fb[s][i][q] = ldg_nt(pacc, q * ps + o);
           }
-          if (mode == MODE_S2 || mode == MODE_S3) {
-            // the stage input at the output point (read moments ago by the phases)
-#pragma unroll
-            for (int q = 0; q < 4; ++q) fc[s][i][q] = ldg(in, q * ps + o);
-          }
         }
       }
   };
+
+  double ownq[OWNQ ? RY : 1][4];  // stage input at the own output points (OWNQ)
 
   // ------------------------------------------------------------ η-phase
   {
@@ -342,26 +347,63 @@ __global__ __launch_bounds__((TX / RX) * (TY / RY)) void stage_kernel(const Stag
       load_q(oc, qc);
       R = mix_ref(qc, ldg(met, 3 * ps + oc), ldg(met, 4 * ps + oc), gamma);
     }
-    CT cache[G::ITY][NC];
+    // region point (row r from ry0, column c from x0) of this thread's
+    // iteration it: own output points first (OWNQ), then the halo rows
+    const int yendr = min(y0 + TY, ny);
+    const int ntop = y0 - ry0, nbot = ry1 - yendr;
+    auto eta_pt = [&](int it, int& r, int& c) -> bool {
+      if constexpr (OWNQ) {
+        if (it < RY) {
+          r = yb - ry0 + it;
+          c = tc;
+          return yb + it < yendr && x0 + c < xend;
+        }
+        const int h = tid + (it - RY) * NT;
+        if (h < ntop * TX) {
+          r = h / TX;
+          c = h - r * TX;
+        } else {
+          const int h2 = h - ntop * TX;
+          const int rr = h2 / TX;
+          r = yendr - ry0 + rr;
+          c = h2 - rr * TX;
+          if (rr >= nbot) return false;
+        }
+        return x0 + c < xend;
+      } else {
+        const int l = tid + it * NT;
+        r = l / TX;
+        c = l - r * TX;
+        return l < G::NPY && r < yh && x0 + c < xend;
+      }
+    };
+    CT cache[ITYE][NC];
 #pragma unroll
-    for (int i0 = 0; i0 < G::ITY; i0 += LCH) {
+    for (int i0 = 0; i0 < ITYE; i0 += LCH) {
       double v[LCH][6];
 #pragma unroll
       for (int j = 0; j < LCH; ++j) {
-        const int l = tid + (i0 + j) * NT;
-        const int r = l / TX, c = l - r * TX;
-        const bool ok = i0 + j < G::ITY && l < G::NPY && r < yh && x0 + c < xend;
+        int r, c;
+        const bool ok = i0 + j < ITYE && eta_pt(i0 + j, r, c);
         const uint32_t o = ok ? (uint32_t)((ry0 + r) * ld + x0 + c) : (uint32_t)(ry0 * ld + x0);
         load_q(o, v[j]);
         v[j][4] = ldg(met, 3 * ps + o);
         v[j][5] = ldg(met, 4 * ps + o);
       }
+      if constexpr (OWNQ) {
+        if (i0 == 0) {
+#pragma unroll
+          for (int s2 = 0; s2 < RY; ++s2)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) ownq[s2][q] = v[s2][q];
+        }
+      }
 #pragma unroll
       for (int j = 0; j < LCH; ++j) {
         const int it = i0 + j;
-        const int l = tid + it * NT;
-        const int r = l / TX, c = l - r * TX;
-        if (it < G::ITY && l < G::NPY && r < yh && x0 + c < xend) {
+        int r, c;
+        if (it < ITYE && eta_pt(it, r, c)) {
+          const int l = r * TX + c;
           CT f[4], aq[4];
           double al = 0.0;
           if constexpr (MIX) {
@@ -415,10 +457,10 @@ __global__ __launch_bounds__((TX / RX) * (TY / RY)) void stage_kernel(const Stag
       if constexpr (VAR != 2) {
         __syncthreads();
 #pragma unroll
-        for (int it = 0; it < G::ITY; ++it) {
-          const int l = tid + it * NT;
-          const int r = l / TX, c = l - r * TX;
-          if (l < G::NPY && r < yh && x0 + c < xend) {
+        for (int it = 0; it < ITYE; ++it) {
+          int r, c;
+          if (eta_pt(it, r, c)) {
+            const int l = r * TX + c;
             if constexpr (NC == 4) {
 #pragma unroll
               for (int q = 0; q < 4; ++q) lds[q * NP + l] = cache[it][q];
@@ -482,18 +524,19 @@ __global__ __launch_bounds__((TX / RX) * (TY / RY)) void stage_kernel(const Stag
       // tableau; the rounding differs from the reference's running sum by a
       // few ulps of y.
       double* accb = B.buf[BUF_ACC];
+      const double* qs = ownq[OWNQ ? s : 0];  // the stage input here
       if (mode == MODE_S0) {
         double* nxt = B.buf[BUF_YA];
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
-          stg_nt(accb, q * ps + o, fa[s][i][q] + sacc * k[q]);
-          stg_nt(nxt, q * ps + o, fa[s][i][q] + snext * k[q]);
+          stg_nt(accb, q * ps + o, qs[q] + sacc * k[q]);
+          stg_nt(nxt, q * ps + o, qs[q] + snext * k[q]);
         }
       } else if (mode == MODE_S1) {
         double* nxt = B.buf[BUF_YB];
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
-          const double yv = (3.0 * fb[s][i][q] - fa[s][i][q]) * 0.5;
+          const double yv = (3.0 * fb[s][i][q] - qs[q]) * 0.5;
           stg_nt(nxt, q * ps + o, yv + snext * k[q]);
           stg_nt(accb, q * ps + o, fb[s][i][q] + sacc * k[q] - yv * (1.0 / 3.0));
         }
@@ -501,13 +544,13 @@ __global__ __launch_bounds__((TX / RX) * (TY / RY)) void stage_kernel(const Stag
         double* nxt = B.buf[BUF_YA];
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
-          const double yv = fa[s][i][q] + 2.0 * fc[s][i][q] - 3.0 * fb[s][i][q];
+          const double yv = fa[s][i][q] + 2.0 * qs[q] - 3.0 * fb[s][i][q];
           stg_nt(nxt, q * ps + o, yv + snext * k[q]);
         }
       } else {
         double* out = B.buf[BUF_YN];
 #pragma unroll
-        for (int q = 0; q < 4; ++q) stg_nt(out, q * ps + o, fa[s][i][q] + fc[s][i][q] * (1.0 / 3.0) + sacc * k[q]);
+        for (int q = 0; q < 4; ++q) stg_nt(out, q * ps + o, fa[s][i][q] + qs[q] * (1.0 / 3.0) + sacc * k[q]);
       }
     }
   }

@@ -30,16 +30,15 @@ HBM_PEAK_GBS = 8000.0
 # Algorithmic bytes per point per RK4 step of the fused schedule (each
 # stage reads its input and the 5 metric planes once, plus the own-point
 # operands of its stage combination, and writes its outputs once).
-# FP64: stage 1 reads y_n, metrics, writes acc, Y2 (136 B); stage 2 reads Y2,
-# metrics, acc, writes Y3, W (168 B); stage 3 reads Y3, metrics, Y2, W,
-# writes Y4 (168 B); stage 4 reads Y4, metrics, W, writes y_{n+1} (136 B):
-# 608 B. (The reference's running-sum order needs 672 B: y_n in stages 2-3
-# and the accumulator write in stage 3.)
+# FP64: stage 1 reads y_n, metrics, writes Y2 (104 B); stage 2 reads Y2,
+# metrics, y_n, writes Y3 (136 B); stage 3 reads Y3, metrics, y_n, writes Y4
+# (136 B); stage 4 reads Y4, metrics, Y2, Y3, y_n and writes y_{n+1} (200 B):
+# 576 B. (The reference's running-sum order needs 672 B.)
 # Mixed: the stage inputs are y_n + Δ_s with FP32 increments Δ_s: stage 1
 # reads y_n, metrics, writes acc, Δ_1 (120 B); stages 2-3 read y_n, Δ_s,
 # metrics, acc and write acc, Δ_{s+1} (168 B each); stage 4 reads y_n, Δ_3,
 # metrics, acc and writes y_{n+1} (152 B): 608 B.
-BYTES_PER_POINT_STEP = {"fp64": 608.0, "mixed": 608.0}
+BYTES_PER_POINT_STEP = {"fp64": 576.0, "mixed": 608.0}
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "stage_kernel_upwind9_4096_traffic.json")
 
 

@@ -376,12 +376,13 @@ __device__ __forceinline__ void vortex_point(const VortexDev& vp, double gamma, 
   }
 }
 
+// FP64 stage inputs: y_n, Y2 (BUF_YA), Y3 (BUF_YB), Y4 (BUF_ACC).
 __device__ __forceinline__ const double* stage_input(const BlockDev& B, int mode) {
   switch (mode) {
     case MODE_S0: return B.buf[BUF_YN];
     case MODE_S1: return B.buf[BUF_YA];
     case MODE_S2: return B.buf[BUF_YB];
-    case MODE_S3: return B.buf[BUF_YA];
+    case MODE_S3: return B.buf[BUF_ACC];
     default: return B.rhs_in;
   }
 }

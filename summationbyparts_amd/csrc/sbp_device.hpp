@@ -7,8 +7,8 @@
 namespace sbp {
 
 // Roles of the four state buffers of a block during one fused RK4 step.
-// FP64 form: y_n, the accumulator (acc after stage 1, W after stage 2; see
-// the stage kernel's finish) and two stage inputs (YA: Y2, then Y4; YB: Y3).
+// FP64 form: y_n and the stage inputs Y2 (YA), Y3 (YB) and Y4 (ACC); the
+// last stage forms y_{n+1} from them (see the stage kernel's finish).
 // Mixed form: YA
 // holds the FP32 stage increments Δ_s = Y_s - y_n (two sets of 4 planes, the
 // second at float offset 4·ps), and the last stage writes y_{n+1} over the

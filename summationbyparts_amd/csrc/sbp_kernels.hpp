@@ -14,7 +14,7 @@ namespace sbp {
 struct TileShape {
   int tx, ty, rx, ry, var;
 };
-constexpr int kNumTileShapes = 12;
+constexpr int kNumTileShapes = 16;
 extern const TileShape kTileShapes[kNumTileShapes];
 
 // Operator application along one axis (0 = ξ, 1 = η) of a (ny, nx) array.

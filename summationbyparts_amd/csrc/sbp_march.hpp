@@ -22,60 +22,40 @@
 
 namespace sbp {
 
-// FP64 stage combination of one point (the W form of classical RK4, see the
-// tiled kernel's finish). q is the stage input at the point; fa, fb are
-// loaded here: S0 fa = y_n (= q); S1 fa = Y2 (= q), fb = acc; S2 fa = Y2,
-// fb = W, fc = Y3 (= q); S3 fa = W, fc = Y4 (= q).
-__device__ __forceinline__ void rk_operands_fp64(const BlockDev& B, int mode, const double q4[4], uint32_t o, uint32_t ps,
-                                                 double fa[4], double fb[4], double fc[4]) {
-  if (mode == MODE_RHS) return;
+// FP64 stage combination of one point (see the tiled kernel's finish). q is
+// the stage input at the point; S1, S2 fa = y_n; S3 fa = Y2, fb = Y3, fc = y_n.
+__device__ __forceinline__ void rk_operands_fp64(const BlockDev& B, int mode, uint32_t o, uint32_t ps, double fa[4],
+                                                 double fb[4], double fc[4]) {
+  if (mode == MODE_RHS || mode == MODE_S0) return;
+  const double* pa = B.buf[mode == MODE_S3 ? BUF_YA : BUF_YN];
 #pragma unroll
-  for (int q = 0; q < 4; ++q) fc[q] = q4[q];
-  if (mode == MODE_S0 || mode == MODE_S1) {
+  for (int q = 0; q < 4; ++q) fa[q] = ldg_nt(pa, q * ps + o);
+  if (mode == MODE_S3) {
 #pragma unroll
-    for (int q = 0; q < 4; ++q) fa[q] = q4[q];
-  } else {
-    const double* pa = mode == MODE_S3 ? B.buf[BUF_ACC] : B.buf[BUF_YA];
-#pragma unroll
-    for (int q = 0; q < 4; ++q) fa[q] = ldg_nt(pa, q * ps + o);
-  }
-  if (mode == MODE_S1 || mode == MODE_S2) {
-#pragma unroll
-    for (int q = 0; q < 4; ++q) fb[q] = ldg_nt((const double*)B.buf[BUF_ACC], q * ps + o);
+    for (int q = 0; q < 4; ++q) {
+      fb[q] = ldg_nt((const double*)B.buf[BUF_YB], q * ps + o);
+      fc[q] = ldg_nt((const double*)B.buf[BUF_YN], q * ps + o);
+    }
   }
 }
 
 __device__ __forceinline__ void rk_combine_fp64(const BlockDev& B, int mode, uint32_t o, uint32_t ps, const double k[4],
-                                                const double fa[4], const double fb[4], const double fc[4], double sacc,
-                                                double snext) {
+                                                const double q4[4], const double fa[4], const double fb[4],
+                                                const double fc[4], double sacc, double snext) {
   if (mode == MODE_RHS) {
 #pragma unroll
     for (int q = 0; q < 4; ++q) stg_nt(B.rhs_out, q * ps + o, k[q]);
-    return;
-  }
-  double* accb = B.buf[BUF_ACC];
-  if (mode == MODE_S0) {
+  } else if (mode == MODE_S0) {
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      stg_nt(accb, q * ps + o, fa[q] + sacc * k[q]);
-      stg_nt(B.buf[BUF_YA], q * ps + o, fa[q] + snext * k[q]);
-    }
-  } else if (mode == MODE_S1) {
+    for (int q = 0; q < 4; ++q) stg_nt(B.buf[BUF_YA], q * ps + o, q4[q] + snext * k[q]);
+  } else if (mode == MODE_S1 || mode == MODE_S2) {
+    double* nxt = B.buf[mode == MODE_S1 ? BUF_YB : BUF_ACC];
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      const double yv = (3.0 * fb[q] - fa[q]) * 0.5;
-      stg_nt(B.buf[BUF_YB], q * ps + o, yv + snext * k[q]);
-      stg_nt(accb, q * ps + o, fb[q] + sacc * k[q] - yv * (1.0 / 3.0));
-    }
-  } else if (mode == MODE_S2) {
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      const double yv = fa[q] + 2.0 * fc[q] - 3.0 * fb[q];
-      stg_nt(B.buf[BUF_YA], q * ps + o, yv + snext * k[q]);
-    }
+    for (int q = 0; q < 4; ++q) stg_nt(nxt, q * ps + o, fa[q] + snext * k[q]);
   } else {
 #pragma unroll
-    for (int q = 0; q < 4; ++q) stg_nt(B.buf[BUF_YN], q * ps + o, fa[q] + fc[q] * (1.0 / 3.0) + sacc * k[q]);
+    for (int q = 0; q < 4; ++q)
+      stg_nt(B.buf[BUF_YN], q * ps + o, (fa[q] + 2.0 * fb[q] + q4[q] - fc[q]) * (1.0 / 3.0) + sacc * k[q]);
   }
 }
 
@@ -145,7 +125,7 @@ __global__ __launch_bounds__(TX* R, 2) void march_kernel(const StageArgs A, cons
 
   double cur[9], curh[6], nxt[9], nxth[6];
   double sxi[4] = {0.0, 0.0, 0.0, 0.0}, jp = 1.0;  // ξ sums and J of the pending point
-  double fa[4], fb[4], fc[4];                       // stage-combination operands of the pending point
+  double fa[4], fb[4], fc[4], qp[4];                // stage-combination operands of the pending point
   load_chunk(0, cur, curh);
 
   for (int c = 0; c <= nchunks + 1; ++c) {
@@ -251,14 +231,18 @@ __global__ __launch_bounds__(TX* R, 2) void march_kernel(const StageArgs A, cons
           if constexpr (UP) a = fma_t(idx_eta, eta_sum(Blk<OY, SBP_DISS>{}, 4 + q), a);
           k[q] = a * ij;
         }
-        rk_combine_fp64(B, mode, (uint32_t)(j * ld + xo), ps, k, fa, fb, fc, sacc, snext);
+        rk_combine_fp64(B, mode, (uint32_t)(j * ld + xo), ps, k, qp, fa, fb, fc, sacc, snext);
       }
     }
     // chunk c's point becomes the pending one
 #pragma unroll
     for (int q = 0; q < 4; ++q) sxi[q] = sn[q];
     jp = cur[4];
-    if (fin_c) rk_operands_fp64(B, mode, cur, (uint32_t)(r * ld + xo), ps, fa, fb, fc);
+    if (fin_c) {
+#pragma unroll
+      for (int q = 0; q < 4; ++q) qp[q] = cur[q];
+      rk_operands_fp64(B, mode, (uint32_t)(r * ld + xo), ps, fa, fb, fc);
+    }
 #pragma unroll
     for (int m = 0; m < 9; ++m) cur[m] = nxt[m];
 #pragma unroll

@@ -11,6 +11,7 @@ const TileShape kTileShapes[kNumTileShapes] = {
     {32, 16, 1, 2, 1},  {32, 16, 1, 2, 0},  {32, 24, 1, 2, 0},   {32, 20, 1, 2, 0},
     {32, 12, 1, 2, 0},  {64, 16, 1, 2, 0},  {32, 24, 1, 2, 1},   {16, 16, 1, 2, 0},
     {16, 24, 1, 2, 0},  {32, 256, 1, 8, -1}, {64, 256, 1, 8, -1}, {32, 512, 1, 8, -1},
+    {32, 8, 1, 1, 0},   {32, 16, 1, 1, 0},  {64, 8, 1, 1, 0},    {32, 12, 1, 1, 0},
 };
 
 // ---------------------------------------------------------------------------
@@ -352,15 +353,10 @@ __global__ __launch_bounds__(256) void sat_edge_kernel(const StageArgs A, const 
 #pragma unroll
     for (int c = 0; c < 4; ++c) stg(out, c * ps + o, ldg((const double*)out, c * ps + o) + sacc * sat[c]);
   } else {
-    // FP64 stages 1-3: the accumulator (stage 3 keeps none: its k enters
-    // y_{n+1} through Y4) and the next stage input
-    double* accb = B.buf[BUF_ACC];
-    double* nxt = B.buf[mode == MODE_S1 ? BUF_YB : BUF_YA];
+    // FP64 stages 1-3: the next stage input (Y2, Y3, Y4)
+    double* nxt = B.buf[mode == MODE_S0 ? BUF_YA : (mode == MODE_S1 ? BUF_YB : BUF_ACC)];
 #pragma unroll
-    for (int c = 0; c < 4; ++c) {
-      if (mode != MODE_S2) stg(accb, c * ps + o, ldg((const double*)accb, c * ps + o) + sacc * sat[c]);
-      stg(nxt, c * ps + o, ldg((const double*)nxt, c * ps + o) + snext * sat[c]);
-    }
+    for (int c = 0; c < 4; ++c) stg(nxt, c * ps + o, ldg((const double*)nxt, c * ps + o) + snext * sat[c]);
   }
   if (err) record_error(A.error, err);
 }

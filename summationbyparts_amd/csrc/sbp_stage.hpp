@@ -292,11 +292,11 @@ __global__ __launch_bounds__((TX / RX) * (TY / RY)) void stage_kernel(const Stag
   }
 
   // Finish operands at the output points, all loaded before any store.
-  // FP64 (W form of the classical RK4 combination, see the finish), with
-  // q the stage input held from the η phase:
-  //   S0 q = y_n;  S1 q = Y2, fb = acc;  S2 fa = Y2, fb = W, q = Y3;  S3 fa = W, q = Y4
+  // FP64 (see the finish), with q the stage input held from the η phase:
+  //   S0 q = y_n;  S1 fa = y_n;  S2 fa = y_n;  S3 fa = Y2, fb = Y3, fc = y_n, q = Y4
   // mixed: S0 fa = y_n;  S1, S2 fb = acc;  S3 fa = acc
-  double fJ[RY][RX], fa[RY][RX][4], fb[RY][RX][4];
+  constexpr int FC = MIX ? 1 : RY;
+  double fJ[RY][RX], fa[RY][RX][4], fb[RY][RX][4], fc[FC][RX][4];
   static_assert(MIX || OWNQ, "the FP64 stage combination takes the stage input from the η phase");
   auto load_finish = [&]() {
     const double* pacc = MIX ? acc_buf(B, A) : B.buf[BUF_ACC];
@@ -317,16 +317,17 @@ __global__ __launch_bounds__((TX / RX) * (TY / RY)) void stage_kernel(const Stag
 #pragma unroll
             for (int q = 0; q < 4; ++q) fb[s][i][q] = ldg_nt(pacc, q * ps + o);
           }
-        } else {
-          // S0, S1: fa is the stage input, S2, S3: fc is (held from the η phase)
-          if (mode == MODE_S2 || mode == MODE_S3) {
-            const double* pa = mode == MODE_S3 ? pacc : B.buf[BUF_YA];
+        } else if (mode != MODE_S0) {
+          // S1, S2: fa = y_n; S3: fa = Y2, fb = Y3, fc = y_n
+          const double* pa = B.buf[mode == MODE_S3 ? BUF_YA : BUF_YN];
 #pragma unroll
-            for (int q = 0; q < 4; ++q) fa[s][i][q] = ldg_nt(pa, q * ps + o);
-          }
-          if (mode == MODE_S1 || mode == MODE_S2) {
+          for (int q = 0; q < 4; ++q) fa[s][i][q] = ldg_nt(pa, q * ps + o);
+          if (mode == MODE_S3) {
 #pragma unroll
-            for (int q = 0; q < 4; ++q) fb[s][i][q] = ldg_nt(pacc, q * ps + o);
+            for (int q = 0; q < 4; ++q) {
+              fb[s][i][q] = ldg_nt((const double*)B.buf[BUF_YB], q * ps + o);
+              fc[s][i][q] = ldg_nt((const double*)B.buf[BUF_YN], q * ps + o);
+            }
           }
         }
       }
@@ -515,42 +516,26 @@ __global__ __launch_bounds__((TX / RX) * (TY / RY)) void stage_kernel(const Stag
         }
       }
     } else {
-      // FP64 stage combination, classical RK4 (utils/integrate/src/lib.rs:30-35,176-222)
-      // with y_n read once per step: Y2 = y + dt/2 k1, Y3 = y + dt/2 k2,
-      // Y4 = y + dt k3, y_{n+1} = y + dt/6 (k1 + 2k2 + 2k3 + k4). Stage 1
-      // keeps acc = y + dt/6 k1; stage 2 recovers y = (3 acc - Y2)/2 and
-      // keeps W = acc + dt/3 k2 - y/3; stage 3 recovers y = Y2 + 2 Y3 - 3 W;
-      // stage 4 forms y_{n+1} = W + Y4/3 + dt/6 k4. Identities of the
-      // tableau; the rounding differs from the reference's running sum by a
-      // few ulps of y.
-      double* accb = B.buf[BUF_ACC];
+      // FP64 stage combination, classical RK4 (utils/integrate/src/lib.rs:30-35,176-222):
+      // Y2 = y + dt/2 k1, Y3 = y + dt/2 k2, Y4 = y + dt k3 and
+      // y_{n+1} = y + dt/6 (k1 + 2k2 + 2k3 + k4) = (Y2 + 2 Y3 + Y4 - y)/3 + dt/6 k4,
+      // an identity of the tableau (dt k1 = 2(Y2 - y), dt k2 = 2(Y3 - y),
+      // dt k3 = Y4 - y). No accumulator is kept between stages; the rounding
+      // differs from the reference's running sum by a few ulps of y.
       const double* qs = ownq[OWNQ ? s : 0];  // the stage input here
       if (mode == MODE_S0) {
-        double* nxt = B.buf[BUF_YA];
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
-          stg_nt(accb, q * ps + o, qs[q] + sacc * k[q]);
-          stg_nt(nxt, q * ps + o, qs[q] + snext * k[q]);
-        }
-      } else if (mode == MODE_S1) {
-        double* nxt = B.buf[BUF_YB];
+        for (int q = 0; q < 4; ++q) stg_nt(B.buf[BUF_YA], q * ps + o, qs[q] + snext * k[q]);
+      } else if (mode == MODE_S1 || mode == MODE_S2) {
+        double* nxt = B.buf[mode == MODE_S1 ? BUF_YB : BUF_ACC];
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
-          const double yv = (3.0 * fb[s][i][q] - qs[q]) * 0.5;
-          stg_nt(nxt, q * ps + o, yv + snext * k[q]);
-          stg_nt(accb, q * ps + o, fb[s][i][q] + sacc * k[q] - yv * (1.0 / 3.0));
-        }
-      } else if (mode == MODE_S2) {
-        double* nxt = B.buf[BUF_YA];
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-          const double yv = fa[s][i][q] + 2.0 * qs[q] - 3.0 * fb[s][i][q];
-          stg_nt(nxt, q * ps + o, yv + snext * k[q]);
-        }
+        for (int q = 0; q < 4; ++q) stg_nt(nxt, q * ps + o, fa[s][i][q] + snext * k[q]);
       } else {
-        double* out = B.buf[BUF_YN];
 #pragma unroll
-        for (int q = 0; q < 4; ++q) stg_nt(out, q * ps + o, fa[s][i][q] + qs[q] * (1.0 / 3.0) + sacc * k[q]);
+        for (int q = 0; q < 4; ++q) {
+          const double sum = fa[s][i][q] + 2.0 * fb[s][i][q] + qs[q] - fc[s][i][q];
+          stg_nt(B.buf[BUF_YN], q * ps + o, sum * (1.0 / 3.0) + sacc * k[q]);
+        }
       }
     }
   }

@@ -47,6 +47,10 @@ static hipError_t launch_shape(int cfg, int mode, const StageArgs& args, int nti
     SBP_CFG(6, 32, 24, 1, 2, 1)
     SBP_CFG(7, 16, 16, 1, 2, 0)
     SBP_CFG(8, 16, 24, 1, 2, 0)
+    SBP_CFG(12, 32, 8, 1, 1, 0)
+    SBP_CFG(13, 32, 16, 1, 1, 0)
+    SBP_CFG(14, 64, 8, 1, 1, 0)
+    SBP_CFG(15, 32, 12, 1, 1, 0)
 #undef SBP_CFG
 #define SBP_MARCH(N, TX, R, SL)                                                                    \
   case N:                                                                                        \

@@ -127,7 +127,7 @@ def test_rhs_periodic(gpu, names, shape, rhs_kind):
     assert rel_global(k, k_ref) < RHS_TOL
 
 
-N_TILE_SHAPES = 12
+N_TILE_SHAPES = 16
 
 
 @pytest.mark.parametrize("shape_index", list(range(N_TILE_SHAPES)))

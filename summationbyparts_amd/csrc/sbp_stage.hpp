@@ -128,6 +128,7 @@ __global__ __launch_bounds__((TX / RX) * (TY / RY)) void stage_kernel(const Stag
   // whose stage input then serves the stage combination without a reload
   constexpr bool OWNQ = !MIX && RX == 1 && RY <= LCH;
   constexpr int ITYE = OWNQ ? RY + (2 * G::HY * TX + NT - 1) / NT : G::ITY;
+  constexpr int ITXE = OWNQ ? RY + (2 * G::HX * TY + NT - 1) / NT : G::ITX;
   __shared__ __align__(16) CT lds[NV * NP];
 
   const TileDev T = A.tiles[blockIdx.x];
@@ -180,6 +181,13 @@ __global__ __launch_bounds__((TX / RX) * (TY / RY)) void stage_kernel(const Stag
   const uint32_t oc = (uint32_t)(min(y0 + TY / 2, ny - 1) * ld + min(x0 + TX / 2, nx - 1));
 
   CT acc[RY][RX][4];
+  // metric products at the own output points (OWNQ): J·ξx, J·ξy from the ξ
+  // phase (parked in LDS, each thread reading back its own slots, to keep
+  // them out of registers), J·ηx, J·ηy from the η phase;
+  // J = J·ξx J·ηy - J·ξy J·ηx (= y_η x_ξ - x_η y_ξ, sbp/src/grid.rs:110-131)
+  // replaces a load of J.
+  double ownm[OWNQ ? RY : 1][2];
+  __shared__ double xsave[OWNQ ? 2 * RY * NT : 1];
 
   // ------------------------------------------------------------ ξ-phase
   {
@@ -193,28 +201,58 @@ __global__ __launch_bounds__((TX / RX) * (TY / RY)) void stage_kernel(const Stag
       load_q(oc, qc);
       R = mix_ref(qc, ldg(met, ps + oc), ldg(met, 2 * ps + oc), gamma);
     }
-    CT cache[G::ITX][NC];
+    // region point (tile row r, column c from rx0) of this thread's
+    // iteration it: own output points first (OWNQ), then the halo columns
+    auto xi_pt = [&](int it, int& r, int& c) -> bool {
+      if constexpr (OWNQ) {
+        if (it < RY) {
+          r = tr * RY + it;
+          c = xb - rx0;
+          return y0 + r < yend && xb < nx;
+        }
+        const int h = tid + (it - RY) * NT;
+        r = h / (2 * HX);
+        const int hc = h - r * (2 * HX);
+        const int ax = hc < HX ? x0 - HX + hc : x0 + TX + (hc - HX);
+        c = ax - rx0;
+        return h < 2 * HX * TY && y0 + r < yend && ax >= rx0 && ax < rx1;
+      } else {
+        const int l = tid + it * NT;
+        r = l / WX;
+        c = l - r * WX;
+        return l < G::NPX && c < xw && y0 + r < yend;
+      }
+    };
+    CT cache[ITXE][NC];
     // loads of up to LCH region points are issued together (invalid slots
     // read a clamped in-bounds point) so that their latencies overlap
 #pragma unroll
-    for (int i0 = 0; i0 < G::ITX; i0 += LCH) {
+    for (int i0 = 0; i0 < ITXE; i0 += LCH) {
       double v[LCH][6];
 #pragma unroll
       for (int j = 0; j < LCH; ++j) {
-        const int l = tid + (i0 + j) * NT;
-        const int r = l / WX, c = l - r * WX;
-        const bool ok = i0 + j < G::ITX && l < G::NPX && c < xw && y0 + r < yend;
+        int r, c;
+        const bool ok = i0 + j < ITXE && xi_pt(i0 + j, r, c);
         const uint32_t o = ok ? (uint32_t)((y0 + r) * ld + rx0 + c) : (uint32_t)(y0 * ld + rx0);
         load_q(o, v[j]);
         v[j][4] = ldg(met, ps + o);
         v[j][5] = ldg(met, 2 * ps + o);
       }
+      if constexpr (OWNQ) {
+        if (i0 == 0) {
+#pragma unroll
+          for (int s2 = 0; s2 < RY; ++s2) {
+            xsave[(2 * s2) * NT + tid] = v[s2][4];
+            xsave[(2 * s2 + 1) * NT + tid] = v[s2][5];
+          }
+        }
+      }
 #pragma unroll
       for (int j = 0; j < LCH; ++j) {
         const int it = i0 + j;
-        const int l = tid + it * NT;
-        const int r = l / WX, c = l - r * WX;
-        if (it < G::ITX && l < G::NPX && c < xw && y0 + r < yend) {
+        int r, c;
+        if (it < ITXE && xi_pt(it, r, c)) {
+          const int l = r * WX + c;
           CT f[4], aq[4];
           double al = 0.0;
           if constexpr (MIX) {
@@ -268,10 +306,10 @@ __global__ __launch_bounds__((TX / RX) * (TY / RY)) void stage_kernel(const Stag
       if constexpr (VAR != 2) {
         __syncthreads();
 #pragma unroll
-        for (int it = 0; it < G::ITX; ++it) {
-          const int l = tid + it * NT;
-          const int r = l / WX, c = l - r * WX;
-          if (l < G::NPX && c < xw && y0 + r < yend) {
+        for (int it = 0; it < ITXE; ++it) {
+          int r, c;
+          if (xi_pt(it, r, c)) {
+            const int l = r * WX + c;
             if constexpr (NC == 4) {
 #pragma unroll
               for (int q = 0; q < 4; ++q) lds[q * NP + l] = cache[it][q];
@@ -305,7 +343,11 @@ __global__ __launch_bounds__((TX / RX) * (TY / RY)) void stage_kernel(const Stag
 #pragma unroll
       for (int i = 0; i < RX; ++i) {
         const uint32_t o = (xin[i] && yin[s]) ? (uint32_t)((yb + s) * ld + xb + i) : (uint32_t)(y0 * ld + x0);
-        fJ[s][i] = ldg(met, o);
+        if constexpr (OWNQ) {
+          fJ[s][i] = xsave[(2 * s) * NT + tid] * ownm[s][1] - xsave[(2 * s + 1) * NT + tid] * ownm[s][0];
+        } else {
+          fJ[s][i] = ldg(met, o);
+        }
         if (mode == MODE_RHS) continue;
         if constexpr (MIX) {
           if (mode == MODE_S0 || mode == MODE_S3) {
@@ -394,9 +436,12 @@ __global__ __launch_bounds__((TX / RX) * (TY / RY)) void stage_kernel(const Stag
       if constexpr (OWNQ) {
         if (i0 == 0) {
 #pragma unroll
-          for (int s2 = 0; s2 < RY; ++s2)
+          for (int s2 = 0; s2 < RY; ++s2) {
 #pragma unroll
             for (int q = 0; q < 4; ++q) ownq[s2][q] = v[s2][q];
+            ownm[s2][0] = v[s2][4];
+            ownm[s2][1] = v[s2][5];
+          }
         }
       }
 #pragma unroll

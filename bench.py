@@ -28,17 +28,21 @@ METRIC = "Gpoint-RK4-steps/s + % HBM roofline; mixed vs FP64 at ≤1e-8 rel L2 e
 UNIT = "Gpoint-RK4-steps/s"
 HBM_PEAK_GBS = 8000.0
 # Algorithmic bytes per point per RK4 step of the fused schedule (each
-# stage reads its input and the 5 metric planes once, plus the own-point
-# operands of its stage combination, and writes its outputs once).
-# FP64: stage 1 reads y_n, metrics, writes Y2 (104 B); stage 2 reads Y2,
-# metrics, y_n, writes Y3 (136 B); stage 3 reads Y3, metrics, y_n, writes Y4
-# (136 B); stage 4 reads Y4, metrics, Y2, Y3, y_n and writes y_{n+1} (200 B):
-# 576 B. (The reference's running-sum order needs 672 B.)
+# stage reads its input and the metric planes it uses once, plus the
+# own-point operands of its stage combination, and writes its outputs once).
+# FP64 (J is formed from the four metric products, so 4 metric planes):
+# stage 1 reads y_n, metrics, writes Y2 (96 B); stage 2 reads Y2, metrics,
+# y_n, writes Y3 (128 B); stage 3 reads Y3, metrics, y_n, writes Y4 (128 B);
+# stage 4 reads Y4, metrics, Y2, Y3, y_n and writes y_{n+1} (192 B): 544 B.
+# The reference's schedule (running accumulator, 5 metric planes) needs
+# 672 B; bench also reports the rate at which that schedule's bytes would
+# have to move to match this kernel ("reference_schedule").
 # Mixed: the stage inputs are y_n + Δ_s with FP32 increments Δ_s: stage 1
 # reads y_n, metrics, writes acc, Δ_1 (120 B); stages 2-3 read y_n, Δ_s,
 # metrics, acc and write acc, Δ_{s+1} (168 B each); stage 4 reads y_n, Δ_3,
 # metrics, acc and writes y_{n+1} (152 B): 608 B.
-BYTES_PER_POINT_STEP = {"fp64": 576.0, "mixed": 608.0}
+BYTES_PER_POINT_STEP = {"fp64": 544.0, "mixed": 608.0}
+REFERENCE_SCHEDULE_BYTES = 672.0
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "stage_kernel_upwind9_4096_traffic.json")
 
 
@@ -188,12 +192,31 @@ def main():
     if args.calibrate_copy:
         import torch
 
-        a = torch.ones(1 << 28, dtype=torch.float64, device="cuda")
+        a = torch.ones(1 << 28, dtype=torch.float64, device="cuda")  # 2 GiB
         b = torch.empty_like(a)
         for _ in range(3):
             b.copy_(a)
         torch.cuda.synchronize()
-        print(json.dumps({"copy_bytes_read": a.numel() * 8, "copy_bytes_written": b.numel() * 8}))
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+
+        def rate(fn, nbytes, reps=10):
+            fn()
+            torch.cuda.synchronize()
+            e0.record()
+            for _ in range(reps):
+                fn()
+            e1.record()
+            torch.cuda.synchronize()
+            return nbytes / (e0.elapsed_time(e1) / reps * 1e-3) / 1e9
+
+        n = a.numel() * 8
+        q = a.numel() // 4
+        c, d, e, o = a[:q], a[q:2 * q], a[2 * q:3 * q], b[:q]
+        out = {"copy_bytes_read": n, "copy_bytes_written": n,
+               "copy_gbs": rate(lambda: b.copy_(a), 2 * n),
+               "sum_gbs": rate(lambda: a.sum(), n),
+               "addcmul_3r1w_gbs": rate(lambda: torch.addcmul(c, d, e, out=o), 4 * q * 8)}
+        print(json.dumps(out))
         return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (this build has no multi-GPU path); got --gpus {args.gpus}",
@@ -268,6 +291,9 @@ def main():
         "config": cfg,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic},
+        "reference_schedule": {"bytes_per_point_step": REFERENCE_SCHEDULE_BYTES,
+                               "equivalent_gbs": REFERENCE_SCHEDULE_BYTES / 4.0 * pts / (ms_stage * 1e-3) / 1e9,
+                               "frac": REFERENCE_SCHEDULE_BYTES / 4.0 * pts / (ms_stage * 1e-3) / 1e9 / HBM_PEAK_GBS},
     }
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = w.cpu()

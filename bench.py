@@ -269,7 +269,8 @@ def main():
     if os.path.exists(TRAFFIC_FILE) and args.workload == "c2":
         with open(TRAFFIC_FILE) as f:
             tr = json.load(f)
-        if tr.get("n") == args.n and tr.get("precision", "fp64") == args.precision:
+        if (tr.get("n") == args.n and tr.get("precision", "fp64") == args.precision
+                and tr.get("tile") == list(sys_.tile_shape())):
             traffic = tr.get("hbm_bytes_per_launch")
     cfg = {"workload": w.desc, **w.cfg, "precision": args.precision, "tile": list(tile),
            "event_ms_per_step": ms_events / 8, "stage_launch_ms": ms_stage}

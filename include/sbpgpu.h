@@ -193,8 +193,8 @@ int sbp_system_check(sbp_system sys);
 /* ---- Measurement ---------------------------------------------------------
  * Times nsteps RK4 steps (the state is advanced) with HIP events on the
  * context stream into *ms_total; then runs min(nsteps, 8) more steps with
- * every stage-kernel launch bracketed by its own events and returns the mean
- * launch duration in *ms_stage_avg. */
+ * every stage (the launches of its edge and interior tiles) bracketed by its
+ * own events and returns the mean stage duration in *ms_stage_avg. */
 int sbp_system_profile(sbp_system sys, double dt, int64_t nsteps, double* ms_total, double* ms_stage_avg);
 /* Tile shape currently used for block 0 (after autotuning). */
 int sbp_system_tile_shape(sbp_system sys, int* tx, int* ty, int* ry);

@@ -6,15 +6,13 @@
 
 namespace sbp {
 
-// Configurations of the fused stage kernel. Tiled (var >= 0): TX points
-// along ξ, TY along η, each thread owning RX adjacent ξ-columns by RY
-// consecutive η-rows, and the variant of keeping the dissipation terms
-// between passes. Marching (var == -1): strips of TX columns over segments
-// of TY rows, walked in chunks of RY rows.
+// Configurations of the fused stage kernel: TX points along ξ, TY along η,
+// each thread owning RX adjacent ξ-columns by RY consecutive η-rows, and the
+// variant of keeping the dissipation terms between passes.
 struct TileShape {
   int tx, ty, rx, ry, var;
 };
-constexpr int kNumTileShapes = 16;
+constexpr int kNumTileShapes = 13;
 extern const TileShape kTileShapes[kNumTileShapes];
 
 // Operator application along one axis (0 = ξ, 1 = η) of a (ny, nx) array.
@@ -22,15 +20,14 @@ hipError_t launch_apply_op(int op, int kind, int axis, const double* prev, int64
                            int nx, hipStream_t stream);
 
 // Fused stage kernel over `ntiles` tiles of blocks sharing one operator pair.
-// mode: StageMode. Returns hipErrorInvalidValue for an unsupported combination.
-hipError_t launch_stage(int op_eta, int op_xi, bool upwind, int shape, int mode, const StageArgs& args, int ntiles,
-                        hipStream_t stream);
+// mode: StageMode; edge: the tiles lie on block edges and add the edge SATs.
+// Returns hipErrorInvalidValue for an unsupported combination and
+// hipErrorNotSupported for a configuration not built for the pair.
+hipError_t launch_stage(int op_eta, int op_xi, bool upwind, bool edge, int shape, int mode, const StageArgs& args,
+                        int ntiles, hipStream_t stream);
 // Launch bounds (threads per workgroup) and LDS bytes for a tile shape.
 int stage_threads(int shape);
 
-// Edge SATs of every block (args.blocks[0..nblocks)) for one stage, after
-// its stage kernels.
-hipError_t launch_sat_edges(int mode, const StageArgs& args, int nblocks, int max_perimeter, hipStream_t stream);
 // Pointwise metrics from the four derivative planes (grid.rs:110-131).
 hipError_t launch_metrics_finish(const double* dx_dxi, const double* dx_deta, const double* dy_dxi, const double* dy_deta,
                                  double* metrics, int64_t ps, int64_t ld, int ny, int nx, unsigned* err, hipStream_t stream);

@@ -8,10 +8,9 @@
 namespace sbp {
 
 const TileShape kTileShapes[kNumTileShapes] = {
-    {32, 16, 1, 2, 1},  {32, 16, 1, 2, 0},  {32, 24, 1, 2, 0},   {32, 20, 1, 2, 0},
-    {32, 12, 1, 2, 0},  {64, 16, 1, 2, 0},  {32, 24, 1, 2, 1},   {16, 16, 1, 2, 0},
-    {16, 24, 1, 2, 0},  {32, 256, 1, 8, -1}, {64, 256, 1, 8, -1}, {32, 512, 1, 8, -1},
-    {32, 8, 1, 1, 0},   {32, 16, 1, 1, 0},  {64, 8, 1, 1, 0},    {32, 12, 1, 1, 0},
+    {32, 16, 1, 2, 1}, {32, 16, 1, 2, 0}, {32, 24, 1, 2, 0}, {32, 20, 1, 2, 0}, {32, 12, 1, 2, 0},
+    {64, 16, 1, 2, 0}, {32, 24, 1, 2, 1}, {16, 16, 1, 2, 0}, {16, 24, 1, 2, 0}, {32, 8, 1, 1, 0},
+    {32, 16, 1, 1, 0}, {64, 8, 1, 1, 0},  {32, 12, 1, 1, 0},
 };
 
 // ---------------------------------------------------------------------------
@@ -278,93 +277,6 @@ __global__ __launch_bounds__(256) void maxspeed_kernel(const double* __restrict_
 hipError_t launch_maxspeed(const double* q, const double* metrics, int64_t ps, int64_t ld, int ny, int nx, double* partials,
                            hipStream_t stream) {
   maxspeed_kernel<<<reduce_max_workgroups(ny, nx), 256, 0, stream>>>(q, metrics, ps, ld, ny, nx, partials);
-  return hipGetLastError();
-}
-
-// ---------------------------------------------------------------------------
-// Characteristic SATs of the block edges (SAT_characteristics,
-// euler/src/lib.rs:1002-1013), after the stage kernel of the same stage. One
-// thread per perimeter point of block blockIdx.y: the point's penalties are
-// summed in the reference's order north, south, east, west (corners get two)
-// and enter the outputs of the stage through its linear combination.
-__global__ __launch_bounds__(256) void sat_edge_kernel(const StageArgs A, const int mode) {
-  const BlockDev& B = A.blocks[blockIdx.y];
-  const int nx = B.nx, ny = B.ny;
-  const int per = 2 * nx + 2 * (ny - 2);
-  const int t = blockIdx.x * 256 + threadIdx.x;
-  if (t >= per) return;
-  int x, y;
-  if (t < nx) {
-    x = t;
-    y = ny - 1;
-  } else if (t < 2 * nx) {
-    x = t - nx;
-    y = 0;
-  } else if (t < 2 * nx + ny - 2) {
-    x = nx - 1;
-    y = t - 2 * nx + 1;
-  } else {
-    x = 0;
-    y = t - (2 * nx + ny - 2) + 1;
-  }
-  const uint32_t ps = (uint32_t)B.ps;
-  const uint32_t o = (uint32_t)(y * B.ld + x);
-  const double* met = B.metrics;
-  const double gamma = A.gamma;
-  const double time = ((A.time ? *A.time : 0.0) + A.time_offset) * B.dt_scale;
-  const double dtb = A.dt * B.dt_scale;
-  const double sacc = A.rk_b * dtb, snext = A.rk_a * dtb;
-  unsigned err = 0;
-  double q[4], g[4], sat[4] = {0.0, 0.0, 0.0, 0.0};
-  load_stage_state(A, B, mode, o, q);
-  const double dj = ldg(met, o);
-  if (y == ny - 1) {
-    edge_data(A, B, SBP_NORTH, x, x, y, mode, time, g, err);
-    sat_point(sat, q, g, B.hi_eta, -1.0, 1.0, dj, ldg(met, 3 * ps + o), ldg(met, 4 * ps + o), gamma);
-  }
-  if (y == 0) {
-    edge_data(A, B, SBP_SOUTH, x, x, y, mode, time, g, err);
-    sat_point(sat, q, g, B.hi_eta, 1.0, -1.0, dj, ldg(met, 3 * ps + o), ldg(met, 4 * ps + o), gamma);
-  }
-  if (x == nx - 1) {
-    edge_data(A, B, SBP_EAST, y, x, y, mode, time, g, err);
-    sat_point(sat, q, g, B.hi_xi, -1.0, 1.0, dj, ldg(met, ps + o), ldg(met, 2 * ps + o), gamma);
-  }
-  if (x == 0) {
-    edge_data(A, B, SBP_WEST, y, x, y, mode, time, g, err);
-    sat_point(sat, q, g, B.hi_xi, 1.0, -1.0, dj, ldg(met, ps + o), ldg(met, 2 * ps + o), gamma);
-  }
-  if (mode == MODE_RHS) {
-#pragma unroll
-    for (int c = 0; c < 4; ++c) stg(B.rhs_out, c * ps + o, ldg((const double*)B.rhs_out, c * ps + o) + sat[c]);
-  } else if (A.mixed) {
-    // y_{n+1} (stage 4) or the accumulator, and the FP32 stage increment
-    double* accb = acc_buf(B, A);
-#pragma unroll
-    for (int c = 0; c < 4; ++c) stg(accb, c * ps + o, ldg((const double*)accb, c * ps + o) + sacc * sat[c]);
-    if (mode != MODE_S3) {
-      float* dn = delta_out(B, mode);
-#pragma unroll
-      for (int c = 0; c < 4; ++c)
-        stg(dn, c * ps + o, (float)((double)ldg((const float*)dn, c * ps + o) + snext * sat[c]));
-    }
-  } else if (mode == MODE_S3) {
-    double* out = B.buf[BUF_YN];
-#pragma unroll
-    for (int c = 0; c < 4; ++c) stg(out, c * ps + o, ldg((const double*)out, c * ps + o) + sacc * sat[c]);
-  } else {
-    // FP64 stages 1-3: the next stage input (Y2, Y3, Y4)
-    double* nxt = B.buf[mode == MODE_S0 ? BUF_YA : (mode == MODE_S1 ? BUF_YB : BUF_ACC)];
-#pragma unroll
-    for (int c = 0; c < 4; ++c) stg(nxt, c * ps + o, ldg((const double*)nxt, c * ps + o) + snext * sat[c]);
-  }
-  if (err) record_error(A.error, err);
-}
-
-hipError_t launch_sat_edges(int mode, const StageArgs& args, int nblocks, int max_perimeter, hipStream_t stream) {
-  if (nblocks <= 0) return hipSuccess;
-  dim3 grid((max_perimeter + 255) / 256, nblocks);
-  sat_edge_kernel<<<grid, 256, 0, stream>>>(args, mode);
   return hipGetLastError();
 }
 

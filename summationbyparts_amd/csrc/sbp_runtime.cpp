@@ -19,23 +19,23 @@
 #include "sbp_operators.hpp"
 
 namespace sbp {
-hipError_t launch_stage_opxi_0(int, bool, int, int, const StageArgs&, int, hipStream_t);
-hipError_t launch_stage_opxi_1(int, bool, int, int, const StageArgs&, int, hipStream_t);
-hipError_t launch_stage_opxi_2(int, bool, int, int, const StageArgs&, int, hipStream_t);
-hipError_t launch_stage_opxi_3(int, bool, int, int, const StageArgs&, int, hipStream_t);
-hipError_t launch_stage_opxi_4(int, bool, int, int, const StageArgs&, int, hipStream_t);
-hipError_t launch_stage_opxi_5(int, bool, int, int, const StageArgs&, int, hipStream_t);
+hipError_t launch_stage_opxi_0(int, bool, bool, int, int, const StageArgs&, int, hipStream_t);
+hipError_t launch_stage_opxi_1(int, bool, bool, int, int, const StageArgs&, int, hipStream_t);
+hipError_t launch_stage_opxi_2(int, bool, bool, int, int, const StageArgs&, int, hipStream_t);
+hipError_t launch_stage_opxi_3(int, bool, bool, int, int, const StageArgs&, int, hipStream_t);
+hipError_t launch_stage_opxi_4(int, bool, bool, int, int, const StageArgs&, int, hipStream_t);
+hipError_t launch_stage_opxi_5(int, bool, bool, int, int, const StageArgs&, int, hipStream_t);
 
-hipError_t launch_stage(int op_eta, int op_xi, bool upwind, int shape, int mode, const StageArgs& args, int ntiles,
-                        hipStream_t stream) {
+hipError_t launch_stage(int op_eta, int op_xi, bool upwind, bool edge, int shape, int mode, const StageArgs& args,
+                        int ntiles, hipStream_t stream) {
   if (ntiles <= 0) return hipSuccess;
   switch (op_xi) {
-    case 0: return launch_stage_opxi_0(op_eta, upwind, shape, mode, args, ntiles, stream);
-    case 1: return launch_stage_opxi_1(op_eta, upwind, shape, mode, args, ntiles, stream);
-    case 2: return launch_stage_opxi_2(op_eta, upwind, shape, mode, args, ntiles, stream);
-    case 3: return launch_stage_opxi_3(op_eta, upwind, shape, mode, args, ntiles, stream);
-    case 4: return launch_stage_opxi_4(op_eta, upwind, shape, mode, args, ntiles, stream);
-    case 5: return launch_stage_opxi_5(op_eta, upwind, shape, mode, args, ntiles, stream);
+    case 0: return launch_stage_opxi_0(op_eta, upwind, edge, shape, mode, args, ntiles, stream);
+    case 1: return launch_stage_opxi_1(op_eta, upwind, edge, shape, mode, args, ntiles, stream);
+    case 2: return launch_stage_opxi_2(op_eta, upwind, edge, shape, mode, args, ntiles, stream);
+    case 3: return launch_stage_opxi_3(op_eta, upwind, edge, shape, mode, args, ntiles, stream);
+    case 4: return launch_stage_opxi_4(op_eta, upwind, edge, shape, mode, args, ntiles, stream);
+    case 5: return launch_stage_opxi_5(op_eta, upwind, edge, shape, mode, args, ntiles, stream);
     default: return hipErrorInvalidValue;
   }
 }
@@ -153,6 +153,10 @@ struct sbp_context_s {
   int device = 0;
   hipStream_t stream = nullptr;
   bool own_stream = false;
+  // second stream of the context (default priority): the edge tiles of a
+  // stage run on it next to the interior tiles; fork/join events order it
+  hipStream_t aux = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   unsigned* d_err = nullptr;
   BlockDev* d_tmpblock = nullptr;
   TileDev* d_tmptiles = nullptr;
@@ -182,7 +186,8 @@ struct Group {
   bool upwind;
   std::vector<int> blocks;
   int shape = -1;
-  int tile_offset = 0, ntiles = 0;
+  int tile_offset = 0, ntiles = 0;    // interior tiles
+  int etile_offset = 0, entiles = 0;  // tiles on block edges (they add the edge SATs)
 };
 
 }  // namespace
@@ -238,11 +243,6 @@ void destroy_graph(sbp_system_s* s) {
 
 int ntiles_along(int n, int t) { return n <= t ? 1 : (n + t - 1) / t; }
 
-int max_perimeter(const sbp_system_s* s) {
-  int m = 0;
-  for (auto& b : s->blocks) m = std::max(m, 2 * b.nx + 2 * (b.ny - 2));
-  return m;
-}
 
 // Device descriptor for block i with the tile counts of `shape`.
 BlockDev make_dev(const sbp_system_s* s, int i, int shape) {
@@ -291,15 +291,7 @@ int tune_kind(const sbp_system_s* s, const Group& g) { return (int)g.upwind + 2 
 // visit 2-8 tile rows of a column before the next column, or narrow vertical
 // strips per XCD, read the shared η halos closer together (the strips raise
 // their L2 hit rate from 39% to 51%) but run 1-20% slower at 4096².
-void build_tiles(const sbp_system_s* s, const Group& g, int shape, std::vector<TileDev>& out) {
-  const TileShape& ts = kTileShapes[shape];
-  std::vector<TileDev> nat;
-  for (int bi : g.blocks) {
-    const HostBlock& h = s->blocks[bi];
-    const int kx = ntiles_along(h.nx, ts.tx), ky = ntiles_along(h.ny, ts.ty);
-    for (int ty = 0; ty < ky; ++ty)
-      for (int tx = 0; tx < kx; ++tx) nat.push_back({bi, (int16_t)tx, (int16_t)ty});
-  }
+void xcd_order(const std::vector<TileDev>& nat, std::vector<TileDev>& out) {
   const int T = (int)nat.size();
   const int X = 8;
   std::vector<int> start((size_t)X + 1, 0);
@@ -308,6 +300,24 @@ void build_tiles(const sbp_system_s* s, const Group& g, int shape, std::vector<T
   for (int j = 0; j < maxlen; ++j)
     for (int x = 0; x < X; ++x)
       if (start[x] + j < start[x + 1]) out.push_back(nat[start[x] + j]);
+}
+
+// The group's interior tiles and its tiles on block edges, each list in XCD order.
+void build_tiles(const sbp_system_s* s, const Group& g, int shape, std::vector<TileDev>& interior,
+                 std::vector<TileDev>& edge) {
+  const TileShape& ts = kTileShapes[shape];
+  std::vector<TileDev> ni, ne;
+  for (int bi : g.blocks) {
+    const HostBlock& h = s->blocks[bi];
+    const int kx = ntiles_along(h.nx, ts.tx), ky = ntiles_along(h.ny, ts.ty);
+    for (int ty = 0; ty < ky; ++ty)
+      for (int tx = 0; tx < kx; ++tx) {
+        const TileDev t{bi, (int16_t)tx, (int16_t)ty};
+        (tx == 0 || tx == kx - 1 || ty == 0 || ty == ky - 1 ? ne : ni).push_back(t);
+      }
+  }
+  xcd_order(ni, interior);
+  xcd_order(ne, edge);
 }
 
 // Arguments of stage `stage` (0..3) of an RK4 step of size dt; the tile
@@ -342,6 +352,26 @@ int normalize(sbp_system_s* s) {
     SBP_TRY_HIP(launch_copy_planes(h.d_buf[BUF_ACC], h.ps, h.ld, h.d_buf[BUF_YN], h.ps, h.ld, 4, h.ny, h.nx, s->ctx->stream));
   s->parity = 0;
   return SBP_OK;
+}
+
+// One stage of one group: its edge tiles (with the edge SATs) on the
+// context's second stream and its interior tiles on `st`, concurrently; `st`
+// continues once both are done (also under graph capture: a fork and a join).
+hipError_t launch_group(const sbp_system_s* s, const Group& g, int shape, int mode, const StageArgs& a, hipStream_t st) {
+  const sbp_context_s* c = s->ctx;
+  hipError_t e;
+  if ((e = hipEventRecord(c->ev_fork, st)) != hipSuccess) return e;
+  if ((e = hipStreamWaitEvent(c->aux, c->ev_fork, 0)) != hipSuccess) return e;
+  StageArgs ae = a;
+  ae.tiles = s->d_tiles + g.etile_offset;
+  const hipError_t ee = launch_stage(g.op_eta, g.op_xi, g.upwind, true, shape, mode, ae, g.entiles, c->aux);
+  StageArgs ai = a;
+  ai.tiles = s->d_tiles + g.tile_offset;
+  const hipError_t ei = ee == hipSuccess ? launch_stage(g.op_eta, g.op_xi, g.upwind, false, shape, mode, ai, g.ntiles, st) : ee;
+  // join even after an error, so that a capture in progress stays balanced
+  if ((e = hipEventRecord(c->ev_join, c->aux)) != hipSuccess) return e;
+  if ((e = hipStreamWaitEvent(st, c->ev_join, 0)) != hipSuccess) return e;
+  return ei;
 }
 
 int rebuild(sbp_system_s* s) {
@@ -382,9 +412,14 @@ int rebuild(sbp_system_s* s) {
     tiles.clear();
     for (auto& g : s->groups) {
       const int sh = g.shape < 0 ? 0 : g.shape;
+      std::vector<TileDev> ti, te;
+      build_tiles(s, g, sh, ti, te);
       g.tile_offset = (int)tiles.size();
-      build_tiles(s, g, sh, tiles);
-      g.ntiles = (int)tiles.size() - g.tile_offset;
+      g.ntiles = (int)ti.size();
+      tiles.insert(tiles.end(), ti.begin(), ti.end());
+      g.etile_offset = (int)tiles.size();
+      g.entiles = (int)te.size();
+      tiles.insert(tiles.end(), te.begin(), te.end());
       for (int bi : g.blocks) devs[bi] = make_dev(s, bi, sh);
     }
     if (s->d_blocks) hipFree(s->d_blocks);
@@ -402,43 +437,49 @@ int rebuild(sbp_system_s* s) {
     // Autotuner: time the stage-1..3 kernels (modes S0, S1, S2 read y_n and
     // write only the accumulator and the stage inputs, so the state is
     // untouched) for every configuration and keep the fastest per group.
+    // two rounds over all configurations, each configuration's time the
+    // smaller of its two (clock and cache state drift between timings)
     std::vector<int> best(s->groups.size(), 0);
     std::vector<float> best_ms(s->groups.size(), 1e30f);
+    std::vector<std::vector<float>> cfg_ms(s->groups.size(), std::vector<float>(kNumTileShapes, 1e30f));
     hipEvent_t e0, e1;
     SBP_TRY_HIP(hipEventCreate(&e0));
     SBP_TRY_HIP(hipEventCreate(&e1));
-    for (int sh = 0; sh < kNumTileShapes; ++sh) {
-      // temporarily give every untuned group this shape
-      std::vector<int> saved;
-      for (auto& g : s->groups) {
-        saved.push_back(g.shape);
-        if (g.shape < 0) g.shape = sh;
+    for (int round = 0; round < 2; ++round)
+      for (int sh = 0; sh < kNumTileShapes; ++sh) {
+        // temporarily give every untuned group this shape
+        std::vector<int> saved;
+        for (auto& g : s->groups) {
+          saved.push_back(g.shape);
+          if (g.shape < 0) g.shape = sh;
+        }
+        SBP_TRY(upload());
+        for (size_t gi = 0; gi < s->groups.size(); ++gi) {
+          Group& g = s->groups[gi];
+          if (saved[gi] >= 0) continue;
+          const StageArgs a = stage_args(s, 0.0, 0, 0);
+          const hipError_t w = launch_group(s, g, sh, MODE_S0, a, st);  // warm-up
+          if (w == hipErrorNotSupported) continue;  // configuration not built for this operator pair
+          SBP_TRY_HIP(w);
+          const int reps = 3;
+          SBP_TRY_HIP(hipEventRecord(e0, st));
+          for (int r = 0; r < reps; ++r)
+            for (int m = MODE_S0; m <= MODE_S2; ++m)
+              SBP_TRY_HIP(launch_group(s, g, sh, m, a, st));
+          SBP_TRY_HIP(hipEventRecord(e1, st));
+          SBP_TRY_HIP(hipEventSynchronize(e1));
+          float ms = 0.f;
+          SBP_TRY_HIP(hipEventElapsedTime(&ms, e0, e1));
+          cfg_ms[gi][sh] = std::min(cfg_ms[gi][sh], ms);
+        }
+        for (size_t gi = 0; gi < s->groups.size(); ++gi) s->groups[gi].shape = saved[gi];
       }
-      SBP_TRY(upload());
-      for (size_t gi = 0; gi < s->groups.size(); ++gi) {
-        Group& g = s->groups[gi];
-        if (saved[gi] >= 0) continue;
-        StageArgs a = stage_args(s, 0.0, 0, 0);
-        a.tiles = s->d_tiles + g.tile_offset;
-        const hipError_t w = launch_stage(g.op_eta, g.op_xi, g.upwind, sh, MODE_S0, a, g.ntiles, st);  // warm-up
-        if (w == hipErrorNotSupported) continue;  // configuration not built for this operator pair
-        SBP_TRY_HIP(w);
-        const int reps = 3;
-        SBP_TRY_HIP(hipEventRecord(e0, st));
-        for (int r = 0; r < reps; ++r)
-          for (int m = MODE_S0; m <= MODE_S2; ++m)
-            SBP_TRY_HIP(launch_stage(g.op_eta, g.op_xi, g.upwind, sh, m, a, g.ntiles, st));
-        SBP_TRY_HIP(hipEventRecord(e1, st));
-        SBP_TRY_HIP(hipEventSynchronize(e1));
-        float ms = 0.f;
-        SBP_TRY_HIP(hipEventElapsedTime(&ms, e0, e1));
-        if (ms < best_ms[gi]) {
-          best_ms[gi] = ms;
+    for (size_t gi = 0; gi < s->groups.size(); ++gi)
+      for (int sh = 0; sh < kNumTileShapes; ++sh)
+        if (cfg_ms[gi][sh] < best_ms[gi]) {
+          best_ms[gi] = cfg_ms[gi][sh];
           best[gi] = sh;
         }
-      }
-      for (size_t gi = 0; gi < s->groups.size(); ++gi) s->groups[gi].shape = saved[gi];
-    }
     hipEventDestroy(e0);
     hipEventDestroy(e1);
     // the timing launches may have raised error bits on garbage accumulators: clear
@@ -470,13 +511,8 @@ int check_err(sbp_system_s* s) {
 // stage and the time update (integrate::<Rk4>, utils/integrate/src/lib.rs:176-222).
 int enqueue_step(sbp_system_s* s, double dt, int swap, hipStream_t st) {
   for (int stage = 0; stage < 4; ++stage) {
-    StageArgs a = stage_args(s, dt, stage, swap);
-    for (const Group& g : s->groups) {
-      StageArgs ag = a;
-      ag.tiles = s->d_tiles + g.tile_offset;
-      SBP_TRY_HIP(launch_stage(g.op_eta, g.op_xi, g.upwind, g.shape, stage, ag, g.ntiles, st));
-    }
-    SBP_TRY_HIP(launch_sat_edges(stage, a, (int)s->blocks.size(), max_perimeter(s), st));
+    const StageArgs a = stage_args(s, dt, stage, swap);
+    for (const Group& g : s->groups) SBP_TRY_HIP(launch_group(s, g, g.shape, stage, a, st));
   }
   SBP_TRY_HIP(launch_advance_time(s->d_time, dt, st));
   return SBP_OK;
@@ -660,7 +696,12 @@ int sbp_context_create(int device, sbp_context* out) {
   }
   c->own_stream = true;
   if ((e = hipMalloc(&c->d_err, sizeof(unsigned))) != hipSuccess || (e = hipMemset(c->d_err, 0, sizeof(unsigned))) != hipSuccess ||
-      (e = hipMalloc(&c->d_tmpblock, sizeof(BlockDev))) != hipSuccess) {
+      (e = hipMalloc(&c->d_tmpblock, sizeof(BlockDev))) != hipSuccess ||
+      (e = hipStreamCreateWithFlags(&c->aux, hipStreamNonBlocking)) != hipSuccess ||
+      (e = hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming)) != hipSuccess ||
+      (e = hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming)) != hipSuccess) {
+    if (c->aux) hipStreamDestroy(c->aux);
+    if (c->ev_fork) hipEventDestroy(c->ev_fork);
     hipStreamDestroy(c->stream);
     delete c;
     return from_hip(e);
@@ -673,7 +714,11 @@ int sbp_context_destroy(sbp_context ctx) {
   if (!ctx) return SBP_ERR_INVALID;
   hipSetDevice(ctx->device);
   hipStreamSynchronize(ctx->stream);
+  hipStreamSynchronize(ctx->aux);
   if (ctx->own_stream) hipStreamDestroy(ctx->stream);
+  hipStreamDestroy(ctx->aux);
+  hipEventDestroy(ctx->ev_fork);
+  hipEventDestroy(ctx->ev_join);
   hipFree(ctx->d_err);
   hipFree(ctx->d_tmpblock);
   if (ctx->d_tmptiles) hipFree(ctx->d_tmptiles);
@@ -819,8 +864,7 @@ int sbp_rhs(sbp_context ctx, int kind, int op_eta, int op_xi, int ny, int nx, do
   a.time = nullptr;
   a.error = ctx->d_err;
   a.gamma = gamma;
-  SBP_TRY_HIP(launch_stage(op_eta, op_xi, up, shape, MODE_RHS, a, (int)tiles.size(), st));
-  SBP_TRY_HIP(launch_sat_edges(MODE_RHS, a, 1, 2 * nx + 2 * (ny - 2), st));
+  SBP_TRY_HIP(launch_stage(op_eta, op_xi, up, true, shape, MODE_RHS, a, (int)tiles.size(), st));
   unsigned w = 0;
   SBP_TRY_HIP(hipMemcpyAsync(&w, ctx->d_err, sizeof(unsigned), hipMemcpyDeviceToHost, st));
   SBP_TRY_HIP(hipStreamSynchronize(st));
@@ -1055,12 +1099,7 @@ int sbp_system_rhs(sbp_system s, double t, double* k_host) {
   StageArgs a = stage_args(s, 0.0, 0, 0);
   a.time = s->d_time_rhs;
   a.rk_b = a.rk_a = 0.0;
-  for (const Group& g : s->groups) {
-    StageArgs ag = a;
-    ag.tiles = s->d_tiles + g.tile_offset;
-    SBP_TRY_HIP(launch_stage(g.op_eta, g.op_xi, g.upwind, g.shape, MODE_RHS, ag, g.ntiles, st));
-  }
-  SBP_TRY_HIP(launch_sat_edges(MODE_RHS, a, (int)s->blocks.size(), max_perimeter(s), st));
+  for (const Group& g : s->groups) SBP_TRY_HIP(launch_group(s, g, g.shape, MODE_RHS, a, st));
   size_t off = 0;
   for (auto& h : s->blocks) {
     for (int c = 0; c < 4; ++c)
@@ -1219,15 +1258,12 @@ int sbp_system_profile(sbp_system s, double dt, int64_t nsteps, double* ms_total
         hipEvent_t a0, a1;
         SBP_TRY_HIP(hipEventCreate(&a0));
         SBP_TRY_HIP(hipEventCreate(&a1));
-        StageArgs ag = a;
-        ag.tiles = s->d_tiles + g.tile_offset;
         SBP_TRY_HIP(hipEventRecord(a0, st));
-        SBP_TRY_HIP(launch_stage(g.op_eta, g.op_xi, g.upwind, g.shape, stage, ag, g.ntiles, st));
+        SBP_TRY_HIP(launch_group(s, g, g.shape, stage, a, st));
         SBP_TRY_HIP(hipEventRecord(a1, st));
         ev.push_back(a0);
         ev.push_back(a1);
       }
-      SBP_TRY_HIP(launch_sat_edges(stage, a, (int)s->blocks.size(), max_perimeter(s), st));
     }
     SBP_TRY_HIP(launch_advance_time(s->d_time, dt, st));
     if (s->precision == SBP_MIXED) s->parity ^= 1;

@@ -5,7 +5,7 @@
 #include <type_traits>
 
 #include "sbp_kernels.hpp"
-#include "sbp_march.hpp"
+#include "sbp_stage.hpp"
 #include "sbp_stage.hpp"
 
 #ifndef SBP_OPX
@@ -17,25 +17,24 @@
 
 namespace sbp {
 
-// Kernel configurations: tiled (tile TX x TY, RY points per thread, caching
-// variant) and marching (strip TX, chunk R rows, segment SL rows, FP64 only);
-// the list matches kTileShapes in sbp_misc.hip.
+// Kernel configurations (tile TX x TY, RY points per thread, caching
+// variant); the list matches kTileShapes in sbp_misc.hip.
 // The mixed form always caches α·q in registers (VAR 0).
-template <class OX, class OY, bool UP, bool MIX, int TX, int TY, int RX, int RY, int VAR>
+template <class OX, class OY, bool UP, bool MIX, bool EDGE, int TX, int TY, int RX, int RY, int VAR>
 static void launch_one(int mode, const StageArgs& args, int ntiles, hipStream_t stream) {
-  stage_kernel<OX, OY, UP, TX, TY, RX, RY, (MIX ? 0 : VAR), MIX>
+  stage_kernel<OX, OY, UP, TX, TY, RX, RY, (MIX ? 0 : VAR), MIX, EDGE>
       <<<ntiles, (TX / RX) * (TY / RY), 0, stream>>>(args, mode);
 }
 
-template <class OX, class OY, bool UP, bool MIX>
+template <class OX, class OY, bool UP, bool MIX, bool EDGE>
 static hipError_t launch_shape(int cfg, int mode, const StageArgs& args, int ntiles, hipStream_t stream) {
   constexpr bool full = std::is_same<OX, OY>::value;  // every configuration only for equal operator pairs
   switch (cfg) {
-    case 0: launch_one<OX, OY, UP, MIX, 32, 16, 1, 2, 1>(mode, args, ntiles, stream); break;
+    case 0: launch_one<OX, OY, UP, MIX, EDGE, 32, 16, 1, 2, 1>(mode, args, ntiles, stream); break;
 #define SBP_CFG(N, TX, TY, RX, RY, VAR)                                                            \
   case N:                                                                                        \
     if constexpr (full) {                                                                        \
-      launch_one<OX, OY, UP, MIX, TX, TY, RX, RY, VAR>(mode, args, ntiles, stream);              \
+      launch_one<OX, OY, UP, MIX, EDGE, TX, TY, RX, RY, VAR>(mode, args, ntiles, stream);        \
       break;                                                                                     \
     }                                                                                            \
     return hipErrorNotSupported;
@@ -47,52 +46,48 @@ static hipError_t launch_shape(int cfg, int mode, const StageArgs& args, int nti
     SBP_CFG(6, 32, 24, 1, 2, 1)
     SBP_CFG(7, 16, 16, 1, 2, 0)
     SBP_CFG(8, 16, 24, 1, 2, 0)
-    SBP_CFG(12, 32, 8, 1, 1, 0)
-    SBP_CFG(13, 32, 16, 1, 1, 0)
-    SBP_CFG(14, 64, 8, 1, 1, 0)
-    SBP_CFG(15, 32, 12, 1, 1, 0)
+    SBP_CFG(9, 32, 8, 1, 1, 0)
+    SBP_CFG(10, 32, 16, 1, 1, 0)
+    SBP_CFG(11, 64, 8, 1, 1, 0)
+    SBP_CFG(12, 32, 12, 1, 1, 0)
 #undef SBP_CFG
-#define SBP_MARCH(N, TX, R, SL)                                                                    \
-  case N:                                                                                        \
-    if constexpr (full && !MIX) {                                                                \
-      march_kernel<OX, OY, UP, TX, R, SL><<<ntiles, TX * R, 0, stream>>>(args, mode);            \
-      break;                                                                                     \
-    }                                                                                            \
-    return hipErrorNotSupported;
-    SBP_MARCH(9, 32, 8, 256)
-    SBP_MARCH(10, 64, 8, 256)
-    SBP_MARCH(11, 32, 8, 512)
-#undef SBP_MARCH
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
 }
 
-template <class OX, class OY, bool MIX>
+template <class OX, class OY, bool MIX, bool EDGE>
 static hipError_t launch_pair_p(bool up, int shape, int mode, const StageArgs& args, int ntiles, hipStream_t stream) {
   if (up) {
-    if constexpr (OX::is_upwind && OY::is_upwind) return launch_shape<OX, OY, true, MIX>(shape, mode, args, ntiles, stream);
+    if constexpr (OX::is_upwind && OY::is_upwind)
+      return launch_shape<OX, OY, true, MIX, EDGE>(shape, mode, args, ntiles, stream);
     return hipErrorInvalidValue;
   }
-  return launch_shape<OX, OY, false, MIX>(shape, mode, args, ntiles, stream);
+  return launch_shape<OX, OY, false, MIX, EDGE>(shape, mode, args, ntiles, stream);
 }
 
+// edge: the tiles on block edges, which add the edge SATs
 template <class OX, class OY>
-static hipError_t launch_pair(bool up, int shape, int mode, const StageArgs& args, int ntiles, hipStream_t stream) {
-  if (args.mixed) return launch_pair_p<OX, OY, true>(up, shape, mode, args, ntiles, stream);
-  return launch_pair_p<OX, OY, false>(up, shape, mode, args, ntiles, stream);
+static hipError_t launch_pair(bool up, bool edge, int shape, int mode, const StageArgs& args, int ntiles,
+                              hipStream_t stream) {
+  if (args.mixed) {
+    if (edge) return launch_pair_p<OX, OY, true, true>(up, shape, mode, args, ntiles, stream);
+    return launch_pair_p<OX, OY, true, false>(up, shape, mode, args, ntiles, stream);
+  }
+  if (edge) return launch_pair_p<OX, OY, false, true>(up, shape, mode, args, ntiles, stream);
+  return launch_pair_p<OX, OY, false, false>(up, shape, mode, args, ntiles, stream);
 }
 
-hipError_t SBP_CAT(launch_stage_opxi_, SBP_OPX_ID)(int op_eta, bool up, int shape, int mode, const StageArgs& args, int ntiles,
-                                                   hipStream_t stream) {
+hipError_t SBP_CAT(launch_stage_opxi_, SBP_OPX_ID)(int op_eta, bool up, bool edge, int shape, int mode, const StageArgs& args,
+                                                   int ntiles, hipStream_t stream) {
   using OX = ops::SBP_OPX;
   switch (op_eta) {
-    case UPWIND4: return launch_pair<OX, ops::Upwind4>(up, shape, mode, args, ntiles, stream);
-    case UPWIND4H2: return launch_pair<OX, ops::Upwind4h2>(up, shape, mode, args, ntiles, stream);
-    case UPWIND9: return launch_pair<OX, ops::Upwind9>(up, shape, mode, args, ntiles, stream);
-    case UPWIND9H2: return launch_pair<OX, ops::Upwind9h2>(up, shape, mode, args, ntiles, stream);
-    case SBP4: return launch_pair<OX, ops::SBP4>(up, shape, mode, args, ntiles, stream);
-    case SBP8: return launch_pair<OX, ops::SBP8>(up, shape, mode, args, ntiles, stream);
+    case UPWIND4: return launch_pair<OX, ops::Upwind4>(up, edge, shape, mode, args, ntiles, stream);
+    case UPWIND4H2: return launch_pair<OX, ops::Upwind4h2>(up, edge, shape, mode, args, ntiles, stream);
+    case UPWIND9: return launch_pair<OX, ops::Upwind9>(up, edge, shape, mode, args, ntiles, stream);
+    case UPWIND9H2: return launch_pair<OX, ops::Upwind9h2>(up, edge, shape, mode, args, ntiles, stream);
+    case SBP4: return launch_pair<OX, ops::SBP4>(up, edge, shape, mode, args, ntiles, stream);
+    case SBP8: return launch_pair<OX, ops::SBP8>(up, edge, shape, mode, args, ntiles, stream);
     default: return hipErrorInvalidValue;
   }
 }

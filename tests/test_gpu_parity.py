@@ -127,7 +127,7 @@ def test_rhs_periodic(gpu, names, shape, rhs_kind):
     assert rel_global(k, k_ref) < RHS_TOL
 
 
-N_TILE_SHAPES = 16
+N_TILE_SHAPES = 13
 
 
 @pytest.mark.parametrize("shape_index", list(range(N_TILE_SHAPES)))
@@ -150,39 +150,6 @@ def test_rhs_all_tile_shapes(gpu, shape_index):
         sys.advance_steps(dt, 1)
         ref_sys.advance(dt, 1)
         assert rel_maxnorm(sys.get_state(0), ref_sys.get_state(0)) < 1e-12, ("step", nx, ny)
-
-
-MARCH_SHAPES = [9, 10, 11]  # strip/segment configurations of the stage kernel
-
-
-@pytest.mark.parametrize("shape_index", MARCH_SHAPES)
-@pytest.mark.parametrize("op_name,shape", [("upwind9", (150, 600)), ("upwind9", (70, 1100)), ("upwind9", (33, 520)),
-                                           ("upwind4", (97, 530)), ("sbp8", (66, 700))])
-def test_march_segments(gpu, shape_index, op_name, shape):
-    """The marching configurations on blocks taller than one segment (the
-    last segment right-aligned over its neighbour) and wider than one strip:
-    RHS and one fused step against the oracle."""
-    from summationbyparts_amd import euler
-    nx, ny = shape
-    X, Y = curvilinear(nx, ny)
-    q = _c2_state(X, Y, seed=7)
-    op = O.OPERATORS[op_name]
-    ref_sys = O.System([(X, Y)], [(op, op)])
-    ref_sys.set_state(0, q)
-    sys = euler.BlockSystem([(X, Y)], [_ops_mod().BY_NAME[op_name]()])
-    sys.set_state(0, q)
-    k_tiled = sys.rhs(0.0)[0]  # default tiled configuration, same device metrics
-    sys.set_tile_shape(shape_index)
-    k = sys.rhs(0.0)[0]
-    assert rel_global(k, k_tiled) < 1e-12
-    # against the oracle's own metrics the rounding of the metrics is amplified
-    # by idx·Σ|c| (idx ≈ 600..1100 here), hence the size-scaled bound
-    assert rel_global(k, ref_sys.rhs(0.0)[0]) < RHS_TOL * max(nx, ny) / 130.0
-    dt = 0.2 / (max(nx, ny) - 1)
-    for _ in range(2):
-        sys.advance_steps(dt, 1)
-        ref_sys.advance(dt, 1)
-        assert rel_maxnorm(sys.get_state(0), ref_sys.get_state(0)) < 1e-12
 
 
 def test_rhs_device_buffers(gpu):

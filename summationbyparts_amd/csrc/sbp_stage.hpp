@@ -21,10 +21,10 @@
 // reference state (face_terms_mix), LDS and stencil sums are FP32, and the
 // stage combination, accumulator and y_n stay FP64; the next stage input is
 // stored as the FP32 increment Δ = a·dt·k.
-// The characteristic SATs of the block-edge points are added to k in the
-// finish by the EDGE instantiation, which the host launches for the tiles on
-// block edges only; the interior tiles' instantiation carries none of the
-// 4x4 characteristic algebra, so its register count stays low.
+// The characteristic SATs of the block-edge points are added to k by the
+// EDGE instantiation, which the host launches for the tiles on block edges
+// only; the interior tiles' instantiation carries none of the 4x4
+// characteristic algebra, so its register count stays low.
 #pragma once
 #include <type_traits>
 
@@ -525,11 +525,64 @@ __global__ __launch_bounds__((TX / RX) * (TY / RY)) void stage_kernel(const Stag
     }
   }
 
+  // ------------------------------------------------------------ edge SATs
+  // Tiles on a block edge (EDGE) compute the characteristic SATs of their
+  // edge points (SAT_characteristics, euler/src/lib.rs:1002-1013: north,
+  // south, east, west in this order; corners get two) between the stencil
+  // phases and the finish loads, one point at a time, and park them in LDS
+  // slots of their own thread; the finish adds them to k. Out of the
+  // stencil phases and before the finish operands are live, the 4x4
+  // characteristic algebra does not raise the kernel's register peak.
+  __shared__ double satl[EDGE ? 4 * RY * RX * NT : 1];
+  if constexpr (EDGE) {
+#pragma unroll
+    for (int s = 0; s < RY; ++s)
+#pragma unroll
+      for (int i = 0; i < RX; ++i) {
+        const int gx = xb + i, gy = yb + s;
+        double sat[4] = {0.0, 0.0, 0.0, 0.0};
+        if (xin[i] && yin[s] && (gx == 0 || gx == nx - 1 || gy == 0 || gy == ny - 1)) {
+          const uint32_t o = (uint32_t)(gy * ld + gx);
+          double q4[4], g[4], mxi0, mxi1, meta0, meta1;
+          if constexpr (OWNQ) {
+#pragma unroll
+            for (int q = 0; q < 4; ++q) q4[q] = ownq[s][q];
+            mxi0 = xsave[(2 * s) * NT + tid];
+            mxi1 = xsave[(2 * s + 1) * NT + tid];
+            meta0 = ownm[s][0];
+            meta1 = ownm[s][1];
+          } else {
+            load_stage_state(A, B, mode, o, q4);
+            mxi0 = ldg(met, ps + o);
+            mxi1 = ldg(met, 2 * ps + o);
+            meta0 = ldg(met, 3 * ps + o);
+            meta1 = ldg(met, 4 * ps + o);
+          }
+          const double dj = ldg(met, o);
+          const double time = ((A.time ? *A.time : 0.0) + A.time_offset) * B.dt_scale;
+          if (gy == ny - 1) {
+            edge_data(A, B, SBP_NORTH, gx, gx, gy, mode, time, g, err);
+            sat_point(sat, q4, g, B.hi_eta, -1.0, 1.0, dj, meta0, meta1, gamma);
+          }
+          if (gy == 0) {
+            edge_data(A, B, SBP_SOUTH, gx, gx, gy, mode, time, g, err);
+            sat_point(sat, q4, g, B.hi_eta, 1.0, -1.0, dj, meta0, meta1, gamma);
+          }
+          if (gx == nx - 1) {
+            edge_data(A, B, SBP_EAST, gy, gx, gy, mode, time, g, err);
+            sat_point(sat, q4, g, B.hi_xi, -1.0, 1.0, dj, mxi0, mxi1, gamma);
+          }
+          if (gx == 0) {
+            edge_data(A, B, SBP_WEST, gy, gx, gy, mode, time, g, err);
+            sat_point(sat, q4, g, B.hi_xi, 1.0, -1.0, dj, mxi0, mxi1, gamma);
+          }
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q) satl[((s * RX + i) * 4 + q) * NT + tid] = sat[q];
+      }
+  }
+
   // ------------------------------------------------------------ finish
-  // Tiles on a block edge (EDGE) add the characteristic SATs of their edge
-  // points to k (SAT_characteristics, euler/src/lib.rs:1002-1013: north,
-  // south, east, west in this order; corners get two); the 4x4
-  // characteristic algebra lives only in this instantiation.
   load_finish();
   // (b_s·dt)·k and (a_s·dt)·k with this block's step (integrate, lib.rs:188-211)
   const double dtb = A.dt * B.dt_scale;
@@ -545,47 +598,8 @@ __global__ __launch_bounds__((TX / RX) * (TY / RY)) void stage_kernel(const Stag
 #pragma unroll
     for (int q = 0; q < 4; ++q) k[q] = (double)acc[s][i][q] * ij;
     if constexpr (EDGE) {
-      const int gx = xb + i, gy = yb + s;
-      if (gx == 0 || gx == nx - 1 || gy == 0 || gy == ny - 1) {
-        double q4[4], g[4];
-        if constexpr (OWNQ) {
 #pragma unroll
-          for (int q = 0; q < 4; ++q) q4[q] = ownq[s][q];
-        } else {
-          load_stage_state(A, B, mode, o, q4);
-        }
-        // metric products here: J·ξx, J·ξy (E/W edges), J·ηx, J·ηy (N/S edges)
-        double mxi0, mxi1, meta0, meta1;
-        if constexpr (OWNQ) {
-          mxi0 = xsave[(2 * s) * NT + tid];
-          mxi1 = xsave[(2 * s + 1) * NT + tid];
-          meta0 = ownm[s][0];
-          meta1 = ownm[s][1];
-        } else {
-          mxi0 = ldg(met, ps + o);
-          mxi1 = ldg(met, 2 * ps + o);
-          meta0 = ldg(met, 3 * ps + o);
-          meta1 = ldg(met, 4 * ps + o);
-        }
-        const double dj = fJ[s][i];
-        const double time = ((A.time ? *A.time : 0.0) + A.time_offset) * B.dt_scale;
-        if (gy == ny - 1) {
-          edge_data(A, B, SBP_NORTH, gx, gx, gy, mode, time, g, err);
-          sat_point(k, q4, g, B.hi_eta, -1.0, 1.0, dj, meta0, meta1, gamma);
-        }
-        if (gy == 0) {
-          edge_data(A, B, SBP_SOUTH, gx, gx, gy, mode, time, g, err);
-          sat_point(k, q4, g, B.hi_eta, 1.0, -1.0, dj, meta0, meta1, gamma);
-        }
-        if (gx == nx - 1) {
-          edge_data(A, B, SBP_EAST, gy, gx, gy, mode, time, g, err);
-          sat_point(k, q4, g, B.hi_xi, -1.0, 1.0, dj, mxi0, mxi1, gamma);
-        }
-        if (gx == 0) {
-          edge_data(A, B, SBP_WEST, gy, gx, gy, mode, time, g, err);
-          sat_point(k, q4, g, B.hi_xi, 1.0, -1.0, dj, mxi0, mxi1, gamma);
-        }
-      }
+      for (int q = 0; q < 4; ++q) k[q] += satl[((s * RX + i) * 4 + q) * NT + tid];
     }
     if (mode == MODE_RHS) {
 #pragma unroll

@@ -46,6 +46,64 @@ REFERENCE_SCHEDULE_BYTES = 672.0
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "stage_kernel_upwind9_4096_traffic.json")
 
 
+def _counter_rows(path, counter):
+    """(dispatch id, kernel name, value) of one counter in a rocprofv3 --pmc
+    CSV (run_counter_collection.csv), in dispatch order."""
+    import csv
+    rows = []
+    with open(path) as f:
+        for r in csv.DictReader(f):
+            if r["Counter_Name"] == counter:
+                rows.append((int(r["Dispatch_Id"]), r["Kernel_Name"], float(r["Counter_Value"])))
+    rows.sort()
+    return rows
+
+
+def summarize_counters(fetch_csv, write_csv, copy_fetch_csv, copy_write_csv, n, tile_index, precision):
+    """HBM bytes per stage launch from FETCH_SIZE / WRITE_SIZE (KiB per
+    dispatch) of a run of this bench, scaled by what the same counters read
+    for the 2 GiB device copy of `--calibrate-copy`. A stage launch is the
+    interior-tile dispatch plus the edge-tile dispatch (consecutive in the
+    trace); launches cycle through the four RK4 stages; the first step is
+    left out (cold caches)."""
+    copy_bytes = float(1 << 31)
+
+    def factor(path, counter):
+        vals = sorted(v for _, k, v in _counter_rows(path, counter) if "copyBuffer" in k)
+        return copy_bytes / (vals[len(vals) // 2] * 1024.0)
+
+    ff, wf = factor(copy_fetch_csv, "FETCH_SIZE"), factor(copy_write_csv, "WRITE_SIZE")
+
+    def per_launch(path, counter, scale):
+        rows = [(k, v) for _, k, v in _counter_rows(path, counter) if "stage_kernel" in k]
+        if len(rows) % 2:
+            raise SystemExit(f"{path}: odd number of stage dispatches")
+        kernel = next(k for k, _ in rows if k.rstrip(")").split("(")[0].endswith("false>"))
+        pairs = [(rows[i][1] + rows[i + 1][1]) * 1024.0 * scale for i in range(0, len(rows), 2)]
+        stages = [[] for _ in range(4)]
+        for i, v in enumerate(pairs[4:]):
+            stages[i % 4].append(v)
+        return kernel, [sum(s) / len(s) for s in stages]
+
+    kernel, fetch = per_launch(fetch_csv, "FETCH_SIZE", ff)
+    _, write = per_launch(write_csv, "WRITE_SIZE", wf)
+    return {"tile_index": tile_index, "tile": list(_tile_of(tile_index)), "kernel": kernel.split("(")[0],
+            "hbm_bytes_per_launch": (sum(fetch) + sum(write)) / 4.0,
+            "algorithmic_bytes_per_launch": BYTES_PER_POINT_STEP[precision] / 4.0 * n * n,
+            "fetch_factor": ff, "write_factor": wf,
+            "per_stage": [{"stage": s, "fetch_bytes": fetch[s], "write_bytes": write[s]} for s in range(4)]}
+
+
+def _tile_of(index):
+    """(TX, TY, RY) of a tile shape index (a host-side table lookup)."""
+    import ctypes
+    from summationbyparts_amd import _lib
+    a, b, c = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+    _lib.check(_lib.lib().sbp_tile_shape_get(index, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c)),
+               "sbp_tile_shape_get")
+    return a.value, b.value, c.value
+
+
 class Workload:
     """A system ready to step, its step size, point count, description and a
     bounded CPU-oracle sample."""
@@ -188,7 +246,13 @@ def main():
     ap.add_argument("--tile", type=int, default=-1, help="force a tile shape index (-1: autotune)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain 2 GiB device copy (reference kernel for HBM byte counters)")
+    ap.add_argument("--counters", nargs=4, metavar=("FETCH_CSV", "WRITE_CSV", "COPY_FETCH_CSV", "COPY_WRITE_CSV"),
+                    help="summarize rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE runs of this bench (with --tile) "
+                         "into HBM bytes per stage launch, calibrated by two runs of --calibrate-copy")
     args = ap.parse_args()
+    if args.counters:
+        print(json.dumps(summarize_counters(*args.counters, args.n, args.tile, args.precision)))
+        return
     if args.calibrate_copy:
         import torch
 
@@ -244,7 +308,8 @@ def main():
     sys_.advance_steps(dt, args.warmup)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    sys_.advance_steps(dt, args.steps)
+    # HIP events on the library's stream bracket the K steps (graph replays)
+    ms_timed = sys_.advance_timed(dt, args.steps)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     upload_ms = None
@@ -256,24 +321,27 @@ def main():
         sys_.set_state(0, q)
         torch.cuda.synchronize()
         upload_ms = (time.perf_counter() - t1) * 1e3
-    # after the timed region: HIP-event time of 8 graph-replayed steps, then
-    # the mean duration of one stage-kernel launch over 8 more steps (events
-    # on the library's stream around each launch)
-    ms_events, ms_stage = sys_.profile(dt, 8)
-    tile = sys_.tile_shape()
+    # mean duration of one stage launch (the stage kernel over the interior
+    # tiles, concurrent with its edge-tile instantiation) over the timed
+    # region: 4 launches per step plus a 1-thread time update
+    ms_stage = ms_timed / (4 * args.steps)
+    tile, tile_index = sys_.tile_shape(), sys_.tile_index()
     pts = w.points
     value = pts * args.steps / wall / 1e9
     bytes_per_launch = BYTES_PER_POINT_STEP[args.precision] / 4.0 * pts
     achieved = bytes_per_launch / (ms_stage * 1e-3) / 1e9
+    # HBM bytes per launch measured with rocprofv3 counters for this
+    # workload and tile shape, if recorded
     traffic = None
     if os.path.exists(TRAFFIC_FILE) and args.workload == "c2":
         with open(TRAFFIC_FILE) as f:
             tr = json.load(f)
-        if (tr.get("n") == args.n and tr.get("precision", "fp64") == args.precision
-                and tr.get("tile") == list(sys_.tile_shape())):
-            traffic = tr.get("hbm_bytes_per_launch")
+        if tr.get("n") == args.n and tr.get("precision", "fp64") == args.precision:
+            for e in tr.get("tiles", []):
+                if e.get("tile_index") == tile_index:
+                    traffic = e.get("hbm_bytes_per_launch")
     cfg = {"workload": w.desc, **w.cfg, "precision": args.precision, "tile": list(tile),
-           "event_ms_per_step": ms_events / 8, "stage_launch_ms": ms_stage}
+           "tile_index": tile_index, "event_ms_per_step": ms_timed / args.steps, "stage_launch_ms": ms_stage}
     if upload_ms is not None:
         cfg["state_upload_ms"] = upload_ms
     out = {

@@ -196,8 +196,15 @@ int sbp_system_check(sbp_system sys);
  * every stage (the launches of its edge and interior tiles) bracketed by its
  * own events and returns the mean stage duration in *ms_stage_avg. */
 int sbp_system_profile(sbp_system sys, double dt, int64_t nsteps, double* ms_total, double* ms_stage_avg);
+/* sbp_system_advance(sys, dt, nsteps) with HIP events recorded on the
+ * context stream around the steps; *ms = their elapsed time. A graph capture
+ * the steps need happens before the first event. */
+int sbp_system_advance_timed(sbp_system sys, double dt, int64_t nsteps, double* ms);
 /* Tile shape currently used for block 0 (after autotuning). */
 int sbp_system_tile_shape(sbp_system sys, int* tx, int* ty, int* ry);
+/* Index into the library's shape list of the tile shape used for block 0
+ * (after autotuning); shapes of one (TX, TY, RY) differ in their variant. */
+int sbp_system_tile_index(sbp_system sys, int* index);
 /* Force a tile shape (index into the library's shape list) or -1 = autotune. */
 int sbp_system_set_tile_shape(sbp_system sys, int shape_index);
 int sbp_tile_shape_count(void);

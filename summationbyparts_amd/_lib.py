@@ -101,6 +101,8 @@ def _declare(L):
         "sbp_system_profile": ([c_void_p, c_double, c_int64, dp, dp], c_int),
         "sbp_system_tile_shape": ([c_void_p, ip, ip, ip], c_int),
         "sbp_system_set_tile_shape": ([c_void_p, c_int], c_int),
+        "sbp_system_tile_index": ([c_void_p, ip], c_int),
+        "sbp_system_advance_timed": ([c_void_p, c_double, c_int64, dp], c_int),
         "sbp_tile_shape_count": ([], c_int),
         "sbp_tile_shape_get": ([c_int, ip, ip, ip], c_int),
     }
@@ -119,7 +121,8 @@ EXPORTED = [
     "sbp_system_get_metrics", "sbp_system_vortex", "sbp_system_vortex_block", "sbp_system_set_dt_scale",
     "sbp_system_error_vortex_block", "sbp_system_rhs", "sbp_system_advance", "sbp_system_get_time",
     "sbp_system_set_time", "sbp_system_max_dt", "sbp_system_error_vortex", "sbp_system_energy", "sbp_system_check",
-    "sbp_system_profile", "sbp_system_tile_shape", "sbp_system_set_tile_shape", "sbp_tile_shape_count",
+    "sbp_system_profile", "sbp_system_tile_shape", "sbp_system_set_tile_shape", "sbp_system_tile_index", "sbp_system_advance_timed",
+    "sbp_tile_shape_count",
     "sbp_tile_shape_get",
 ]
 

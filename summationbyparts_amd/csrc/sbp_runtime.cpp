@@ -1283,6 +1283,32 @@ int sbp_system_profile(sbp_system s, double dt, int64_t nsteps, double* ms_total
   return check_err(s);
 }
 
+int sbp_system_advance_timed(sbp_system s, double dt, int64_t nsteps, double* ms) {
+  if (!s || nsteps <= 0 || !ms || !std::isfinite(dt)) return SBP_ERR_INVALID;
+  double gamma;
+  if (!gamma_get(gamma)) return SBP_ERR_GAMMA;
+  SBP_TRY_HIP(hipSetDevice(s->ctx->device));
+  SBP_TRY(rebuild(s));
+  hipStream_t st = s->ctx->stream;
+  const int G = steps_per_graph(s);
+  if (nsteps >= G) SBP_TRY(ensure_graph(s, dt, G));
+  hipEvent_t e0, e1;
+  SBP_TRY_HIP(hipEventCreate(&e0));
+  SBP_TRY_HIP(hipEventCreate(&e1));
+  hipError_t he = hipEventRecord(e0, st);
+  int rc = he == hipSuccess ? advance_impl(s, dt, nsteps) : from_hip(he);
+  if (rc == SBP_OK) he = hipEventRecord(e1, st);
+  if (rc == SBP_OK && he == hipSuccess) he = hipEventSynchronize(e1);
+  float m = 0.f;
+  if (rc == SBP_OK && he == hipSuccess) he = hipEventElapsedTime(&m, e0, e1);
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  if (rc != SBP_OK) return rc;
+  if (he != hipSuccess) return from_hip(he);
+  *ms = m;
+  return check_err(s);
+}
+
 int sbp_system_tile_shape(sbp_system s, int* tx, int* ty, int* ry) {
   if (!s) return SBP_ERR_INVALID;
   SBP_TRY(rebuild(s));
@@ -1290,6 +1316,13 @@ int sbp_system_tile_shape(sbp_system s, int* tx, int* ty, int* ry) {
   if (tx) *tx = t.tx;
   if (ty) *ty = t.ty;
   if (ry) *ry = t.ry;
+  return SBP_OK;
+}
+
+int sbp_system_tile_index(sbp_system s, int* index) {
+  if (!s || !index) return SBP_ERR_INVALID;
+  SBP_TRY(rebuild(s));
+  *index = s->groups.empty() ? 0 : s->groups[0].shape;
   return SBP_OK;
 }
 

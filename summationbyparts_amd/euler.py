@@ -114,6 +114,11 @@ class BlockSystem:
                    "sbp_system_tile_shape")
         return a.value, b.value, c.value
 
+    def tile_index(self):
+        i = ctypes.c_int()
+        _lib.check(_lib.lib().sbp_system_tile_index(self._h, ctypes.byref(i)), "sbp_system_tile_index")
+        return i.value
+
     # ---- state
     def set_state(self, block, q):
         q = np.ascontiguousarray(q, dtype=np.float64)
@@ -192,6 +197,14 @@ class BlockSystem:
         out = ctypes.c_double()
         _lib.check(_lib.lib().sbp_system_energy(self._h, ctypes.byref(out)), "sbp_system_energy")
         return out.value
+
+    def advance_timed(self, dt, nsteps):
+        """advance_steps(dt, nsteps); returns the ms between HIP events
+        recorded on the library's stream around the steps."""
+        ms = ctypes.c_double()
+        _lib.check(_lib.lib().sbp_system_advance_timed(self._h, dt, nsteps, ctypes.byref(ms)),
+                   "sbp_system_advance_timed")
+        return ms.value
 
     def profile(self, dt, nsteps):
         """(ms for nsteps graph-replayed steps, mean ms of one stage-kernel launch)."""

@@ -152,6 +152,25 @@ def test_rhs_all_tile_shapes(gpu, shape_index):
         assert rel_maxnorm(sys.get_state(0), ref_sys.get_state(0)) < 1e-12, ("step", nx, ny)
 
 
+def test_advance_timed_and_tile_index(gpu):
+    """The timed advance steps the state bit for bit as the plain advance
+    does and returns a positive event time; the tile index names the forced
+    shape."""
+    from summationbyparts_amd import euler
+    nx, ny = 96, 80
+    X, Y = curvilinear(nx, ny)
+    q = _c2_state(X, Y, seed=2)
+    dt = 0.2 / (nx - 1)
+    a, b = (euler.BlockSystem([(X, Y)], [_ops_mod().Upwind9()]) for _ in range(2))
+    for s in (a, b):
+        s.set_tile_shape(8)
+        s.set_state(0, q)
+    a.advance_steps(dt, 9)
+    assert b.advance_timed(dt, 9) > 0.0
+    assert np.array_equal(a.get_state(0), b.get_state(0))
+    assert b.tile_index() == 8 and b.tile_shape() == (16, 24, 2)
+
+
 def test_rhs_device_buffers(gpu):
     """euler::RHS_upwind on caller-owned device buffers with explicit boundary data."""
     from summationbyparts_amd import euler

@@ -32,8 +32,9 @@ HBM_PEAK_GBS = 8000.0
 # own-point operands of its stage combination, and writes its outputs once).
 # FP64 (J is formed from the four metric products, so 4 metric planes):
 # stage 1 reads y_n, metrics, writes Y2 (96 B); stage 2 reads Y2, metrics,
-# y_n, writes Y3 (128 B); stage 3 reads Y3, metrics, y_n, writes Y4 (128 B);
-# stage 4 reads Y4, metrics, Y2, Y3, y_n and writes y_{n+1} (192 B): 544 B.
+# y_n, writes Y3 (128 B); stage 3 reads Y3, metrics, y_n, Y2, writes Y4 and
+# W = Y2 + 2 Y3 - y_n (192 B); stage 4 reads Y4, metrics, W and writes
+# y_{n+1} (128 B): 544 B.
 # The reference's schedule (running accumulator, 5 metric planes) needs
 # 672 B; bench also reports the rate at which that schedule's bytes would
 # have to move to match this kernel ("reference_schedule").

@@ -37,14 +37,20 @@ __device__ __forceinline__ double fma_t(double a, double b, double c) { return _
 __device__ __forceinline__ float fma_t(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
 
 // Load through a uniform base pointer and a 32-bit element offset (selects
-// the SGPR-base + VGPR-offset addressing of global loads).
+// the SGPR-base + VGPR-offset addressing of global loads). The pointers come
+// from device descriptors, so the cast to the global address space is what
+// keeps the compiler from emitting flat loads (which also wait on LDS).
 template <typename T>
 __device__ __forceinline__ T ldg(const T* base, uint32_t idx) {
-  return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + (idx << (sizeof(T) == 8 ? 3 : 2)));
+  using GP = const __attribute__((address_space(1))) T*;
+  using GC = const __attribute__((address_space(1))) char*;
+  return *(GP)((GC)base + (idx << (sizeof(T) == 8 ? 3 : 2)));
 }
 template <typename T>
 __device__ __forceinline__ void stg(T* base, uint32_t idx, T v) {
-  *reinterpret_cast<T*>(reinterpret_cast<char*>(base) + (idx << (sizeof(T) == 8 ? 3 : 2))) = v;
+  using GP = __attribute__((address_space(1))) T*;
+  using GC = __attribute__((address_space(1))) char*;
+  *(GP)((GC)base + (idx << (sizeof(T) == 8 ? 3 : 2))) = v;
 }
 // Streaming store/load (non-temporal hint): for data written or read once per
 // stage, so that it neither allocates nor evicts L2 lines holding halos.

@@ -7,8 +7,9 @@
 namespace sbp {
 
 // Roles of the four state buffers of a block during one fused RK4 step.
-// FP64 form: y_n and the stage inputs Y2 (YA), Y3 (YB) and Y4 (ACC); the
-// last stage forms y_{n+1} from them (see the stage kernel's finish).
+// FP64 form: y_n and the stage inputs Y2 (YA), Y3 (YB) and Y4 (ACC); stage 3
+// overwrites Y2 with W = Y2 + 2 Y3 - y_n, from which and Y4 the last stage
+// forms y_{n+1} (see the stage kernel's finish).
 // Mixed form: YA
 // holds the FP32 stage increments Δ_s = Y_s - y_n (two sets of 4 planes, the
 // second at float offset 4·ps), and the last stage writes y_{n+1} over the

@@ -113,7 +113,9 @@ __device__ __forceinline__ void col_window(const T* p, int stride, T out[RS][RX]
 // region points in registers between the flux and dissipation passes; VAR 1
 // keeps only α and re-reads q (cache hits); VAR 2 stores flux and α·q
 // together (8 values per point in LDS, one pass per direction). The mixed
-// form uses VAR 0.
+// form uses VAR 0. (VAR 1 at 16x24 and 16x16 fits 5 waves per SIMD, at most
+// 96 VGPRs, but its re-reads cost more than the occupancy gains: 6.2 and 5.8
+// against 7.2 Gpoint-steps/s at 4096².)
 // Threads own RX adjacent ξ-columns and RY consecutive η-rows of the tile.
 template <class OX, class OY, bool UP, int TX, int TY, int RX, int RY, int VAR, bool MIX = false, bool EDGE = false>
 __global__ __launch_bounds__((TX / RX) * (TY / RY)) void stage_kernel(const StageArgs A, const int mode) {
@@ -331,10 +333,9 @@ __global__ __launch_bounds__((TX / RX) * (TY / RY)) void stage_kernel(const Stag
 
   // Finish operands at the output points, all loaded before any store.
   // FP64 (see the finish), with q the stage input held from the η phase:
-  //   S0 q = y_n;  S1 fa = y_n;  S2 fa = y_n;  S3 fa = Y2, fb = Y3, fc = y_n, q = Y4
+  //   S0 q = y_n;  S1 fa = y_n, q = Y2;  S2 fa = y_n, fb = Y2, q = Y3;  S3 fa = W, q = Y4
   // mixed: S0 fa = y_n;  S1, S2 fb = acc;  S3 fa = acc
-  constexpr int FC = MIX ? 1 : RY;
-  double fJ[RY][RX], fa[RY][RX][4], fb[RY][RX][4], fc[FC][RX][4];
+  double fJ[RY][RX], fa[RY][RX][4], fb[RY][RX][4];
   static_assert(MIX || OWNQ, "the FP64 stage combination takes the stage input from the η phase");
   auto load_finish = [&]() {
     const double* pacc = MIX ? acc_buf(B, A) : B.buf[BUF_ACC];
@@ -360,16 +361,13 @@ __global__ __launch_bounds__((TX / RX) * (TY / RY)) void stage_kernel(const Stag
             for (int q = 0; q < 4; ++q) fb[s][i][q] = ldg_nt(pacc, q * ps + o);
           }
         } else if (mode != MODE_S0) {
-          // S1, S2: fa = y_n; S3: fa = Y2, fb = Y3, fc = y_n
+          // S1, S2: fa = y_n; S2 also fb = Y2; S3: fa = W (in the Y2 buffer)
           const double* pa = B.buf[mode == MODE_S3 ? BUF_YA : BUF_YN];
 #pragma unroll
           for (int q = 0; q < 4; ++q) fa[s][i][q] = ldg_nt(pa, q * ps + o);
-          if (mode == MODE_S3) {
+          if (mode == MODE_S2) {
 #pragma unroll
-            for (int q = 0; q < 4; ++q) {
-              fb[s][i][q] = ldg_nt((const double*)B.buf[BUF_YB], q * ps + o);
-              fc[s][i][q] = ldg_nt((const double*)B.buf[BUF_YN], q * ps + o);
-            }
+            for (int q = 0; q < 4; ++q) fb[s][i][q] = ldg_nt((const double*)B.buf[BUF_YA], q * ps + o);
           }
         }
       }
@@ -624,22 +622,27 @@ __global__ __launch_bounds__((TX / RX) * (TY / RY)) void stage_kernel(const Stag
       // Y2 = y + dt/2 k1, Y3 = y + dt/2 k2, Y4 = y + dt k3 and
       // y_{n+1} = y + dt/6 (k1 + 2k2 + 2k3 + k4) = (Y2 + 2 Y3 + Y4 - y)/3 + dt/6 k4,
       // an identity of the tableau (dt k1 = 2(Y2 - y), dt k2 = 2(Y3 - y),
-      // dt k3 = Y4 - y). No accumulator is kept between stages; the rounding
-      // differs from the reference's running sum by a few ulps of y.
+      // dt k3 = Y4 - y). Stage 3, which holds Y3 and reads y and Y2 at its
+      // points, also forms W = Y2 + 2 Y3 - y over Y2 (no longer needed), so
+      // that stage 4 reads one operand: y_{n+1} = (W + Y4)/3 + dt/6 k4. No
+      // accumulator is kept between stages; the rounding differs from the
+      // reference's running sum by a few ulps of y.
       const double* qs = ownq[OWNQ ? s : 0];  // the stage input here
       if (mode == MODE_S0) {
 #pragma unroll
         for (int q = 0; q < 4; ++q) stg_nt(B.buf[BUF_YA], q * ps + o, qs[q] + snext * k[q]);
-      } else if (mode == MODE_S1 || mode == MODE_S2) {
-        double* nxt = B.buf[mode == MODE_S1 ? BUF_YB : BUF_ACC];
+      } else if (mode == MODE_S1) {
 #pragma unroll
-        for (int q = 0; q < 4; ++q) stg_nt(nxt, q * ps + o, fa[s][i][q] + snext * k[q]);
-      } else {
+        for (int q = 0; q < 4; ++q) stg_nt(B.buf[BUF_YB], q * ps + o, fa[s][i][q] + snext * k[q]);
+      } else if (mode == MODE_S2) {
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
-          const double sum = fa[s][i][q] + 2.0 * fb[s][i][q] + qs[q] - fc[s][i][q];
-          stg_nt(B.buf[BUF_YN], q * ps + o, sum * (1.0 / 3.0) + sacc * k[q]);
+          stg_nt(B.buf[BUF_ACC], q * ps + o, fa[s][i][q] + snext * k[q]);
+          stg_nt(B.buf[BUF_YA], q * ps + o, fma_t(2.0, qs[q], fb[s][i][q]) - fa[s][i][q]);
         }
+      } else {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) stg_nt(B.buf[BUF_YN], q * ps + o, (fa[s][i][q] + qs[q]) * (1.0 / 3.0) + sacc * k[q]);
       }
     }
   }

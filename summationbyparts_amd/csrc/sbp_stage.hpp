@@ -190,6 +190,9 @@ __global__ __launch_bounds__((TX / RX) * (TY / RY)) void stage_kernel(const Stag
   // replaces a load of J.
   double ownm[OWNQ ? RY : 1][2];
   __shared__ double xsave[OWNQ ? 2 * RY * NT : 1];
+  // stage input at the own output points (OWNQ): loaded in the ξ phase, used
+  // again by the η phase (which loads only J·ηx, J·ηy there) and the finish
+  double ownq[OWNQ ? RY : 1][4];
 
   // ------------------------------------------------------------ ξ-phase
   {
@@ -246,6 +249,8 @@ __global__ __launch_bounds__((TX / RX) * (TY / RY)) void stage_kernel(const Stag
           for (int s2 = 0; s2 < RY; ++s2) {
             xsave[(2 * s2) * NT + tid] = v[s2][4];
             xsave[(2 * s2 + 1) * NT + tid] = v[s2][5];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) ownq[s2][q] = v[s2][q];
           }
         }
       }
@@ -373,8 +378,6 @@ __global__ __launch_bounds__((TX / RX) * (TY / RY)) void stage_kernel(const Stag
       }
   };
 
-  double ownq[OWNQ ? RY : 1][4];  // stage input at the own output points (OWNQ)
-
   // ------------------------------------------------------------ η-phase
   {
     using BD = Blk<OY, SBP_DIFF>;
@@ -427,7 +430,12 @@ __global__ __launch_bounds__((TX / RX) * (TY / RY)) void stage_kernel(const Stag
         int r, c;
         const bool ok = i0 + j < ITYE && eta_pt(i0 + j, r, c);
         const uint32_t o = ok ? (uint32_t)((ry0 + r) * ld + x0 + c) : (uint32_t)(ry0 * ld + x0);
-        load_q(o, v[j]);
+        if (OWNQ && i0 + j < RY) {
+#pragma unroll
+          for (int q = 0; q < 4; ++q) v[j][q] = ownq[OWNQ ? i0 + j : 0][q];
+        } else {
+          load_q(o, v[j]);
+        }
         v[j][4] = ldg(met, 3 * ps + o);
         v[j][5] = ldg(met, 4 * ps + o);
       }
@@ -435,8 +443,6 @@ __global__ __launch_bounds__((TX / RX) * (TY / RY)) void stage_kernel(const Stag
         if (i0 == 0) {
 #pragma unroll
           for (int s2 = 0; s2 < RY; ++s2) {
-#pragma unroll
-            for (int q = 0; q < 4; ++q) ownq[s2][q] = v[s2][q];
             ownm[s2][0] = v[s2][4];
             ownm[s2][1] = v[s2][5];
           }

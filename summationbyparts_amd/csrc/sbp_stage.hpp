@@ -242,6 +242,11 @@ __global__ __launch_bounds__((TX / RX) * (TY / RY)) void stage_kernel(const Stag
         load_q(o, v[j]);
         v[j][4] = ldg(met, ps + o);
         v[j][5] = ldg(met, 2 * ps + o);
+        // the own points' η metrics are issued with them (used by the η phase)
+        if (OWNQ && i0 + j < RY) {
+          ownm[OWNQ ? i0 + j : 0][0] = ldg(met, 3 * ps + o);
+          ownm[OWNQ ? i0 + j : 0][1] = ldg(met, 4 * ps + o);
+        }
       }
       if constexpr (OWNQ) {
         if (i0 == 0) {
@@ -433,19 +438,12 @@ __global__ __launch_bounds__((TX / RX) * (TY / RY)) void stage_kernel(const Stag
         if (OWNQ && i0 + j < RY) {
 #pragma unroll
           for (int q = 0; q < 4; ++q) v[j][q] = ownq[OWNQ ? i0 + j : 0][q];
+          v[j][4] = ownm[OWNQ ? i0 + j : 0][0];
+          v[j][5] = ownm[OWNQ ? i0 + j : 0][1];
         } else {
           load_q(o, v[j]);
-        }
-        v[j][4] = ldg(met, 3 * ps + o);
-        v[j][5] = ldg(met, 4 * ps + o);
-      }
-      if constexpr (OWNQ) {
-        if (i0 == 0) {
-#pragma unroll
-          for (int s2 = 0; s2 < RY; ++s2) {
-            ownm[s2][0] = v[s2][4];
-            ownm[s2][1] = v[s2][5];
-          }
+          v[j][4] = ldg(met, 3 * ps + o);
+          v[j][5] = ldg(met, 4 * ps + o);
         }
       }
 #pragma unroll

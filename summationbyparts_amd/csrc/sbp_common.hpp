@@ -115,23 +115,28 @@ __device__ __forceinline__ T interior_sum(const T* p, int stride) {
 
 // Full operator row i of an axis of length n: start block, interior or end
 // block (algos.rs:101-187). `line` points at axis index r0, stride `stride`.
-// The closure loops are kept rolled: their coefficient rows are selected per
-// lane at run time, and unrolling them would hold a whole row in registers.
-template <class B, typename T>
+// The coefficient row of a closure is selected per lane at run time (read
+// from the constant tables); the N taps are unrolled so that their loads are
+// in flight together. CL = false: the caller never asks for a closure row
+// (the stage kernel's tiles away from block edges), and the closure code is
+// left out.
+template <class B, bool CL = true, typename T>
 __device__ __forceinline__ T row_sum(const T* line, int stride, int i, int n, int r0) {
-  if (i < B::M) {
-    T s = T(0);
-#pragma unroll 1
-    for (int k = 0; k < B::N; ++k) s += T(B::start(i, k)) * line[(k - r0) * stride];
-    return s;
-  }
-  if (i >= n - B::M) {
-    const int r = i - (n - B::M);
-    const int b0 = n - B::N - r0;
-    T s = T(0);
-#pragma unroll 1
-    for (int k = 0; k < B::N; ++k) s += T(B::end(r, k)) * line[(b0 + k) * stride];
-    return s;
+  if constexpr (CL) {
+    if (i < B::M) {
+      T s = T(0);
+#pragma unroll
+      for (int k = 0; k < B::N; ++k) s += T(B::start(i, k)) * line[(k - r0) * stride];
+      return s;
+    }
+    if (i >= n - B::M) {
+      const int r = i - (n - B::M);
+      const int b0 = n - B::N - r0;
+      T s = T(0);
+#pragma unroll
+      for (int k = 0; k < B::N; ++k) s += T(B::end(r, k)) * line[(b0 + k) * stride];
+      return s;
+    }
   }
   return interior_sum<B>(line + (i - r0) * stride, stride);
 }

@@ -305,7 +305,7 @@ __global__ __launch_bounds__((TX / RX) * (TY / RY)) void stage_kernel(const Stag
             row_window<BB, RX>(lq + (xb - rx0 - HX), o);
           } else {
 #pragma unroll
-            for (int i = 0; i < RX; ++i) o[i] = xin[i] ? row_sum<BB>(lq, 1, xb + i, nx, rx0) : CT(0);
+            for (int i = 0; i < RX; ++i) o[i] = xin[i] ? row_sum<BB, EDGE>(lq, 1, xb + i, nx, rx0) : CT(0);
           }
 #pragma unroll
           for (int i = 0; i < RX; ++i) acc[s][i][q] = first ? sg * o[i] : fma_t(sg, o[i], acc[s][i][q]);
@@ -491,7 +491,7 @@ __global__ __launch_bounds__((TX / RX) * (TY / RY)) void stage_kernel(const Stag
           for (int s = 0; s < RY; ++s)
 #pragma unroll
             for (int i = 0; i < RX; ++i)
-              o[s][i] = (yin[s] && xin[i]) ? row_sum<BB>(col + i, TX, yb + s, ny, ry0) : CT(0);
+              o[s][i] = (yin[s] && xin[i]) ? row_sum<BB, EDGE>(col + i, TX, yb + s, ny, ry0) : CT(0);
         }
 #pragma unroll
         for (int s = 0; s < RY; ++s)

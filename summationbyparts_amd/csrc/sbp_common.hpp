@@ -420,16 +420,16 @@ __device__ __forceinline__ void load_stage_state(const StageArgs& A, const Block
   if (A.mixed && mode >= MODE_S0) {
     const double* yn = yn_buf(B, A);
 #pragma unroll
-    for (int c = 0; c < 4; ++c) q[c] = ldg(yn, c * ps + o);
+    for (int c = 0; c < 4; ++c) q[c] = ldg(yn + c * ps, o);
     if (mode >= MODE_S1) {
       const float* d = delta_in(B, mode);
 #pragma unroll
-      for (int c = 0; c < 4; ++c) q[c] += (double)ldg(d, c * ps + o);
+      for (int c = 0; c < 4; ++c) q[c] += (double)ldg(d + c * ps, o);
     }
   } else {
     const double* in = stage_input(B, mode);
 #pragma unroll
-    for (int c = 0; c < 4; ++c) q[c] = ldg(in, c * ps + o);
+    for (int c = 0; c < 4; ++c) q[c] = ldg(in + c * ps, o);
   }
 }
 

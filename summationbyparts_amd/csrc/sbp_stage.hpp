@@ -171,11 +171,11 @@ __global__ __launch_bounds__((TX / RX) * (TY / RY)) void stage_kernel(const Stag
   // state of the stage input at offset o (FP64; the mixed form adds Δ)
   auto load_q = [&](uint32_t o, double q[4]) {
 #pragma unroll
-    for (int c = 0; c < 4; ++c) q[c] = ldg(in, c * ps + o);
+    for (int c = 0; c < 4; ++c) q[c] = ldg(in + c * ps, o);
     if constexpr (MIX) {
       if (din) {
 #pragma unroll
-        for (int c = 0; c < 4; ++c) q[c] += (double)ldg(din, c * ps + o);
+        for (int c = 0; c < 4; ++c) q[c] += (double)ldg(din + c * ps, o);
       }
     }
   };
@@ -204,7 +204,7 @@ __global__ __launch_bounds__((TX / RX) * (TY / RY)) void stage_kernel(const Stag
     if constexpr (MIX) {
       double qc[4];
       load_q(oc, qc);
-      R = mix_ref(qc, ldg(met, ps + oc), ldg(met, 2 * ps + oc), gamma);
+      R = mix_ref(qc, ldg(met + ps, oc), ldg(met + 2 * ps, oc), gamma);
     }
     // region point (tile row r, column c from rx0) of this thread's
     // iteration it: own output points first (OWNQ), then the halo columns
@@ -240,12 +240,12 @@ __global__ __launch_bounds__((TX / RX) * (TY / RY)) void stage_kernel(const Stag
         const bool ok = i0 + j < ITXE && xi_pt(i0 + j, r, c);
         const uint32_t o = ok ? (uint32_t)((y0 + r) * ld + rx0 + c) : (uint32_t)(y0 * ld + rx0);
         load_q(o, v[j]);
-        v[j][4] = ldg(met, ps + o);
-        v[j][5] = ldg(met, 2 * ps + o);
+        v[j][4] = ldg(met + ps, o);
+        v[j][5] = ldg(met + 2 * ps, o);
         // the own points' η metrics are issued with them (used by the η phase)
         if (OWNQ && i0 + j < RY) {
-          ownm[OWNQ ? i0 + j : 0][0] = ldg(met, 3 * ps + o);
-          ownm[OWNQ ? i0 + j : 0][1] = ldg(met, 4 * ps + o);
+          ownm[OWNQ ? i0 + j : 0][0] = ldg(met + 3 * ps, o);
+          ownm[OWNQ ? i0 + j : 0][1] = ldg(met + 4 * ps, o);
         }
       }
       if constexpr (OWNQ) {
@@ -328,7 +328,7 @@ __global__ __launch_bounds__((TX / RX) * (TY / RY)) void stage_kernel(const Stag
             } else {
               const uint32_t o = (uint32_t)((y0 + r) * ld + rx0 + c);
 #pragma unroll
-              for (int q = 0; q < 4; ++q) lds[q * NP + l] = cache[it][0] * ldg(in, q * ps + o);
+              for (int q = 0; q < 4; ++q) lds[q * NP + l] = cache[it][0] * ldg(in + q * ps, o);
             }
           }
         }
@@ -364,20 +364,20 @@ __global__ __launch_bounds__((TX / RX) * (TY / RY)) void stage_kernel(const Stag
           if (mode == MODE_S0 || mode == MODE_S3) {
             const double* pa = mode == MODE_S0 ? in : pacc;
 #pragma unroll
-            for (int q = 0; q < 4; ++q) fa[s][i][q] = ldg_nt(pa, q * ps + o);
+            for (int q = 0; q < 4; ++q) fa[s][i][q] = ldg_nt(pa + q * ps, o);
           }
           if (mode == MODE_S1 || mode == MODE_S2) {
 #pragma unroll
-            for (int q = 0; q < 4; ++q) fb[s][i][q] = ldg_nt(pacc, q * ps + o);
+            for (int q = 0; q < 4; ++q) fb[s][i][q] = ldg_nt(pacc + q * ps, o);
           }
         } else if (mode != MODE_S0) {
           // S1, S2: fa = y_n; S2 also fb = Y2; S3: fa = W (in the Y2 buffer)
           const double* pa = B.buf[mode == MODE_S3 ? BUF_YA : BUF_YN];
 #pragma unroll
-          for (int q = 0; q < 4; ++q) fa[s][i][q] = ldg_nt(pa, q * ps + o);
+          for (int q = 0; q < 4; ++q) fa[s][i][q] = ldg_nt(pa + q * ps, o);
           if (mode == MODE_S2) {
 #pragma unroll
-            for (int q = 0; q < 4; ++q) fb[s][i][q] = ldg_nt((const double*)B.buf[BUF_YA], q * ps + o);
+            for (int q = 0; q < 4; ++q) fb[s][i][q] = ldg_nt((const double*)B.buf[BUF_YA] + q * ps, o);
           }
         }
       }
@@ -394,7 +394,7 @@ __global__ __launch_bounds__((TX / RX) * (TY / RY)) void stage_kernel(const Stag
     if constexpr (MIX) {
       double qc[4];
       load_q(oc, qc);
-      R = mix_ref(qc, ldg(met, 3 * ps + oc), ldg(met, 4 * ps + oc), gamma);
+      R = mix_ref(qc, ldg(met + 3 * ps, oc), ldg(met + 4 * ps, oc), gamma);
     }
     // region point (row r from ry0, column c from x0) of this thread's
     // iteration it: own output points first (OWNQ), then the halo rows
@@ -442,8 +442,8 @@ __global__ __launch_bounds__((TX / RX) * (TY / RY)) void stage_kernel(const Stag
           v[j][5] = ownm[OWNQ ? i0 + j : 0][1];
         } else {
           load_q(o, v[j]);
-          v[j][4] = ldg(met, 3 * ps + o);
-          v[j][5] = ldg(met, 4 * ps + o);
+          v[j][4] = ldg(met + 3 * ps, o);
+          v[j][5] = ldg(met + 4 * ps, o);
         }
       }
 #pragma unroll
@@ -515,7 +515,7 @@ __global__ __launch_bounds__((TX / RX) * (TY / RY)) void stage_kernel(const Stag
             } else {
               const uint32_t o = (uint32_t)((ry0 + r) * ld + x0 + c);
 #pragma unroll
-              for (int q = 0; q < 4; ++q) lds[q * NP + l] = cache[it][0] * ldg(in, q * ps + o);
+              for (int q = 0; q < 4; ++q) lds[q * NP + l] = cache[it][0] * ldg(in + q * ps, o);
             }
           }
         }
@@ -555,10 +555,10 @@ __global__ __launch_bounds__((TX / RX) * (TY / RY)) void stage_kernel(const Stag
             meta1 = ownm[s][1];
           } else {
             load_stage_state(A, B, mode, o, q4);
-            mxi0 = ldg(met, ps + o);
-            mxi1 = ldg(met, 2 * ps + o);
-            meta0 = ldg(met, 3 * ps + o);
-            meta1 = ldg(met, 4 * ps + o);
+            mxi0 = ldg(met + ps, o);
+            mxi1 = ldg(met + 2 * ps, o);
+            meta0 = ldg(met + 3 * ps, o);
+            meta1 = ldg(met + 4 * ps, o);
           }
           const double dj = ldg(met, o);
           const double time = ((A.time ? *A.time : 0.0) + A.time_offset) * B.dt_scale;
@@ -605,20 +605,20 @@ __global__ __launch_bounds__((TX / RX) * (TY / RY)) void stage_kernel(const Stag
     }
     if (mode == MODE_RHS) {
 #pragma unroll
-      for (int q = 0; q < 4; ++q) stg_nt(B.rhs_out, q * ps + o, k[q]);
+      for (int q = 0; q < 4; ++q) stg_nt(B.rhs_out + q * ps, o, k[q]);
     } else if constexpr (MIX) {
       double* accb = acc_buf(B, A);
       if (mode == MODE_S3) {
         // y_{n+1} over the accumulator (the roles swap after the step)
 #pragma unroll
-        for (int q = 0; q < 4; ++q) stg_nt(accb, q * ps + o, fa[s][i][q] + sacc * k[q]);
+        for (int q = 0; q < 4; ++q) stg_nt(accb + q * ps, o, fa[s][i][q] + sacc * k[q]);
       } else {
         float* dn = delta_out(B, mode);
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
           const double a0 = mode == MODE_S0 ? fa[s][i][q] : fb[s][i][q];
-          stg_nt(accb, q * ps + o, a0 + sacc * k[q]);
-          stg_nt(dn, q * ps + o, (float)(snext * k[q]));
+          stg_nt(accb + q * ps, o, a0 + sacc * k[q]);
+          stg_nt(dn + q * ps, o, (float)(snext * k[q]));
         }
       }
     } else {
@@ -634,19 +634,19 @@ __global__ __launch_bounds__((TX / RX) * (TY / RY)) void stage_kernel(const Stag
       const double* qs = ownq[OWNQ ? s : 0];  // the stage input here
       if (mode == MODE_S0) {
 #pragma unroll
-        for (int q = 0; q < 4; ++q) stg_nt(B.buf[BUF_YA], q * ps + o, qs[q] + snext * k[q]);
+        for (int q = 0; q < 4; ++q) stg_nt(B.buf[BUF_YA] + q * ps, o, qs[q] + snext * k[q]);
       } else if (mode == MODE_S1) {
 #pragma unroll
-        for (int q = 0; q < 4; ++q) stg_nt(B.buf[BUF_YB], q * ps + o, fa[s][i][q] + snext * k[q]);
+        for (int q = 0; q < 4; ++q) stg_nt(B.buf[BUF_YB] + q * ps, o, fa[s][i][q] + snext * k[q]);
       } else if (mode == MODE_S2) {
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
-          stg_nt(B.buf[BUF_ACC], q * ps + o, fa[s][i][q] + snext * k[q]);
-          stg_nt(B.buf[BUF_YA], q * ps + o, fma_t(2.0, qs[q], fb[s][i][q]) - fa[s][i][q]);
+          stg_nt(B.buf[BUF_ACC] + q * ps, o, fa[s][i][q] + snext * k[q]);
+          stg_nt(B.buf[BUF_YA] + q * ps, o, fma_t(2.0, qs[q], fb[s][i][q]) - fa[s][i][q]);
         }
       } else {
 #pragma unroll
-        for (int q = 0; q < 4; ++q) stg_nt(B.buf[BUF_YN], q * ps + o, (fa[s][i][q] + qs[q]) * (1.0 / 3.0) + sacc * k[q]);
+        for (int q = 0; q < 4; ++q) stg_nt(B.buf[BUF_YN] + q * ps, o, (fa[s][i][q] + qs[q]) * (1.0 / 3.0) + sacc * k[q]);
       }
     }
   }

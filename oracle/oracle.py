@@ -18,7 +18,8 @@ OPERATORS = {"upwind4": UPWIND4, "upwind4h2": UPWIND4H2, "upwind9": UPWIND9, "up
              "sbp4": SBP4, "sbp8": SBP8}
 DIFF, DISS = 0, 1
 XI, ETA = 0, 1
-BC_THIS, BC_GRID, BC_VORTEX = 0, 1, 2
+BC_THIS, BC_GRID, BC_VORTEX, BC_INTERPOLATE, BC_MULTIGRID = 0, 1, 2, 3, 4
+INTERP4, INTERP8, INTERP9, INTERP9H2 = range(4)
 NORTH, SOUTH, EAST, WEST = 0, 1, 2, 3
 RHS_AUTO, RHS_TRAD, RHS_UPWIND = 0, 1, 2
 
@@ -75,6 +76,12 @@ def _load():
     lib.orc_system_set_time.restype = None
     lib.orc_system_max_dt.argtypes = [ctypes.c_void_p, dp]
     lib.orc_system_error_vortex.argtypes = [ctypes.c_void_p, ctypes.POINTER(VortexParams), dp]
+    lib.orc_system_set_bc_interpolate.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int]
+    lib.orc_system_set_bc_multigrid.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ip]
+    lib.orc_interpolate.argtypes = [ctypes.c_int, ctypes.c_int, dp, ctypes.c_int, dp, ctypes.c_int]
+    lib.orc_interp_info.argtypes = [ctypes.c_int, ctypes.c_int, ip]
+    lib.orc_interp_coefficients.argtypes = [ctypes.c_int, ctypes.c_int, dp, dp]
+    lib.orc_system_edge.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_double, dp]
     return lib
 
 
@@ -154,6 +161,25 @@ def h2_err(op_eta, op_xi, a, b):
     return out.value
 
 
+def interpolate(iop, fine2coarse, data, nout):
+    """InterpolationOperator::fine2coarse (fine2coarse=True) or coarse2fine of
+    a 1-D array into nout points."""
+    data = np.ascontiguousarray(data, dtype=np.float64)
+    out = np.zeros(nout)
+    _check(lib().orc_interpolate(iop, int(fine2coarse), _ptr(data), data.size, _ptr(out), nout), "interpolate")
+    return out
+
+
+def interp_tables(iop, fine2coarse):
+    """(block, diag, (jump0, jump1)) of one direction of an interpolation operator."""
+    shape = (ctypes.c_int * 6)()
+    _check(lib().orc_interp_info(iop, int(fine2coarse), shape), "interp_info")
+    br, bc, dr, dc, j0, j1 = list(shape)
+    block, diag = np.zeros((br, bc)), np.zeros((dr, dc))
+    _check(lib().orc_interp_coefficients(iop, int(fine2coarse), _ptr(block), _ptr(diag)), "interp_coefficients")
+    return block, diag, (j0, j1)
+
+
 class System:
     """Multi-block system (multigrid SingleThreadedSystem semantics)."""
 
@@ -183,6 +209,22 @@ class System:
     def set_bc(self, block, direction, kind, neighbour=0, vortex_params=None):
         vp = ctypes.byref(vortex_params) if vortex_params is not None else None
         _check(lib().orc_system_set_bc(self.h, block, direction, kind, neighbour, vp), "set_bc")
+
+    def set_bc_interpolate(self, block, direction, neighbour, iop):
+        _check(lib().orc_system_set_bc_interpolate(self.h, block, direction, neighbour, iop), "set_bc_interpolate")
+
+    def set_bc_multigrid(self, block, direction, pieces):
+        """pieces: list of (block, start, end)."""
+        flat = (ctypes.c_int * (3 * len(pieces)))(*[v for p in pieces for v in p])
+        _check(lib().orc_system_set_bc_multigrid(self.h, block, direction, len(pieces), flat), "set_bc_multigrid")
+
+    def edge(self, block, direction, t=0.0):
+        """Boundary data (4, n) of one edge for the current state."""
+        ny, nx = self.shapes[block]
+        n = nx if direction in (NORTH, SOUTH) else ny
+        out = np.zeros((4, n))
+        _check(lib().orc_system_edge(self.h, block, direction, t, _ptr(out)), "edge")
+        return out
 
     def set_rhs_kind(self, kind):
         _check(lib().orc_system_set_rhs_kind(self.h, kind), "set_rhs_kind")

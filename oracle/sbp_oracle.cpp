@@ -21,7 +21,10 @@
 //   dissipation    euler/src/lib.rs:619-674
 //   rhs            euler/src/lib.rs:483-515 (trad), :571-616 (upwind)
 //   sat            euler/src/lib.rs:1002-1252
-//   boundaries     euler/src/lib.rs:763-802, :804-915 (This / Grid / Vortex)
+//   boundaries     euler/src/lib.rs:763-802, :804-915 (This / Grid / Vortex /
+//                  Interpolate / MultiGrid)
+//   interpolate    sbp/src/operators/interpolation.rs:15-50 (2:1 interfaces;
+//                  tables in sbp_interp_tables.h)
 //   integrate      utils/integrate/src/lib.rs:30-35, :176-222 (Rk4)
 //   h2_err         euler/src/lib.rs:377-432
 //   max_dt         multigrid/src/system.rs:542-589
@@ -33,6 +36,7 @@
 #include <thread>
 #include <vector>
 
+#include "sbp_interp_tables.h"
 #include "sbp_tables.h"
 
 namespace {
@@ -41,7 +45,67 @@ enum { OP_UPWIND4 = 0, OP_UPWIND4H2 = 1, OP_UPWIND9 = 2, OP_UPWIND9H2 = 3, OP_SB
 enum { KIND_DIFF = 0, KIND_DISS = 1 };
 enum { ST_OK = 0, ST_INVALID = 1, ST_SHAPE = 2, ST_DETJ = 3, ST_RHO = 4, ST_PRESSURE = 5, ST_GAMMA = 6, ST_EDGE = 7,
        ST_NOT_UPWIND = 9 };
-enum { BC_THIS = 0, BC_GRID = 1, BC_VORTEX = 2 };
+enum { BC_THIS = 0, BC_GRID = 1, BC_VORTEX = 2, BC_INTERPOLATE = 3, BC_MULTIGRID = 4 };
+enum { IOP_4 = 0, IOP_8 = 1, IOP_9 = 2, IOP_9H2 = 3, IOP_COUNT = 4 };
+
+// One direction (fine to coarse or coarse to fine) of an interpolation
+// operator: start block, repeating diagonal rows and the jump that places
+// them (interpolation4.rs:74-93 and the other three alike).
+struct InterpDef {
+  const double* block;
+  const double* diag;
+  int brows, bcols, drows, dcols, j0, j1;
+  bool h2;
+};
+
+template <class I>
+InterpDef interp_def_of(bool f2c) {
+  if (f2c)
+    return {&I::f2c_block[0][0], &I::f2c_diag[0][0], I::f2c_block_rows, I::f2c_block_cols, I::f2c_diag_rows,
+            I::f2c_diag_cols, I::f2c_jump[0], I::f2c_jump[1], I::is_h2};
+  return {&I::c2f_block[0][0], &I::c2f_diag[0][0], I::c2f_block_rows, I::c2f_block_cols, I::c2f_diag_rows,
+          I::c2f_diag_cols, I::c2f_jump[0], I::c2f_jump[1], I::is_h2};
+}
+
+bool interp_def(int op, bool f2c, InterpDef& d) {
+  using namespace interp_tables;
+  switch (op) {
+    case IOP_4: d = interp_def_of<Interpolation4>(f2c); return true;
+    case IOP_8: d = interp_def_of<Interpolation8>(f2c); return true;
+    case IOP_9: d = interp_def_of<Interpolation9>(f2c); return true;
+    case IOP_9H2: d = interp_def_of<Interpolation9h2>(f2c); return true;
+    default: return false;
+  }
+}
+
+// fine2coarse / coarse2fine length asserts (interpolation4.rs:75,85;
+// interpolation9h2.rs:53,64): fine = 2 coarse - 1, or 2 (coarse - 1) on h2 grids
+bool interp_lengths(const InterpDef& d, bool f2c, int nin, int nout) {
+  const int coarse = f2c ? nout : nin, fine = f2c ? nin : nout;
+  return d.h2 ? fine == 2 * (coarse - 1) : fine == 2 * coarse - 1;
+}
+
+// interpolation.rs:15-50: output rows below the block height take the block,
+// rows within the block height of the end take the block mirrored (row from
+// the end, columns reversed, aligned with the input's end), the rest the
+// diagonal row (n mod rows) at input index jump.0 + jump.1·(n div rows);
+// each output is a sum from 0.0 over the taps in order, unfused.
+void interpolate(const InterpDef& d, const double* in, size_t is, int nin, double* out, size_t os, int nout) {
+  for (int i = 0; i < nout; ++i) {
+    double acc = 0.0;
+    if (i < d.brows) {
+      for (int k = 0; k < d.bcols; ++k) acc += in[(size_t)k * is] * d.block[i * d.bcols + k];
+    } else if (nout - i <= d.brows) {
+      const int row = nout - i - 1, index = nin - d.bcols;
+      for (int k = 0; k < d.bcols; ++k) acc += in[(size_t)(index + k) * is] * d.block[row * d.bcols + (d.bcols - k - 1)];
+    } else {
+      const int n = i - d.brows;
+      const int index = d.j0 + d.j1 * (n / d.drows), row = n % d.drows;
+      for (int k = 0; k < d.dcols; ++k) acc += in[(size_t)(index + k) * is] * d.diag[row * d.dcols + k];
+    }
+    out[(size_t)i * os] = acc;
+  }
+}
 enum { DIR_NORTH = 0, DIR_SOUTH = 1, DIR_EAST = 2, DIR_WEST = 3 };
 enum { RHS_AUTO = 0, RHS_TRAD = 1, RHS_UPWIND = 2 };
 
@@ -285,6 +349,8 @@ struct Block {
   std::vector<double> detj, jxx, jxy, jyx, jyy;  // J, J·ξx, J·ξy, J·ηx, J·ηy
   int bc_kind[4] = {BC_THIS, BC_THIS, BC_THIS, BC_THIS};
   int bc_nb[4] = {0, 0, 0, 0};
+  int bc_iop[4] = {0, 0, 0, 0};           // Interpolate: operator
+  std::vector<int> bc_pieces[4];          // MultiGrid: (block, start, end) triples
   orc_vortex_params bc_vortex[4];
   size_t npts() const { return (size_t)ny * nx; }
 };
@@ -438,6 +504,55 @@ int extract_edge(System& S, size_t bi, const std::vector<std::vector<double>>& f
     double* eb = w.edge[dir].data();
     int st = vortex_eval(b.bc_vortex[dir], time, gx, gy, n, gs, gs, eb, eb + n, eb + 2 * n, eb + 3 * n, 1);
     if (st) return st;
+    out = {eb, (size_t)n, 1};
+  } else if (kind == BC_INTERPOLATE) {
+    // lib.rs:816-828: the source's opposite edge interpolated onto this edge;
+    // fine to coarse when this block has fewer points along ξ than the
+    // source (the reference compares nx for every edge direction)
+    size_t src = (size_t)b.bc_nb[dir];
+    if (src >= S.blocks.size()) return ST_INVALID;
+    const Block& s = S.blocks[src];
+    const bool ns = dir == DIR_NORTH || dir == DIR_SOUTH;
+    const int nout = ns ? b.nx : b.ny, nin = ns ? s.nx : s.ny;
+    const bool f2c = b.nx < s.nx;
+    InterpDef d;
+    if (!interp_def(b.bc_iop[dir], f2c, d)) return ST_INVALID;
+    if (!interp_lengths(d, f2c, nin, nout)) return ST_EDGE;
+    const EdgeView from = sel(src);
+    double* eb = w.edge[dir].data();
+    for (int c = 0; c < 4; ++c) interpolate(d, from.p + c * from.cs, from.ps, nin, eb + (size_t)c * nout, 1, nout);
+    out = {eb, (size_t)nout, 1};
+  } else if (kind == BC_MULTIGRID) {
+    // lib.rs:829-849: the pieces (block g, start, end) of the sources'
+    // opposite edges are copied one after another, each starting on the last
+    // point of the one before (which it overwrites); after a piece is
+    // copied, its first point is halved unless it is the last piece
+    const int n = (dir == DIR_NORTH || dir == DIR_SOUTH) ? b.nx : b.ny;
+    const std::vector<int>& pc = b.bc_pieces[dir];
+    if (pc.empty() || pc.size() % 3) return ST_EDGE;
+    double* eb = w.edge[dir].data();
+    int i = 0;
+    size_t remaining = pc.size() / 3;
+    for (size_t p = 0; p < pc.size(); p += 3) {
+      const size_t g = (size_t)pc[p];
+      const int start = pc[p + 1], end = pc[p + 2];
+      if (g >= S.blocks.size()) return ST_INVALID;
+      const Block& s = S.blocks[g];
+      const int have = (dir == DIR_NORTH || dir == DIR_SOUTH) ? s.nx : s.ny;
+      const int len = end - start;
+      if (start < 0 || end > have || len < 1 || i + len > n) return ST_EDGE;
+      const EdgeView from = sel(g);
+      for (int c = 0; c < 4; ++c)
+        for (int k = 0; k < len; ++k) eb[(size_t)c * n + i + k] = from.p[c * from.cs + (size_t)(start + k) * from.ps];
+      remaining -= 1;
+      if (remaining == 0) {
+        i += len;
+        if (i != n) return ST_EDGE;
+      } else {
+        for (int c = 0; c < 4; ++c) eb[(size_t)c * n + i] /= 2.0;
+        i += len - 1;
+      }
+    }
     out = {eb, (size_t)n, 1};
   } else {
     return ST_INVALID;
@@ -769,6 +884,70 @@ int orc_system_set_bc(void* s, int block, int dir, int kind, int neighbour, cons
   b.bc_kind[dir] = kind;
   b.bc_nb[dir] = neighbour;
   if (vp) b.bc_vortex[dir] = *vp;
+  return ST_OK;
+}
+
+// BoundaryCharacteristic::Interpolate(neighbour, operator) (lib.rs:743)
+int orc_system_set_bc_interpolate(void* s, int block, int dir, int neighbour, int iop) {
+  System* S = (System*)s;
+  if (block < 0 || block >= (int)S->blocks.size() || dir < 0 || dir > 3 || iop < 0 || iop >= IOP_COUNT) return ST_INVALID;
+  Block& b = S->blocks[block];
+  b.bc_kind[dir] = BC_INTERPOLATE;
+  b.bc_nb[dir] = neighbour;
+  b.bc_iop[dir] = iop;
+  return ST_OK;
+}
+
+// BoundaryCharacteristic::MultiGrid(pieces) (lib.rs:744): npieces triples
+// (block, start, end)
+int orc_system_set_bc_multigrid(void* s, int block, int dir, int npieces, const int* triples) {
+  System* S = (System*)s;
+  if (block < 0 || block >= (int)S->blocks.size() || dir < 0 || dir > 3 || npieces < 1 || !triples) return ST_INVALID;
+  Block& b = S->blocks[block];
+  b.bc_kind[dir] = BC_MULTIGRID;
+  b.bc_pieces[dir].assign(triples, triples + 3 * (size_t)npieces);
+  return ST_OK;
+}
+
+// Boundary data g of one edge of one block for the current state at time t,
+// as boundary_extract assembles it (lib.rs:804-867): 4 x (edge length).
+int orc_system_edge(void* s, int block, int dir, double t, double* out) {
+  System* S = (System*)s;
+  if (block < 0 || block >= (int)S->blocks.size() || dir < 0 || dir > 3) return ST_INVALID;
+  const Block& b = S->blocks[block];
+  S->work[block].ensure(b.npts(), b.nx, b.ny);
+  EdgeView v;
+  const int st = extract_edge(*S, (size_t)block, S->fnow, dir, t, v);
+  if (st) return st;
+  const int n = (dir == DIR_NORTH || dir == DIR_SOUTH) ? b.nx : b.ny;
+  for (int c = 0; c < 4; ++c)
+    for (int i = 0; i < n; ++i) out[(size_t)c * n + i] = v.p[c * v.cs + (size_t)i * v.ps];
+  return ST_OK;
+}
+
+// InterpolationOperator::fine2coarse / coarse2fine (sbp/src/operators.rs:108-113)
+int orc_interpolate(int iop, int f2c, const double* in, int nin, double* out, int nout) {
+  InterpDef d;
+  if (!interp_def(iop, f2c != 0, d)) return ST_INVALID;
+  if (!interp_lengths(d, f2c != 0, nin, nout)) return ST_SHAPE;
+  interpolate(d, in, 1, nin, out, 1, nout);
+  return ST_OK;
+}
+
+// Table shapes and coefficients of one direction of an operator.
+int orc_interp_info(int iop, int f2c, int* shape6) {
+  InterpDef d;
+  if (!interp_def(iop, f2c != 0, d)) return ST_INVALID;
+  const int v[6] = {d.brows, d.bcols, d.drows, d.dcols, d.j0, d.j1};
+  std::memcpy(shape6, v, sizeof v);
+  return ST_OK;
+}
+
+int orc_interp_coefficients(int iop, int f2c, double* block, double* diag) {
+  InterpDef d;
+  if (!interp_def(iop, f2c != 0, d)) return ST_INVALID;
+  std::memcpy(block, d.block, sizeof(double) * d.brows * d.bcols);
+  std::memcpy(diag, d.diag, sizeof(double) * d.drows * d.dcols);
   return ST_OK;
 }
 

@@ -50,8 +50,19 @@ typedef enum { SBP_DIFF = 0, SBP_DISS = 1 } sbp_kind;
 /* sbp/src/utils.rs:15-129 (Direction) */
 typedef enum { SBP_NORTH = 0, SBP_SOUTH = 1, SBP_EAST = 2, SBP_WEST = 3 } sbp_direction;
 
-/* euler/src/lib.rs:737-745 (BoundaryCharacteristic: This, Grid(j), Vortex) */
-typedef enum { SBP_BC_THIS = 0, SBP_BC_GRID = 1, SBP_BC_VORTEX = 2 } sbp_bc_kind;
+/* euler/src/lib.rs:737-745 (BoundaryCharacteristic: This, Grid(j), Vortex,
+ * Interpolate(j, operator), MultiGrid(pieces); the last two are set with
+ * sbp_system_set_bc_interpolate / sbp_system_set_bc_multigrid) */
+typedef enum {
+  SBP_BC_THIS = 0,
+  SBP_BC_GRID = 1,
+  SBP_BC_VORTEX = 2,
+  SBP_BC_INTERPOLATE = 3,
+  SBP_BC_MULTIGRID = 4
+} sbp_bc_kind;
+
+/* 2:1 interface interpolation operators (sbp/src/operators/interpolation.rs:1-13) */
+typedef enum { SBP_INTERP4 = 0, SBP_INTERP8 = 1, SBP_INTERP9 = 2, SBP_INTERP9H2 = 3 } sbp_interp_op;
 
 /* RHS selection: AUTO follows multigrid/src/system.rs:521-525 (upwind iff both
  * operators are upwind); TRAD = euler::System::advance / RHS_trad;
@@ -117,6 +128,17 @@ int sbp_dissxi(sbp_context ctx, int op, const double* prev, int64_t ld_prev, dou
 /* UpwindOperator2d::disseta (sbp/src/operators.rs:98-100) */
 int sbp_disseta(sbp_context ctx, int op, const double* prev, int64_t ld_prev, double* fut, int64_t ld_fut, int ny, int nx);
 
+/* InterpolationOperator::fine2coarse (fine2coarse = 1) / coarse2fine
+ * (sbp/src/operators.rs:108-113; interpolation.rs:15-50) of a device vector
+ * of nin points into nout points: fine = 2 coarse - 1 (2 (coarse - 1) for
+ * SBP_INTERP9H2), else SBP_ERR_SHAPE. out is fully overwritten. */
+int sbp_interpolate(sbp_context ctx, int interp_op, int fine2coarse, const double* in, int nin, double* out, int nout);
+/* Table shapes {block rows, block cols, diag rows, diag cols, jump0, jump1}
+ * and coefficients (block row-major, then diag row-major) of one direction of
+ * an interpolation operator (host only, no device work). */
+int sbp_interp_info(int interp_op, int fine2coarse, int* shape6);
+int sbp_interp_coefficients(int interp_op, int fine2coarse, double* block, double* diag);
+
 /* ---- Block-level right-hand side on device buffers ----------------------
  * euler::RHS_trad / RHS_upwind (euler/src/lib.rs:483-515, :571-616) for one
  * block with operator pair (op_eta, op_xi) (the reference's tuple order,
@@ -146,6 +168,18 @@ int sbp_system_destroy(sbp_system sys);
 /* BoundaryCharacteristic of one edge: SBP_BC_THIS, SBP_BC_GRID (neighbour),
  * SBP_BC_VORTEX (vortex required). */
 int sbp_system_set_bc(sbp_system sys, int block, int dir, int kind, int neighbour, const sbp_vortex_params* vortex);
+/* BoundaryCharacteristic::Interpolate(neighbour, operator) (euler/src/lib.rs:743,
+ * :816-828): the neighbour's opposite edge through a 2:1 interpolation
+ * operator, fine to coarse when this block has fewer points along ξ than the
+ * neighbour (the reference compares nx for every edge). SBP_ERR_EDGE unless
+ * the edge lengths are the operator's coarse/fine pair. */
+int sbp_system_set_bc_interpolate(sbp_system sys, int block, int dir, int neighbour, int interp_op);
+/* BoundaryCharacteristic::MultiGrid(pieces) (euler/src/lib.rs:744, :829-849):
+ * npieces triples (block, start, end) of the sources' opposite edges, copied
+ * one after another, each starting on the last point of the one before; the
+ * first point of every piece but the last is halved. SBP_ERR_EDGE unless the
+ * pieces lie inside their sources and cover this edge exactly. */
+int sbp_system_set_bc_multigrid(sbp_system sys, int block, int dir, int npieces, const int* pieces);
 int sbp_system_set_rhs_kind(sbp_system sys, int kind);
 int sbp_system_set_precision(sbp_system sys, int precision);
 /* State of one block, (4, ny, nx) host arrays. */

@@ -16,7 +16,8 @@ OK, ERR_INVALID, ERR_SHAPE, ERR_DETJ, ERR_RHO, ERR_PRESSURE, ERR_GAMMA, ERR_EDGE
 UPWIND4, UPWIND4H2, UPWIND9, UPWIND9H2, SBP4, SBP8 = range(6)
 DIFF, DISS = 0, 1
 NORTH, SOUTH, EAST, WEST = 0, 1, 2, 3
-BC_THIS, BC_GRID, BC_VORTEX = 0, 1, 2
+BC_THIS, BC_GRID, BC_VORTEX, BC_INTERPOLATE, BC_MULTIGRID = 0, 1, 2, 3, 4
+INTERP4, INTERP8, INTERP9, INTERP9H2 = 0, 1, 2, 3
 RHS_AUTO, RHS_TRAD, RHS_UPWIND = 0, 1, 2
 FP64, MIXED = 0, 1
 
@@ -102,6 +103,11 @@ def _declare(L):
         "sbp_system_tile_shape": ([c_void_p, ip, ip, ip], c_int),
         "sbp_system_set_tile_shape": ([c_void_p, c_int], c_int),
         "sbp_system_tile_index": ([c_void_p, ip], c_int),
+        "sbp_system_set_bc_interpolate": ([c_void_p, c_int, c_int, c_int, c_int], c_int),
+        "sbp_system_set_bc_multigrid": ([c_void_p, c_int, c_int, c_int, ip], c_int),
+        "sbp_interpolate": ([c_void_p, c_int, c_int, c_void_p, c_int, c_void_p, c_int], c_int),
+        "sbp_interp_info": ([c_int, c_int, ip], c_int),
+        "sbp_interp_coefficients": ([c_int, c_int, dp, dp], c_int),
         "sbp_system_advance_timed": ([c_void_p, c_double, c_int64, dp], c_int),
         "sbp_tile_shape_count": ([], c_int),
         "sbp_tile_shape_get": ([c_int, ip, ip, ip], c_int),
@@ -122,6 +128,8 @@ EXPORTED = [
     "sbp_system_error_vortex_block", "sbp_system_rhs", "sbp_system_advance", "sbp_system_get_time",
     "sbp_system_set_time", "sbp_system_max_dt", "sbp_system_error_vortex", "sbp_system_energy", "sbp_system_check",
     "sbp_system_profile", "sbp_system_tile_shape", "sbp_system_set_tile_shape", "sbp_system_tile_index", "sbp_system_advance_timed",
+    "sbp_system_set_bc_interpolate", "sbp_system_set_bc_multigrid", "sbp_interpolate", "sbp_interp_info",
+    "sbp_interp_coefficients",
     "sbp_tile_shape_count",
     "sbp_tile_shape_get",
 ]

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include "sbp_device.hpp"
+#include "sbp_interp.hpp"
 #include "sbp_operators.hpp"
 
 namespace sbp {
@@ -434,23 +435,57 @@ __device__ __forceinline__ void load_stage_state(const StageArgs& A, const Block
 }
 
 // Boundary data g of edge `dir` at edge index t (boundary_extracts, lib.rs:869-915).
+// Offset of point m of the edge of block S opposite to `dir` (the source
+// edge a neighbour reads: boundary_extracts' seldir, lib.rs:869-915).
+__device__ __forceinline__ uint32_t source_edge_offset(const BlockDev& S, int dir, int m) {
+  switch (dir) {
+    case SBP_NORTH: return (uint32_t)m;                                   // source south row
+    case SBP_SOUTH: return (uint32_t)((S.ny - 1) * S.ld + m);            // source north row
+    case SBP_EAST: return (uint32_t)(m * S.ld);                           // source west column
+    default: return (uint32_t)(m * S.ld + (S.nx - 1));                    // source east column
+  }
+}
+
 __device__ __forceinline__ void edge_data(const StageArgs& A, const BlockDev& B, int dir, int t, int x, int y, int mode,
                                           double time, double g[4], unsigned& err) {
   const int src = B.edge_src[dir];
   if (src == EDGE_THIS || src == EDGE_GRID) {
     const BlockDev& S = src == EDGE_THIS ? B : A.blocks[B.edge_nb[dir]];
-    int64_t o;
-    switch (dir) {
-      case SBP_NORTH: o = t; break;                                  // source south row
-      case SBP_SOUTH: o = (int64_t)(S.ny - 1) * S.ld + t; break;     // source north row
-      case SBP_EAST: o = (int64_t)t * S.ld; break;                   // source west column
-      default: o = (int64_t)t * S.ld + (S.nx - 1); break;             // source east column
-    }
-    load_stage_state(A, S, mode, (uint32_t)o, g);
+    load_stage_state(A, S, mode, source_edge_offset(S, dir, t), g);
   } else if (src == EDGE_VORTEX) {
     const int64_t o = (int64_t)y * B.ld + x;
     g[2] = 0.0;
     vortex_point(B.vortex[dir], A.gamma, time, B.xc[o], B.yc[o], g, err);
+  } else if (src == EDGE_INTERP) {
+    // Interpolate (lib.rs:816-828): the neighbour's opposite edge through a
+    // 2:1 operator (the host checked the lengths)
+    const BlockDev& S = A.blocks[B.edge_nb[dir]];
+    const bool ns = dir == SBP_NORTH || dir == SBP_SOUTH;
+    const interp::Table T = interp::table(B.edge_iop[dir], B.edge_f2c[dir] != 0);
+    interp::value<4>(T, t, ns ? B.nx : B.ny, ns ? S.nx : S.ny,
+                     [&](int m, double v[4]) { load_stage_state(A, S, mode, source_edge_offset(S, dir, m), v); }, g);
+  } else if (src == EDGE_MULTIGRID) {
+    // MultiGrid (lib.rs:829-849): piece k covers [i_k, i_k + n_k) with
+    // i_{k+1} = i_k + n_k - 1; a point takes the last piece covering it,
+    // halved when it is that piece's first point and the piece is not the
+    // last (the host checked that the pieces cover the edge)
+    const int* pc = B.edge_pieces[dir];
+    const int np = B.edge_npieces[dir];
+    int i = 0, sel = 0, sel_i = 0;
+    for (int k = 0; k < np; ++k) {
+      const int len = pc[3 * k + 2] - pc[3 * k + 1];
+      if (t >= i && t < i + len) {
+        sel = k;
+        sel_i = i;
+      }
+      i += len - 1;
+    }
+    const BlockDev& S = A.blocks[pc[3 * sel]];
+    load_stage_state(A, S, mode, source_edge_offset(S, dir, pc[3 * sel + 1] + (t - sel_i)), g);
+    if (t == sel_i && sel != np - 1) {
+#pragma unroll
+      for (int c = 0; c < 4; ++c) g[c] /= 2.0;
+    }
   } else {
     const double* p = B.edge_ptr[dir];
 #pragma unroll

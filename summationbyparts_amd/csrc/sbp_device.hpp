@@ -21,6 +21,8 @@ enum EdgeSrc : int {
   EDGE_THIS = SBP_BC_THIS,      // own opposite edge of the stage input (periodic)
   EDGE_GRID = SBP_BC_GRID,      // neighbour block's opposite edge of the stage input
   EDGE_VORTEX = SBP_BC_VORTEX,  // analytic vortex on this edge's coordinates at stage time
+  EDGE_INTERP = SBP_BC_INTERPOLATE,  // neighbour's opposite edge through a 2:1 interpolation operator
+  EDGE_MULTIGRID = SBP_BC_MULTIGRID,  // pieces of several neighbours' opposite edges
   EDGE_EXPLICIT = 100,          // caller-provided 4 x n array (operator-level rhs)
 };
 
@@ -55,6 +57,10 @@ struct BlockDev {
   const double* edge_ptr[4];  // EDGE_EXPLICIT data
   int64_t edge_cs[4];         // component stride of EDGE_EXPLICIT data
   VortexDev vortex[4];        // EDGE_VORTEX parameters
+  int edge_iop[4];            // EDGE_INTERP: interpolation operator (InterpId)
+  int edge_f2c[4];            // EDGE_INTERP: 1 = fine to coarse
+  const int* edge_pieces[4];  // EDGE_MULTIGRID: (block, start, end) triples
+  int edge_npieces[4];
 };
 
 struct TileDev {

@@ -47,6 +47,9 @@ hipError_t launch_maxspeed(const double* q, const double* metrics, int64_t ps, i
                            hipStream_t stream);
 // time += dt on the device.
 hipError_t launch_advance_time(double* time, double dt, hipStream_t stream);
+// 2:1 interpolation of a vector (InterpolationOperator; one thread per output).
+hipError_t launch_interpolate(int interp_op, bool fine2coarse, const double* in, int nin, double* out, int nout,
+                              hipStream_t stream);
 // Copy between pitched planes (4 or 5 planes) on the device.
 hipError_t launch_copy_planes(const double* src, int64_t sps, int64_t sld, double* dst, int64_t dps, int64_t dld, int nplanes,
                               int ny, int nx, hipStream_t stream);

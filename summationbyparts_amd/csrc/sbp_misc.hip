@@ -306,4 +306,24 @@ hipError_t launch_copy_planes(const double* src, int64_t sps, int64_t sld, doubl
   return hipGetLastError();
 }
 
+// ---------------------------------------------------------------------------
+// InterpolationOperator::fine2coarse / coarse2fine (interpolation.rs:15-50)
+// of one vector, one thread per output point. The table is looked up on the
+// device: its pointers must be the device copies of the coefficient arrays.
+__global__ __launch_bounds__(256) void interpolate_kernel(int interp_op, int fine2coarse, const double* __restrict__ in,
+                                                          int nin, double* __restrict__ out, int nout) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= nout) return;
+  const interp::Table T = interp::table(interp_op, fine2coarse != 0);
+  double v[1];
+  interp::value<1>(T, i, nout, nin, [&](int m, double w[1]) { w[0] = in[m]; }, v);
+  out[i] = v[0];
+}
+
+hipError_t launch_interpolate(int interp_op, bool fine2coarse, const double* in, int nin, double* out, int nout,
+                              hipStream_t stream) {
+  interpolate_kernel<<<(nout + 255) / 256, 256, 0, stream>>>(interp_op, fine2coarse ? 1 : 0, in, nin, out, nout);
+  return hipGetLastError();
+}
+
 }  // namespace sbp

@@ -15,6 +15,7 @@
 
 #include "../../include/sbpgpu.h"
 #include "sbp_device.hpp"
+#include "sbp_interp.hpp"
 #include "sbp_kernels.hpp"
 #include "sbp_operators.hpp"
 
@@ -178,6 +179,8 @@ struct HostBlock {
   double dt_scale = 1.0;  // the block steps with dt·dt_scale (independent solves in one batch)
   int bc_kind[4] = {SBP_BC_THIS, SBP_BC_THIS, SBP_BC_THIS, SBP_BC_THIS};
   int bc_nb[4] = {0, 0, 0, 0};
+  int bc_iop[4] = {0, 0, 0, 0};   // SBP_BC_INTERPOLATE operator
+  std::vector<int> bc_pieces[4];  // SBP_BC_MULTIGRID (block, start, end) triples
   sbp_vortex_params bc_vortex[4];
 };
 
@@ -198,6 +201,7 @@ struct sbp_system_s {
   std::vector<Group> groups;
   BlockDev* d_blocks = nullptr;
   TileDev* d_tiles = nullptr;
+  int* d_pieces = nullptr;  // MultiGrid piece lists of all edges
   double* d_time = nullptr;      // system time
   double* d_time_rhs = nullptr;  // explicit time for sbp_system_rhs
   unsigned* d_err = nullptr;
@@ -277,6 +281,11 @@ BlockDev make_dev(const sbp_system_s* s, int i, int shape) {
     b.edge_src[d] = h.bc_kind[d];
     b.edge_nb[d] = h.bc_nb[d];
     b.vortex[d] = to_dev(h.bc_vortex[d]);
+    b.edge_iop[d] = h.bc_iop[d];
+    // fine to coarse when this block has fewer points along ξ (lib.rs:818)
+    b.edge_f2c[d] = h.bc_kind[d] == SBP_BC_INTERPOLATE && h.nx < s->blocks[h.bc_nb[d]].nx;
+    b.edge_pieces[d] = nullptr;  // set by the upload, which owns the device copy
+    b.edge_npieces[d] = (int)h.bc_pieces[d].size() / 3;
   }
   return b;
 }
@@ -424,8 +433,25 @@ int rebuild(sbp_system_s* s) {
     }
     if (s->d_blocks) hipFree(s->d_blocks);
     if (s->d_tiles) hipFree(s->d_tiles);
+    if (s->d_pieces) hipFree(s->d_pieces);
     s->d_blocks = nullptr;
     s->d_tiles = nullptr;
+    s->d_pieces = nullptr;
+    // MultiGrid piece lists: one device array, each edge pointing into it
+    std::vector<int> pieces;
+    std::vector<size_t> poff(4 * s->blocks.size(), 0);
+    for (size_t bi = 0; bi < s->blocks.size(); ++bi)
+      for (int d = 0; d < 4; ++d) {
+        poff[4 * bi + d] = pieces.size();
+        pieces.insert(pieces.end(), s->blocks[bi].bc_pieces[d].begin(), s->blocks[bi].bc_pieces[d].end());
+      }
+    if (!pieces.empty()) {
+      SBP_TRY_HIP(hipMalloc(&s->d_pieces, sizeof(int) * pieces.size()));
+      SBP_TRY_HIP(hipMemcpyAsync(s->d_pieces, pieces.data(), sizeof(int) * pieces.size(), hipMemcpyHostToDevice, st));
+      for (size_t bi = 0; bi < s->blocks.size(); ++bi)
+        for (int d = 0; d < 4; ++d)
+          if (!s->blocks[bi].bc_pieces[d].empty()) devs[bi].edge_pieces[d] = s->d_pieces + poff[4 * bi + d];
+    }
     SBP_TRY_HIP(hipMalloc(&s->d_blocks, sizeof(BlockDev) * devs.size()));
     SBP_TRY_HIP(hipMalloc(&s->d_tiles, sizeof(TileDev) * std::max<size_t>(1, tiles.size())));
     SBP_TRY_HIP(hipMemcpyAsync(s->d_blocks, devs.data(), sizeof(BlockDev) * devs.size(), hipMemcpyHostToDevice, st));
@@ -773,6 +799,31 @@ int sbp_disseta(sbp_context ctx, int op, const double* prev, int64_t ld_prev, do
   return apply(ctx, op, SBP_DISS, 1, prev, ld_prev, fut, ld_fut, ny, nx);
 }
 
+int sbp_interpolate(sbp_context ctx, int interp_op, int fine2coarse, const double* in, int nin, double* out, int nout) {
+  if (!ctx || !in || !out || interp_op < 0 || interp_op >= INTERP_COUNT) return SBP_ERR_INVALID;
+  const bool f2c = fine2coarse != 0;
+  if (!interp::lengths_match(interp_op, f2c ? nout : nin, f2c ? nin : nout)) return SBP_ERR_SHAPE;
+  SBP_TRY_HIP(hipSetDevice(ctx->device));
+  SBP_TRY_HIP(launch_interpolate(interp_op, f2c, in, nin, out, nout, ctx->stream));
+  return SBP_OK;
+}
+
+int sbp_interp_info(int interp_op, int fine2coarse, int* shape6) {
+  if (!shape6 || interp_op < 0 || interp_op >= INTERP_COUNT) return SBP_ERR_INVALID;
+  const interp::Table T = interp::table(interp_op, fine2coarse != 0);
+  const int v[6] = {T.brows, T.bcols, T.drows, T.dcols, T.jump0, T.jump1};
+  for (int i = 0; i < 6; ++i) shape6[i] = v[i];
+  return SBP_OK;
+}
+
+int sbp_interp_coefficients(int interp_op, int fine2coarse, double* block, double* diag) {
+  if (!block || !diag || interp_op < 0 || interp_op >= INTERP_COUNT) return SBP_ERR_INVALID;
+  const interp::Table T = interp::table(interp_op, fine2coarse != 0);
+  for (int i = 0; i < T.brows * T.bcols; ++i) block[i] = T.block[i];
+  for (int i = 0; i < T.drows * T.dcols; ++i) diag[i] = T.diag[i];
+  return SBP_OK;
+}
+
 static int check_shape(int op_eta, int op_xi, int ny, int nx) {
   OpInfo oe, ox;
   if (!op_info(op_eta, oe) || !op_info(op_xi, ox)) return SBP_ERR_INVALID;
@@ -957,6 +1008,7 @@ int sbp_system_destroy(sbp_system s) {
   }
   if (s->d_blocks) hipFree(s->d_blocks);
   if (s->d_tiles) hipFree(s->d_tiles);
+  if (s->d_pieces) hipFree(s->d_pieces);
   if (s->d_time) hipFree(s->d_time);
   if (s->d_err) hipFree(s->d_err);
   if (s->d_partials) hipFree(s->d_partials);
@@ -980,6 +1032,49 @@ int sbp_system_set_bc(sbp_system s, int block, int dir, int kind, int neighbour,
   }
   h.bc_kind[dir] = kind;
   h.bc_nb[dir] = neighbour;
+  h.bc_pieces[dir].clear();
+  s->dirty = true;
+  return SBP_OK;
+}
+
+int sbp_system_set_bc_interpolate(sbp_system s, int block, int dir, int neighbour, int interp_op) {
+  if (!s || block < 0 || block >= (int)s->blocks.size() || dir < 0 || dir > 3 || neighbour < 0 ||
+      neighbour >= (int)s->blocks.size() || interp_op < 0 || interp_op >= INTERP_COUNT)
+    return SBP_ERR_INVALID;
+  HostBlock& h = s->blocks[block];
+  const HostBlock& o = s->blocks[neighbour];
+  const bool ns = dir == SBP_NORTH || dir == SBP_SOUTH;
+  const int own = ns ? h.nx : h.ny, src = ns ? o.nx : o.ny;
+  // fine to coarse iff this block has fewer points along ξ (lib.rs:818); the
+  // operator's length assert then decides (interpolation4.rs:75,85)
+  const bool f2c = h.nx < o.nx;
+  if (!interp::lengths_match(interp_op, f2c ? own : src, f2c ? src : own)) return SBP_ERR_EDGE;
+  h.bc_kind[dir] = SBP_BC_INTERPOLATE;
+  h.bc_nb[dir] = neighbour;
+  h.bc_iop[dir] = interp_op;
+  h.bc_pieces[dir].clear();
+  s->dirty = true;
+  return SBP_OK;
+}
+
+int sbp_system_set_bc_multigrid(sbp_system s, int block, int dir, int npieces, const int* pieces) {
+  if (!s || block < 0 || block >= (int)s->blocks.size() || dir < 0 || dir > 3 || npieces < 1 || !pieces)
+    return SBP_ERR_INVALID;
+  HostBlock& h = s->blocks[block];
+  const bool ns = dir == SBP_NORTH || dir == SBP_SOUTH;
+  // lib.rs:829-849: piece k starts on the last point of piece k-1; the
+  // pieces must end exactly on the last point of this edge
+  int covered = 0;
+  for (int k = 0; k < npieces; ++k) {
+    const int g = pieces[3 * k], start = pieces[3 * k + 1], end = pieces[3 * k + 2];
+    if (g < 0 || g >= (int)s->blocks.size()) return SBP_ERR_INVALID;
+    const HostBlock& o = s->blocks[g];
+    if (start < 0 || end > (ns ? o.nx : o.ny) || end - start < 1) return SBP_ERR_EDGE;
+    covered += (end - start) - (k + 1 < npieces ? 1 : 0);
+  }
+  if (covered != (ns ? h.nx : h.ny)) return SBP_ERR_EDGE;
+  h.bc_kind[dir] = SBP_BC_MULTIGRID;
+  h.bc_pieces[dir].assign(pieces, pieces + 3 * (size_t)npieces);
   s->dirty = true;
   return SBP_OK;
 }

@@ -99,6 +99,17 @@ class BlockSystem:
         vp = ctypes.byref(vortex.c()) if vortex is not None else None
         _lib.check(_lib.lib().sbp_system_set_bc(self._h, block, direction, kind, neighbour, vp), "sbp_system_set_bc")
 
+    def set_bc_interpolate(self, block, direction, neighbour, interp_op):
+        """BoundaryCharacteristic::Interpolate(neighbour, operator) (euler/src/lib.rs:743)."""
+        _lib.check(_lib.lib().sbp_system_set_bc_interpolate(self._h, block, direction, neighbour, interp_op),
+                   "sbp_system_set_bc_interpolate")
+
+    def set_bc_multigrid(self, block, direction, pieces):
+        """BoundaryCharacteristic::MultiGrid(pieces) (euler/src/lib.rs:744): (block, start, end) triples."""
+        flat = (ctypes.c_int * (3 * len(pieces)))(*[int(v) for p in pieces for v in p])
+        _lib.check(_lib.lib().sbp_system_set_bc_multigrid(self._h, block, direction, len(pieces), flat),
+                   "sbp_system_set_bc_multigrid")
+
     def set_rhs_kind(self, kind):
         _lib.check(_lib.lib().sbp_system_set_rhs_kind(self._h, kind), "sbp_system_set_rhs_kind")
 

@@ -7,7 +7,7 @@ straight from its stage-input buffer.
     sys.set_boundary(0, NORTH, Grid(1)); sys.set_boundary(1, SOUTH, Grid(0)) ...
     sys.vortex(0.0, params); sys.set_dt(sys.max_dt()); sys.advance(100)
 """
-from dataclasses import dataclass
+from dataclasses import dataclass, field
 
 from . import _lib
 from .euler import BlockSystem, VortexParameters
@@ -31,6 +31,22 @@ class Vortex:
     params: VortexParameters
 
 
+@dataclass
+class Interpolate:
+    """BoundaryCharacteristic::Interpolate(j, operator) (euler/src/lib.rs:743):
+    block j's opposite edge through a 2:1 interpolation operator
+    (operators.Interpolation4() ... Interpolation9h2())."""
+    block: int
+    operator: object
+
+
+@dataclass
+class MultiGrid:
+    """BoundaryCharacteristic::MultiGrid(pieces) (euler/src/lib.rs:744):
+    (block, start, end) pieces of the neighbours' opposite edges."""
+    pieces: list = field(default_factory=list)
+
+
 class System(BlockSystem):
     def __init__(self, grids, operators):
         super().__init__(grids, operators)
@@ -43,6 +59,10 @@ class System(BlockSystem):
             self.set_bc(block, direction, _lib.BC_GRID, bc.block)
         elif isinstance(bc, Vortex):
             self.set_bc(block, direction, _lib.BC_VORTEX, 0, bc.params)
+        elif isinstance(bc, Interpolate):
+            self.set_bc_interpolate(block, direction, bc.block, bc.operator.ID)
+        elif isinstance(bc, MultiGrid):
+            self.set_bc_multigrid(block, direction, bc.pieces)
         else:
             raise TypeError(f"unsupported boundary condition {bc!r}")
 

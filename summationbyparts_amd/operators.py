@@ -206,3 +206,66 @@ class OperatorPair:
 def as_pair_ids(op):
     """(op_eta, op_xi) of an operator or an operator pair."""
     return op.ids
+
+
+class _Interpolation:
+    """InterpolationOperator (sbp/src/operators.rs:108-113): 2:1 interface
+    interpolation of a 1-D array. `fine`/`coarse` are float64 CUDA tensors
+    (used in place) or numpy arrays (staged through device memory); the
+    output is fully overwritten. fine = 2·coarse − 1 points (2·(coarse − 1)
+    for Interpolation9h2)."""
+    ID = None
+    NAME = None
+
+    def fine2coarse(self, fine, coarse):
+        self._run(1, fine, coarse)
+
+    def coarse2fine(self, coarse, fine):
+        self._run(0, coarse, fine)
+
+    def tables(self, fine2coarse):
+        """(block, diag, (jump0, jump1)) of one direction."""
+        shape = (ctypes.c_int * 6)()
+        _lib.check(_lib.lib().sbp_interp_info(self.ID, int(fine2coarse), shape), "sbp_interp_info")
+        br, bc, dr, dc, j0, j1 = list(shape)
+        block, diag = np.zeros((br, bc)), np.zeros((dr, dc))
+        dp = ctypes.POINTER(ctypes.c_double)
+        _lib.check(_lib.lib().sbp_interp_coefficients(self.ID, int(fine2coarse), block.ctypes.data_as(dp),
+                                                       diag.ctypes.data_as(dp)), "sbp_interp_coefficients")
+        return block, diag, (j0, j1)
+
+    def _run(self, f2c, src, dst):
+        import torch
+
+        if _is_torch(src):
+            for a in (src, dst):
+                if a.dtype != torch.float64 or not a.is_cuda or a.dim() != 1 or a.stride(0) != 1:
+                    raise ValueError("interpolation: need contiguous 1-D float64 CUDA tensors")
+            ctx = _lib.context()
+            _lib.check(_lib.lib().sbp_interpolate(ctx, self.ID, f2c, ctypes.c_void_p(src.data_ptr()), src.numel(),
+                                                  ctypes.c_void_p(dst.data_ptr()), dst.numel()), "sbp_interpolate")
+            _lib.check(_lib.lib().sbp_context_synchronize(ctx), "sbp_interpolate")
+            return
+        s = torch.from_numpy(np.ascontiguousarray(src, dtype=np.float64)).cuda()
+        d = torch.empty(len(dst), dtype=torch.float64, device="cuda")
+        self._run(f2c, s, d)
+        dst[...] = d.cpu().numpy()
+
+    def __repr__(self):
+        return self.NAME
+
+
+class Interpolation4(_Interpolation):
+    ID, NAME = _lib.INTERP4, "Interpolation4"
+
+
+class Interpolation8(_Interpolation):
+    ID, NAME = _lib.INTERP8, "Interpolation8"
+
+
+class Interpolation9(_Interpolation):
+    ID, NAME = _lib.INTERP9, "Interpolation9"
+
+
+class Interpolation9h2(_Interpolation):
+    ID, NAME = _lib.INTERP9H2, "Interpolation9h2"

@@ -43,6 +43,19 @@ def test_tables_match_oracle(name, op):
             assert np.array_equal(a, b)
 
 
+@pytest.mark.parametrize("cls,iop", [("Interpolation4", O.INTERP4), ("Interpolation8", O.INTERP8),
+                                     ("Interpolation9", O.INTERP9), ("Interpolation9h2", O.INTERP9H2)])
+def test_interpolation_tables_match_oracle(cls, iop):
+    """The library's 2:1 interpolation tables equal the oracle's bit for bit
+    (both carry the reference's literals)."""
+    from summationbyparts_amd import operators
+    g = getattr(operators, cls)()
+    for f2c in (True, False):
+        gb, gd, gj = g.tables(f2c)
+        ob, od, oj = O.interp_tables(iop, f2c)
+        assert gj == oj and np.array_equal(gb, ob) and np.array_equal(gd, od), f2c
+
+
 def test_status_strings_and_invalid_args():
     from summationbyparts_amd import _lib
     L = _lib.lib()

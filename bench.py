@@ -9,8 +9,9 @@ state resident in HBM.
 Prints one JSON line: throughput in Gpoint-RK4-steps/s, the HBM roofline of
 the stage kernel (algorithmic bytes per launch / mean launch duration from HIP
 events), and the CPU oracle timed on this host on a bounded sample.
-`--precision mixed` runs the FP32-RHS/FP64-state form; `--workload c3|c5`
-prints the same line for the 24-block Grid layout and the 256-solve batch.
+`--precision mixed` runs the FP32-RHS/FP64-state form; `--workload c3|c5|interp`
+prints the same line for the 24-block Grid layout, the 256-solve batch and a
+two-block layout with 2:1 interpolated interfaces.
 """
 import argparse
 import json
@@ -189,6 +190,60 @@ def build_c3(precision, cpu_steps):
     return Workload(sys_, dt, pts, desc, {"blocks": len(grids), "operator": "upwind4"}, cpu)
 
 
+def two_to_one_layout(nc):
+    """A coarse block [-5, 5] x [-5, 0] (nc x nc) under a fine block
+    [-5, 5] x [0, 5] ((2nc-1) x (2nc-1)), Cartesian; the shared edges (and
+    the periodic wrap in y) are Interpolate couplings, x is periodic (This)."""
+    from common import cartesian
+    nf = 2 * nc - 1
+    xc, yc = cartesian(nc, nc)
+    xf, yf = cartesian(nf, nf)
+    return [(xc, (yc + 5.0) / 2.0 - 5.0), (xf, (yf + 5.0) / 2.0)]
+
+
+def build_interp(precision, cpu_steps, nc=2048):
+    """Two blocks with 2:1 interfaces: coarse nc x nc under fine (2nc-1)^2,
+    upwind9, Interpolation9 on the shared and wrapped edges, vortex on the
+    interface, dt = 0.2·(fine spacing)."""
+    from summationbyparts_amd import _lib, euler, multigrid, operators
+
+    grids = two_to_one_layout(nc)
+    sys_ = multigrid.System(grids, [operators.Upwind9()] * 2)
+    if precision == "mixed":
+        sys_.set_precision(_lib.MIXED)
+    I = operators.Interpolation9()
+    for b, o in ((0, 1), (1, 0)):
+        for d in (multigrid.NORTH, multigrid.SOUTH):
+            sys_.set_boundary(b, d, multigrid.Interpolate(o, I))
+    vp = euler.VortexParameters([euler.Vortice(-1.0, 0.0, 0.5, 1.0)], 0.5)
+    sys_.vortex(0.0, vp)
+    dt = 0.2 * 5.0 / (2 * nc - 2)
+    pts = sum(g[0].size for g in grids)
+    sample = 256
+
+    def cpu():
+        # the oracle on the same layout at nc = 256 (one thread per block)
+        from oracle import oracle as O
+        g = two_to_one_layout(sample)
+        ref = O.System(g, [(O.UPWIND9, O.UPWIND9)] * 2)
+        for b, o in ((0, 1), (1, 0)):
+            for d in (O.NORTH, O.SOUTH):
+                ref.set_bc_interpolate(b, d, o, O.INTERP9)
+        ref.vortex(0.0, O.VortexParams.make([(-1.0, 0.0, 0.5, 1.0)], 0.5))
+        n = sum(x.size for x, _ in g)
+        steps = 4 * cpu_steps
+        t0 = time.perf_counter()
+        ref.advance(0.2 * 5.0 / (2 * sample - 2), steps, 2)
+        sec = time.perf_counter() - t0
+        return _baseline(n * steps, sec, 2,
+                         f"{steps} RK4 steps of the same 2:1 layout at coarse {sample}^2 / fine {2 * sample - 1}^2, "
+                         f"oracle, one thread per block")
+
+    desc = (f"2:1 layout: coarse {nc}^2 under fine {2 * nc - 1}^2, Cartesian, upwind9, Interpolation9 interfaces "
+            f"(periodic wrap), vortex, dt={dt:.6g}")
+    return Workload(sys_, dt, pts, desc, {"blocks": 2, "operator": "upwind9", "interpolation": "Interpolation9"}, cpu)
+
+
 def build_c5(precision, cpu_steps):
     """256 independent curvilinear vortex solves (seed 5: sizes in [48, 512]^2,
     Mach, eps, grid amplitude per solve, r* = 1, far-field vortex edges),
@@ -240,8 +295,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--n", type=int, default=4096)
     ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
-    ap.add_argument("--workload", choices=["c2", "c3", "c5"], default="c2",
-                    help="c2: one 4096^2 curvilinear block (default); c3: 24-block Grid layout; c5: 256-solve batch")
+    ap.add_argument("--workload", choices=["c2", "c3", "c5", "interp"], default="c2",
+                    help="c2: one 4096^2 curvilinear block (default); c3: 24-block Grid layout; c5: 256-solve batch; "
+                         "interp: 2048^2 block under a 4095^2 block, 2:1 Interpolate interfaces")
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--tile", type=int, default=-1, help="force a tile shape index (-1: autotune)")
@@ -300,6 +356,8 @@ def main():
         w = build_c2(args.n, args.precision, args.cpu_steps)
     elif args.workload == "c3":
         w = build_c3(args.precision, args.cpu_steps)
+    elif args.workload == "interp":
+        w = build_interp(args.precision, args.cpu_steps)
     else:
         w = build_c5(args.precision, args.cpu_steps)
     sys_ = w.sys

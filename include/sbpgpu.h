@@ -221,6 +221,17 @@ int sbp_system_error_vortex_block(sbp_system sys, int block, const sbp_vortex_pa
 /* ||q||_H^2 summed over blocks and components (the H-norm energy used for
  * drift reports), compensated FP64 sums on the device. */
 int sbp_system_energy(sbp_system sys, double* out);
+/* euler::System::advance_adaptive (euler/src/lib.rs:149-185): embedded
+ * Bogacki-Shampine steps (integrate_embedded_rk, utils/integrate/src/lib.rs:
+ * 138-151, 176-251) of trial size *guess_dt until the loop time reaches dt,
+ * accepting a step when the sum over blocks of Field::h2_err between the
+ * previous state and the embedded solution is below maxerr (the guess then
+ * grows by 1.05, else shrinks by 0.8); *guess_dt returns the last guess. As
+ * in the reference, the loop time grows by the trial step on every attempt
+ * and once more on acceptance, and boundary data sees the loop's stage times
+ * (the reference applies This on every edge); the system time is not
+ * advanced. FP64 only (SBP_ERR_INVALID in the mixed form). */
+int sbp_system_advance_adaptive(sbp_system sys, double dt, double* guess_dt, double maxerr);
 /* Sticky device error word of the last advance (0 = none); cleared by reading. */
 int sbp_system_check(sbp_system sys);
 

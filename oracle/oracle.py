@@ -82,6 +82,8 @@ def _load():
     lib.orc_interp_info.argtypes = [ctypes.c_int, ctypes.c_int, ip]
     lib.orc_interp_coefficients.argtypes = [ctypes.c_int, ctypes.c_int, dp, dp]
     lib.orc_system_edge.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_double, dp]
+    lib.orc_system_embedded_step.argtypes = [ctypes.c_void_p, dp, ctypes.c_double, dp, ctypes.c_int]
+    lib.orc_system_advance_adaptive.argtypes = [ctypes.c_void_p, ctypes.c_double, dp, ctypes.c_double, ctypes.c_int]
     return lib
 
 
@@ -267,6 +269,25 @@ class System:
     @time.setter
     def time(self, t):
         lib().orc_system_set_time(self.h, t)
+
+    def embedded_step(self, time, dt, nthreads=1):
+        """One BogackiShampine step (integrate_embedded_rk) at loop time `time`;
+        the state becomes fut; returns (time + dt, fut2 per block)."""
+        t = ctypes.c_double(time)
+        sizes = [4 * s[0] * s[1] for s in self.shapes]
+        out = np.zeros(sum(sizes))
+        _check(lib().orc_system_embedded_step(self.h, ctypes.byref(t), dt, _ptr(out), nthreads), "embedded_step")
+        res, off = [], 0
+        for s, n in zip(self.shapes, sizes):
+            res.append(out[off:off + n].reshape((4,) + s))
+            off += n
+        return t.value, res
+
+    def advance_adaptive(self, dt, guess_dt, maxerr, nthreads=1):
+        """euler::System::advance_adaptive; returns the updated guess."""
+        g = ctypes.c_double(guess_dt)
+        _check(lib().orc_system_advance_adaptive(self.h, dt, ctypes.byref(g), maxerr, nthreads), "advance_adaptive")
+        return g.value
 
     def max_dt(self):
         out = ctypes.c_double()

@@ -26,6 +26,9 @@
 //   interpolate    sbp/src/operators/interpolation.rs:15-50 (2:1 interfaces;
 //                  tables in sbp_interp_tables.h)
 //   integrate      utils/integrate/src/lib.rs:30-35, :176-222 (Rk4)
+//   embedded RK    utils/integrate/src/lib.rs:138-151, :176-251
+//                  (BogackiShampine, integrate_embedded_rk)
+//   adaptive       euler/src/lib.rs:149-185 (advance_adaptive)
 //   h2_err         euler/src/lib.rs:377-432
 //   max_dt         multigrid/src/system.rs:542-589
 //   error          multigrid/src/system.rs:618-635
@@ -733,6 +736,48 @@ int system_step(System& S, double dt, int nthreads) {
   return ST_OK;
 }
 
+// integrate_embedded_rk::<BogackiShampine> (integrate/src/lib.rs:232-251)
+// over integrate (:176-222): stage i > 0 takes fut = prev + Σ (a·dt)·k_j and
+// time t + dt·C[i-1], where BogackiShampine's C (:150) starts with its 0, so
+// the stage times are t, t, t + dt/2, t + 3dt/4; fut = prev + Σ (b·dt)·k_j,
+// time += dt; fut2 = prev + Σ (b*·dt)·k_j. fut lands in fnext.
+int system_step_bs(System& S, double dt, double& time, std::vector<std::vector<double>>& fut2, int nthreads) {
+  static const double A[3][3] = {{1.0 / 2.0, 0.0, 0.0}, {0.0, 3.0 / 4.0, 0.0}, {2.0 / 9.0, 1.0 / 3.0, 4.0 / 9.0}};
+  static const double Bw[4] = {2.0 / 9.0, 1.0 / 3.0, 4.0 / 9.0, 0.0};
+  static const double Bs[4] = {7.0 / 24.0, 1.0 / 4.0, 1.0 / 3.0, 1.0 / 8.0};
+  static const double C[4] = {0.0, 1.0 / 2.0, 3.0 / 4.0, 1.0};
+  size_t nb = S.blocks.size();
+  auto scaled_add = [&](std::vector<std::vector<double>>& f, int j, double scale) {
+    for (size_t bi = 0; bi < nb; ++bi) {
+      double* y = f[bi].data();
+      const double* kk = S.k[j][bi].data();
+      size_t m = f[bi].size();
+      for (size_t q = 0; q < m; ++q) y[q] = y[q] + scale * kk[q];
+    }
+  };
+  int st;
+  for (int i = 0; i < 4; ++i) {
+    double simtime = time;
+    S.fnext = S.fnow;
+    if (i > 0) {
+      for (int j = 0; j < i; ++j) {
+        if (A[i - 1][j] == 0.0) continue;
+        scaled_add(S.fnext, j, A[i - 1][j] * dt);
+      }
+      simtime = time + dt * C[i - 1];
+    }
+    if ((st = system_rhs(S, S.fnext, simtime, S.k[i], nthreads))) return st;
+  }
+  S.fnext = S.fnow;
+  for (int j = 0; j < 4; ++j)
+    if (Bw[j] != 0.0) scaled_add(S.fnext, j, Bw[j] * dt);
+  time += dt;
+  fut2 = S.fnow;
+  for (int j = 0; j < 4; ++j)
+    if (Bs[j] != 0.0) scaled_add(fut2, j, Bs[j] * dt);
+  return ST_OK;
+}
+
 // Field::h2_err, lib.rs:377-432
 double h2_err_impl(const OpDef* oe, const OpDef* ox, const double* a, const double* b, int ny, int nx) {
   std::vector<double> hx(nx), hy(ny);
@@ -1014,6 +1059,50 @@ int orc_system_advance(void* s, double dt, long nsteps, int nthreads) {
   for (long i = 0; i < nsteps; ++i) {
     int st = system_step(*S, dt, nthreads);
     if (st) return st;
+  }
+  return ST_OK;
+}
+
+// One embedded step at loop time *time (advanced by dt): the state becomes
+// fut; fut2 (all blocks concatenated) is written to out2.
+int orc_system_embedded_step(void* s, double* time, double dt, double* out2, int nthreads) {
+  System* S = (System*)s;
+  std::vector<std::vector<double>> fut2;
+  int st = system_step_bs(*S, dt, *time, fut2, nthreads);
+  if (st) return st;
+  size_t off = 0;
+  for (auto& f : fut2) {
+    std::memcpy(out2 + off, f.data(), f.size() * 8);
+    off += f.size();
+  }
+  std::swap(S->fnow, S->fnext);
+  return ST_OK;
+}
+
+// euler::System::advance_adaptive (lib.rs:149-185), for every block of the
+// system with its own boundary settings: the loop time starts at 0 and
+// grows by the trial step inside integrate and again on acceptance; the
+// error is h2_err(previous state, fut2), summed over blocks; accepted: swap,
+// guess ·1.05; rejected: guess ·0.8.
+int orc_system_advance_adaptive(void* s, double dt, double* guess, double maxerr, int nthreads) {
+  System* S = (System*)s;
+  double time = 0.0;
+  std::vector<std::vector<double>> fut2;
+  while (time < dt) {
+    int st = system_step_bs(*S, *guess, time, fut2, nthreads);
+    if (st) return st;
+    double err = 0.0;
+    for (size_t i = 0; i < S->blocks.size(); ++i) {
+      const Block& b = S->blocks[i];
+      err += h2_err_impl(get_op(b.op_eta), get_op(b.op_xi), S->fnow[i].data(), fut2[i].data(), b.ny, b.nx);
+    }
+    if (err < maxerr) {
+      time += *guess;
+      std::swap(S->fnow, S->fnext);
+      *guess *= 1.05;
+    } else {
+      *guess *= 0.8;
+    }
   }
   return ST_OK;
 }

@@ -388,14 +388,15 @@ __device__ __forceinline__ void vortex_point(const VortexDev& vp, double gamma, 
   }
 }
 
-// FP64 stage inputs: y_n, Y2 (BUF_YA), Y3 (BUF_YB), Y4 (BUF_ACC).
-__device__ __forceinline__ const double* stage_input(const BlockDev& B, int mode) {
+// FP64 stage inputs: y_n, Y2 (BUF_YA), Y3 (BUF_YB), Y4 (BUF_ACC); the
+// right-hand-side mode reads rhs_in or the selected work set.
+__device__ __forceinline__ const double* stage_input(const StageArgs& A, const BlockDev& B, int mode) {
   switch (mode) {
     case MODE_S0: return B.buf[BUF_YN];
     case MODE_S1: return B.buf[BUF_YA];
     case MODE_S2: return B.buf[BUF_YB];
     case MODE_S3: return B.buf[BUF_ACC];
-    default: return B.rhs_in;
+    default: return A.rhs_in_sel ? B.rk[A.rhs_in_sel - 1] : B.rhs_in;
   }
 }
 
@@ -428,7 +429,7 @@ __device__ __forceinline__ void load_stage_state(const StageArgs& A, const Block
       for (int c = 0; c < 4; ++c) q[c] += (double)ldg(d + c * ps, o);
     }
   } else {
-    const double* in = stage_input(B, mode);
+    const double* in = stage_input(A, B, mode);
 #pragma unroll
     for (int c = 0; c < 4; ++c) q[c] = ldg(in + c * ps, o);
   }

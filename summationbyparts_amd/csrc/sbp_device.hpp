@@ -49,6 +49,7 @@ struct BlockDev {
   double* buf[4];  // BufRole
   double* rhs_in;  // MODE_RHS input state (4 planes)
   double* rhs_out;  // MODE_RHS output k (4 planes)
+  double* rk[6];    // embedded-RK work sets (4 planes each): k0..k3, stage input, second solution
   double idx_xi, idx_eta;  // 1/dξ and 1/dη (algos.rs:125-130)
   double hi_xi, hi_eta;    // SAT factors (n-1)/H0 (lib.rs:1034-1038,1095-1099)
   double dt_scale;         // this block steps with dt·dt_scale; its time is t·dt_scale
@@ -80,6 +81,7 @@ struct StageArgs {
   double time_offset;  // c_s·dt added to *time for boundary data
   int mixed;  // FP32 right-hand side with FP64 state (SBP_MIXED)
   int swap;   // mixed form, odd steps: y_n lives in BUF_ACC and the accumulator in BUF_YN
+  int rhs_in_sel, rhs_out_sel;  // MODE_RHS buffers: 0 = rhs_in / rhs_out, s > 0 = rk[s - 1]
 };
 
 }  // namespace sbp

@@ -47,6 +47,16 @@ hipError_t launch_maxspeed(const double* q, const double* metrics, int64_t ps, i
                            hipStream_t stream);
 // time += dt on the device.
 hipError_t launch_advance_time(double* time, double dt, hipStream_t stream);
+// out = prev, then out = out + s_j·k_j for j < nk in order, unfused (the
+// reference's clone_from + scaled_add, integrate/src/lib.rs:187-211) over the
+// 4 planes of one block; out may be prev.
+struct LinComb {
+  const double* k[4];
+  double s[4];
+  int nk;
+};
+hipError_t launch_lincomb(double* out, const double* prev, const LinComb& lc, int64_t ps, int64_t ld, int ny, int nx,
+                          hipStream_t stream);
 // 2:1 interpolation of a vector (InterpolationOperator; one thread per output).
 hipError_t launch_interpolate(int interp_op, bool fine2coarse, const double* in, int nin, double* out, int nout,
                               hipStream_t stream);

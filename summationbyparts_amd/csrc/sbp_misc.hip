@@ -326,4 +326,27 @@ hipError_t launch_interpolate(int interp_op, bool fine2coarse, const double* in,
   return hipGetLastError();
 }
 
+// ---------------------------------------------------------------------------
+// Linear combination of an embedded RK step (clone_from + scaled_add in
+// order, integrate/src/lib.rs:187-211; Field::scaled_add euler/src/lib.rs:205-212):
+// each term is added with two roundings, as ndarray does.
+__global__ __launch_bounds__(256) void lincomb_kernel(double* out, const double* prev, LinComb lc, int64_t ps,
+                                                      int64_t ld, int nx, int64_t npts) {
+#pragma clang fp contract(off)
+  const int64_t g = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (g >= 4 * npts) return;
+  const int64_t c = g / npts, r = g - c * npts;
+  const int64_t o = c * ps + (r / nx) * ld + (r % nx);
+  double v = prev[o];
+  for (int j = 0; j < lc.nk; ++j) v = v + lc.s[j] * lc.k[j][o];
+  out[o] = v;
+}
+
+hipError_t launch_lincomb(double* out, const double* prev, const LinComb& lc, int64_t ps, int64_t ld, int ny, int nx,
+                          hipStream_t stream) {
+  const int64_t npts = (int64_t)nx * ny;
+  lincomb_kernel<<<(unsigned)((4 * npts + 255) / 256), 256, 0, stream>>>(out, prev, lc, ps, ld, nx, npts);
+  return hipGetLastError();
+}
+
 }  // namespace sbp

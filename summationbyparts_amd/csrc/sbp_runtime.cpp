@@ -176,6 +176,7 @@ struct HostBlock {
   double* d_y = nullptr;
   double* d_buf[4] = {nullptr, nullptr, nullptr, nullptr};
   double* d_k = nullptr;
+  double* d_rk = nullptr;  // embedded-RK work sets (6 x 4 planes), allocated on first use
   double dt_scale = 1.0;  // the block steps with dt·dt_scale (independent solves in one batch)
   int bc_kind[4] = {SBP_BC_THIS, SBP_BC_THIS, SBP_BC_THIS, SBP_BC_THIS};
   int bc_nb[4] = {0, 0, 0, 0};
@@ -267,6 +268,7 @@ BlockDev make_dev(const sbp_system_s* s, int i, int shape) {
   for (int r = 0; r < 4; ++r) b.buf[r] = h.d_buf[r];
   b.rhs_in = h.d_buf[BUF_YN];
   b.rhs_out = h.d_k;
+  for (int r = 0; r < 6; ++r) b.rk[r] = h.d_rk ? h.d_rk + (int64_t)r * 4 * h.ps : nullptr;
   OpInfo ox, oe;
   op_info(h.op_xi, ox);
   op_info(h.op_eta, oe);
@@ -1005,6 +1007,7 @@ int sbp_system_destroy(sbp_system s) {
     hipFree(h.d_y);
     for (int r = 0; r < 4; ++r) hipFree(h.d_buf[r]);
     if (h.d_k) hipFree(h.d_k);
+    if (h.d_rk) hipFree(h.d_rk);
   }
   if (s->d_blocks) hipFree(s->d_blocks);
   if (s->d_tiles) hipFree(s->d_tiles);
@@ -1375,6 +1378,92 @@ int sbp_system_profile(sbp_system s, double dt, int64_t nsteps, double* ms_total
   *ms_stage_avg = nl ? sum / nl : 0.0;
   hipEventDestroy(e0);
   hipEventDestroy(e1);
+  return check_err(s);
+}
+
+int sbp_system_advance_adaptive(sbp_system s, double dt, double* guess_dt, double maxerr) {
+  if (!s || !guess_dt || !std::isfinite(dt) || !std::isfinite(*guess_dt) || !(*guess_dt > 0.0) || std::isnan(maxerr))
+    return SBP_ERR_INVALID;
+  if (s->precision != SBP_FP64) return SBP_ERR_INVALID;
+  double gamma;
+  if (!gamma_get(gamma)) return SBP_ERR_GAMMA;
+  SBP_TRY_HIP(hipSetDevice(s->ctx->device));
+  hipStream_t st = s->ctx->stream;
+  bool alloc = false;
+  for (auto& h : s->blocks)
+    if (!h.d_rk) {
+      SBP_TRY_HIP(hipMalloc(&h.d_rk, sizeof(double) * 6 * 4 * h.ps));
+      SBP_TRY_HIP(hipMemsetAsync(h.d_rk, 0, sizeof(double) * 6 * 4 * h.ps, st));
+      alloc = true;
+    }
+  if (alloc) s->dirty = true;
+  SBP_TRY(normalize(s));
+  SBP_TRY(rebuild(s));
+  // BogackiShampine (integrate/src/lib.rs:138-151): A, B, C (C carries its
+  // leading 0, so integrate's stage time t + dt·C[i-1] (:190-196) places
+  // stage i at the previous stage's node), embedded weights B*
+  static const double A[3][3] = {{1.0 / 2.0, 0.0, 0.0}, {0.0, 3.0 / 4.0, 0.0}, {2.0 / 9.0, 1.0 / 3.0, 4.0 / 9.0}};
+  static const double Bw[4] = {2.0 / 9.0, 1.0 / 3.0, 4.0 / 9.0, 0.0};
+  static const double Bs[4] = {7.0 / 24.0, 1.0 / 4.0, 1.0 / 3.0, 1.0 / 8.0};
+  static const double C[4] = {0.0, 1.0 / 2.0, 3.0 / 4.0, 1.0};
+  enum { SET_STAGE = 4, SET_SECOND = 5 };  // work sets after k0..k3
+  auto set_of = [](const HostBlock& h, int r) { return h.d_rk + (int64_t)r * 4 * h.ps; };
+  // out = prev + Σ (w_j·g)·k_j over the nonzero weights, in order
+  auto combine = [&](const double* w, int nw, double g, int out_set) -> int {
+    for (auto& h : s->blocks) {
+      LinComb lc{};
+      for (int j = 0; j < nw; ++j) {
+        if (w[j] == 0.0) continue;
+        lc.k[lc.nk] = set_of(h, j);
+        lc.s[lc.nk] = w[j] * g;
+        ++lc.nk;
+      }
+      double* out = out_set < 0 ? h.d_buf[BUF_YA] : set_of(h, out_set);
+      SBP_TRY_HIP(launch_lincomb(out, h.d_buf[BUF_YN], lc, h.ps, h.ld, h.ny, h.nx, st));
+    }
+    return SBP_OK;
+  };
+  // advance_adaptive (euler/src/lib.rs:149-185): the loop's time starts at 0
+  // and grows by the trial step inside integrate and once more on acceptance;
+  // the error is Field::h2_err of the previous state against the embedded
+  // (second) solution; accepted steps grow the guess by 1.05, rejected ones
+  // shrink it by 0.8. The boundary data sees the loop's stage times.
+  double time = 0.0, g = *guess_dt;
+  while (time < dt) {
+    for (int i = 0; i < 4; ++i) {
+      double stime = time;
+      if (i > 0) {
+        SBP_TRY(combine(A[i - 1], i, g, SET_STAGE));
+        stime = time + g * C[i - 1];
+      }
+      StageArgs a = stage_args(s, 0.0, 0, 0);
+      a.time = nullptr;
+      a.time_offset = stime;
+      a.rk_a = a.rk_b = 0.0;
+      a.rhs_in_sel = i == 0 ? 0 : SET_STAGE + 1;
+      a.rhs_out_sel = i + 1;
+      for (const Group& grp : s->groups) SBP_TRY_HIP(launch_group(s, grp, grp.shape, MODE_RHS, a, st));
+    }
+    SBP_TRY(combine(Bw, 4, g, -1));
+    SBP_TRY(combine(Bs, 4, g, SET_SECOND));
+    time += g;
+    double err = 0.0;
+    for (int bi = 0; bi < (int)s->blocks.size(); ++bi) {
+      long double sum;
+      SBP_TRY(h2_sum(s, bi, set_of(s->blocks[bi], SET_SECOND), nullptr, 0.0, sum));
+      err += std::sqrt((double)sum);
+    }
+    if (err < maxerr) {
+      time += g;
+      for (auto& h : s->blocks)
+        SBP_TRY_HIP(launch_copy_planes(h.d_buf[BUF_YA], h.ps, h.ld, h.d_buf[BUF_YN], h.ps, h.ld, 4, h.ny, h.nx, st));
+      g *= 1.05;
+    } else {
+      g *= 0.8;
+    }
+  }
+  *guess_dt = g;
+  SBP_TRY_HIP(hipStreamSynchronize(st));
   return check_err(s);
 }
 

@@ -143,7 +143,7 @@ __global__ __launch_bounds__((TX / RX) * (TY / RY)) void stage_kernel(const Stag
   tile_place(T.ty, B.nty, ny, TY, OY::M, y0, wy0, wy1);
   // stage input: FP64 planes, plus the FP32 increment planes in the mixed form
   const bool mixs = MIX && mode >= MODE_S0;
-  const double* __restrict__ in = mixs ? yn_buf(B, A) : stage_input(B, mode);
+  const double* __restrict__ in = mixs ? yn_buf(B, A) : stage_input(A, B, mode);
   const float* __restrict__ din = (MIX && mode >= MODE_S1) ? delta_in(B, mode) : nullptr;
   const double* __restrict__ met = B.metrics;
   const double gamma = A.gamma;
@@ -605,7 +605,7 @@ __global__ __launch_bounds__((TX / RX) * (TY / RY)) void stage_kernel(const Stag
     }
     if (mode == MODE_RHS) {
 #pragma unroll
-      for (int q = 0; q < 4; ++q) stg_nt(B.rhs_out + q * ps, o, k[q]);
+      for (int q = 0; q < 4; ++q) stg_nt((A.rhs_out_sel ? B.rk[A.rhs_out_sel - 1] : B.rhs_out) + q * ps, o, k[q]);
     } else if constexpr (MIX) {
       double* accb = acc_buf(B, A);
       if (mode == MODE_S3) {

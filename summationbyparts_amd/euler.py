@@ -209,6 +209,14 @@ class BlockSystem:
         _lib.check(_lib.lib().sbp_system_energy(self._h, ctypes.byref(out)), "sbp_system_energy")
         return out.value
 
+    def advance_adaptive_steps(self, dt, guess_dt, maxerr):
+        """sbp_system_advance_adaptive with the system's boundaries and RHS
+        kind; returns the updated guess."""
+        g = ctypes.c_double(guess_dt)
+        _lib.check(_lib.lib().sbp_system_advance_adaptive(self._h, dt, ctypes.byref(g), maxerr),
+                   "sbp_system_advance_adaptive")
+        return g.value
+
     def advance_timed(self, dt, nsteps):
         """advance_steps(dt, nsteps); returns the ms between HIP events
         recorded on the library's stream around the steps."""
@@ -245,6 +253,15 @@ class System(BlockSystem):
             raise _lib.SbpError(_lib.ERR_NOT_UPWIND, "advance_upwind")
         self.set_rhs_kind(_lib.RHS_UPWIND)
         self.advance_steps(dt, 1)
+
+    def advance_adaptive(self, dt, guess_dt, maxerr):
+        """euler::System::advance_adaptive (lib.rs:149-185): embedded
+        Bogacki-Shampine steps with RHS_upwind until the reference's loop time
+        reaches dt; returns the updated guess (the reference's `&mut guess_dt`)."""
+        if self.op.upwind() is None:
+            raise _lib.SbpError(_lib.ERR_NOT_UPWIND, "advance_adaptive")
+        self.set_rhs_kind(_lib.RHS_UPWIND)
+        return self.advance_adaptive_steps(dt, guess_dt, maxerr)
 
     def advance_n(self, dt, nsteps, upwind=True):
         """nsteps RK4 steps in one call (graph-replayed)."""

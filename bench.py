@@ -44,6 +44,12 @@ HBM_PEAK_GBS = 8000.0
 # metrics, acc and write acc, Δ_{s+1} (168 B each); stage 4 reads y_n, Δ_3,
 # metrics, acc and writes y_{n+1} (152 B): 608 B.
 BYTES_PER_POINT_STEP = {"fp64": 544.0, "mixed": 608.0}
+# advance_adaptive's embedded Bogacki-Shampine step, unfused: four RHS passes
+# (stage input, 4 metric planes in, k out: 96 B each), the stage inputs
+# (y_n + k0; y_n + k1; y_n + k0..k2: 96, 96, 160 B), fut (160 B), the
+# embedded solution (192 B), the H-norm error (64 B) and the copy of an
+# accepted step (64 B): 1216 B per point per accepted step.
+BYTES_PER_POINT_ADAPTIVE_STEP = 1216.0
 REFERENCE_SCHEDULE_BYTES = 672.0
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "stage_kernel_upwind9_4096_traffic.json")
 
@@ -190,6 +196,69 @@ def build_c3(precision, cpu_steps):
     return Workload(sys_, dt, pts, desc, {"blocks": len(grids), "operator": "upwind4"}, cpu)
 
 
+def run_adaptive(args):
+    """advance_adaptive on the 4096² curvilinear upwind9 block (periodic,
+    vortex + noise) with every step accepted (maxerr = inf): up to 40 embedded
+    Bogacki-Shampine steps from a guess of 0.2/4095, the loop time growing by
+    twice the step and the guess by 1.05 per step (after 40 steps the step is
+    7x the first, a CFL number of about 0.4; many more would pass the
+    stability limit)."""
+    import torch
+    from common import curvilinear, density_noise
+    from summationbyparts_amd import euler, operators
+
+    n = args.n
+    X, Y = curvilinear(n, n)
+    vp = euler.VortexParameters([euler.Vortice(-1.0, 0.0, 0.5, 1.0)], 0.5)
+    s = euler.System(X, Y, operators.Upwind9())
+    s.vortex(0.0, vp)
+    q = s.field()
+    q[0] += density_noise(1, 0, n, n)
+    s.set_state(0, q)
+    g0 = 0.2 / (n - 1)
+
+    def span(k):
+        # loop time reached after k accepted steps from guess g0, minus a
+        # margin, so that exactly k steps are taken
+        t, g = 0.0, g0
+        for _ in range(k):
+            t += 2.0 * g
+            g *= 1.05
+        return t - 0.5 * g0
+
+    steps, warmup = min(args.steps, 40), min(args.warmup, 2)
+    s.advance_adaptive(span(warmup), g0, np.inf)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    g = s.advance_adaptive(span(steps), g0, np.inf)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    assert abs(g - g0 * 1.05 ** steps) <= 1e-12 * g
+    pts = n * n
+    ms = wall * 1e3 / steps
+    achieved = BYTES_PER_POINT_ADAPTIVE_STEP * pts / (ms * 1e-3) / 1e9
+    out = {"metric": "Gpoint-embedded-RK-steps/s (advance_adaptive, Bogacki-Shampine)", "value": pts * steps / wall / 1e9,
+           "unit": "Gpoint-steps/s", "n_gpus": 1, "steps": steps, "warmup": warmup, "ms_per_step": ms,
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+           "config": {"workload": f"advance_adaptive on the {n}x{n} curvilinear upwind9 block, periodic, all steps accepted",
+                      "nx": n, "ny": n, "operator": "upwind9"},
+           "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": achieved / HBM_PEAK_GBS, "traffic": None}}
+    if not args.no_cpu_baseline:
+        from oracle import oracle as O
+        m = 512
+        Xs, Ys = curvilinear(m, m)
+        ref = O.System([(Xs, Ys)], [(O.UPWIND9, O.UPWIND9)])
+        ref.set_rhs_kind(O.RHS_UPWIND)
+        ref.vortex(0.0, O.VortexParams.make([(-1.0, 0.0, 0.5, 1.0)], 0.5))
+        t0 = time.perf_counter()
+        ref.advance_adaptive(4.0 * 0.2 / (m - 1), 0.2 / (m - 1), np.inf)
+        sec = time.perf_counter() - t0
+        out["cpu_baseline"] = _baseline(m * m * 2, sec, 1, f"advance_adaptive, 2 accepted steps of a {m}x{m} block, oracle, 1 thread")
+        out["cpu_baseline"]["unit"] = out["unit"]
+    print(json.dumps(out), flush=True)
+
+
 def two_to_one_layout(nc):
     """A coarse block [-5, 5] x [-5, 0] (nc x nc) under a fine block
     [-5, 5] x [0, 5] ((2nc-1) x (2nc-1)), Cartesian; the shared edges (and
@@ -295,9 +364,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--n", type=int, default=4096)
     ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
-    ap.add_argument("--workload", choices=["c2", "c3", "c5", "interp"], default="c2",
+    ap.add_argument("--workload", choices=["c2", "c3", "c5", "interp", "adaptive"], default="c2",
                     help="c2: one 4096^2 curvilinear block (default); c3: 24-block Grid layout; c5: 256-solve batch; "
-                         "interp: 2048^2 block under a 4095^2 block, 2:1 Interpolate interfaces")
+                         "interp: 2048^2 block under a 4095^2 block, 2:1 Interpolate interfaces; "
+                         "adaptive: advance_adaptive (embedded Bogacki-Shampine) on the 4096^2 block")
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--tile", type=int, default=-1, help="force a tile shape index (-1: autotune)")
@@ -352,6 +422,9 @@ def main():
     from summationbyparts_amd import euler
 
     euler.set_gamma(1.4)
+    if args.workload == "adaptive":
+        run_adaptive(args)
+        return
     if args.workload == "c2":
         w = build_c2(args.n, args.precision, args.cpu_steps)
     elif args.workload == "c3":

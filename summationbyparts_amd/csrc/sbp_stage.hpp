@@ -131,6 +131,10 @@ __global__ __launch_bounds__((TX / RX) * (TY / RY)) void stage_kernel(const Stag
   constexpr bool OWNQ = !MIX && RX == 1 && RY <= LCH;
   constexpr int ITYE = OWNQ ? RY + (2 * G::HY * TX + NT - 1) / NT : G::ITY;
   constexpr int ITXE = OWNQ ? RY + (2 * G::HX * TY + NT - 1) / NT : G::ITX;
+  // load batches per phase: all region points of a thread at once when they
+  // are at most four (one memory round trip per phase), else LCH at a time
+  constexpr int LBX = ITXE <= 4 ? ITXE : LCH;
+  constexpr int LBY = ITYE <= 4 ? ITYE : LCH;
   __shared__ __align__(16) CT lds[NV * NP];
 
   const TileDev T = A.tiles[blockIdx.x];
@@ -229,13 +233,15 @@ __global__ __launch_bounds__((TX / RX) * (TY / RY)) void stage_kernel(const Stag
       }
     };
     CT cache[ITXE][NC];
-    // loads of up to LCH region points are issued together (invalid slots
-    // read a clamped in-bounds point) so that their latencies overlap
+    // loads of up to LBX region points are issued together (invalid points
+    // read a clamped in-bounds point; slots past ITXE load nothing) so that
+    // their latencies overlap
 #pragma unroll
-    for (int i0 = 0; i0 < ITXE; i0 += LCH) {
-      double v[LCH][6];
+    for (int i0 = 0; i0 < ITXE; i0 += LBX) {
+      double v[LBX][6];
 #pragma unroll
-      for (int j = 0; j < LCH; ++j) {
+      for (int j = 0; j < LBX; ++j) {
+        if (i0 + j >= ITXE) continue;
         int r, c;
         const bool ok = i0 + j < ITXE && xi_pt(i0 + j, r, c);
         const uint32_t o = ok ? (uint32_t)((y0 + r) * ld + rx0 + c) : (uint32_t)(y0 * ld + rx0);
@@ -260,7 +266,7 @@ __global__ __launch_bounds__((TX / RX) * (TY / RY)) void stage_kernel(const Stag
         }
       }
 #pragma unroll
-      for (int j = 0; j < LCH; ++j) {
+      for (int j = 0; j < LBX; ++j) {
         const int it = i0 + j;
         int r, c;
         if (it < ITXE && xi_pt(it, r, c)) {
@@ -428,10 +434,11 @@ __global__ __launch_bounds__((TX / RX) * (TY / RY)) void stage_kernel(const Stag
     };
     CT cache[ITYE][NC];
 #pragma unroll
-    for (int i0 = 0; i0 < ITYE; i0 += LCH) {
-      double v[LCH][6];
+    for (int i0 = 0; i0 < ITYE; i0 += LBY) {
+      double v[LBY][6];
 #pragma unroll
-      for (int j = 0; j < LCH; ++j) {
+      for (int j = 0; j < LBY; ++j) {
+        if (i0 + j >= ITYE) continue;
         int r, c;
         const bool ok = i0 + j < ITYE && eta_pt(i0 + j, r, c);
         const uint32_t o = ok ? (uint32_t)((ry0 + r) * ld + x0 + c) : (uint32_t)(ry0 * ld + x0);
@@ -447,7 +454,7 @@ __global__ __launch_bounds__((TX / RX) * (TY / RY)) void stage_kernel(const Stag
         }
       }
 #pragma unroll
-      for (int j = 0; j < LCH; ++j) {
+      for (int j = 0; j < LBY; ++j) {
         const int it = i0 + j;
         int r, c;
         if (it < ITYE && eta_pt(it, r, c)) {

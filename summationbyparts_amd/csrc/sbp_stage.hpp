@@ -126,9 +126,10 @@ __global__ __launch_bounds__((TX / RX) * (TY / RY)) void stage_kernel(const Stag
   constexpr int NV = UP ? G::NV : 4;
   constexpr int NC = VAR == 0 ? 4 : 1;  // cached values per region point between passes
   constexpr int LCH = 3;                // region points whose loads are in flight together
-  // FP64: the η phase's first RY points of a thread are its own output points,
-  // whose stage input then serves the stage combination without a reload
-  constexpr bool OWNQ = !MIX && RX == 1 && RY <= LCH;
+  // Both phases start with a thread's own output points, loaded once in the
+  // ξ phase; their stage input serves the η phase and (FP64) the stage
+  // combination without a reload
+  constexpr bool OWNQ = RX == 1 && RY <= LCH;
   constexpr int ITYE = OWNQ ? RY + (2 * G::HY * TX + NT - 1) / NT : G::ITY;
   constexpr int ITXE = OWNQ ? RY + (2 * G::HX * TY + NT - 1) / NT : G::ITX;
   // load batches per phase: all region points of a thread at once when they

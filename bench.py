@@ -39,11 +39,12 @@ HBM_PEAK_GBS = 8000.0
 # The reference's schedule (running accumulator, 5 metric planes) needs
 # 672 B; bench also reports the rate at which that schedule's bytes would
 # have to move to match this kernel ("reference_schedule").
-# Mixed: the stage inputs are y_n + Δ_s with FP32 increments Δ_s: stage 1
-# reads y_n, metrics, writes acc, Δ_1 (120 B); stages 2-3 read y_n, Δ_s,
-# metrics, acc and write acc, Δ_{s+1} (168 B each); stage 4 reads y_n, Δ_3,
-# metrics, acc and writes y_{n+1} (152 B): 608 B.
-BYTES_PER_POINT_STEP = {"fp64": 544.0, "mixed": 608.0}
+# Mixed: the stage inputs are y_n + Δ_s with FP32 increments Δ_s, and the
+# last stage sums y_n and the increments in FP64: stage 1 reads y_n, metrics,
+# writes Δ_1 (80 B); stages 2-3 read y_n, Δ_s, metrics and write Δ_{s+1}
+# (96 B each); stage 4 reads y_n, Δ_3, metrics, Δ_1, Δ_2 and writes y_{n+1}
+# (144 B): 416 B.
+BYTES_PER_POINT_STEP = {"fp64": 544.0, "mixed": 416.0}
 # advance_adaptive's embedded Bogacki-Shampine step, unfused: four RHS passes
 # (stage input, 4 metric planes in, k out: 96 B each), the stage inputs
 # (y_n + k0; y_n + k1; y_n + k0..k2: 96, 96, 160 B), fut (160 B), the

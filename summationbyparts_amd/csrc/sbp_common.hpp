@@ -407,13 +407,15 @@ __device__ __forceinline__ double* yn_buf(const BlockDev& B, const StageArgs& A)
 __device__ __forceinline__ double* acc_buf(const BlockDev& B, const StageArgs& A) {
   return A.swap ? B.buf[BUF_YN] : B.buf[BUF_ACC];
 }
-// Stage-increment set read by stage `mode` (S1, S3: set 0; S2: set 1) and
-// written by it (S0, S2: set 0; S1: set 1).
-__device__ __forceinline__ const float* delta_in(const BlockDev& B, int mode) {
-  return reinterpret_cast<const float*>(B.buf[BUF_YA]) + (mode == MODE_S2 ? 4 * B.ps : 0);
+// Stage-increment sets of the mixed form: Δ_1 (set 0) and Δ_2 (set 1) in the
+// YA buffer, Δ_3 (set 2) in the first half of YB. Stage s (S0..S2) writes
+// set s; stage s (S1..S3) reads set s - 1 for its input y_n + Δ_s.
+__device__ __forceinline__ const float* delta_set(const BlockDev& B, int set) {
+  return reinterpret_cast<const float*>(B.buf[set == 2 ? BUF_YB : BUF_YA]) + (set == 1 ? 4 * B.ps : 0);
 }
+__device__ __forceinline__ const float* delta_in(const BlockDev& B, int mode) { return delta_set(B, mode - 1); }
 __device__ __forceinline__ float* delta_out(const BlockDev& B, int mode) {
-  return reinterpret_cast<float*>(B.buf[BUF_YA]) + (mode == MODE_S1 ? 4 * B.ps : 0);
+  return const_cast<float*>(delta_set(B, mode));
 }
 
 // The state stage `mode` reads at element offset o.

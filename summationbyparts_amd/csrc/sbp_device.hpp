@@ -11,9 +11,10 @@ namespace sbp {
 // overwrites Y2 with W = Y2 + 2 Y3 - y_n, from which and Y4 the last stage
 // forms y_{n+1} (see the stage kernel's finish).
 // Mixed form: YA
-// holds the FP32 stage increments Δ_s = Y_s - y_n (two sets of 4 planes, the
-// second at float offset 4·ps), and the last stage writes y_{n+1} over the
-// accumulator, after which y_n and the accumulator swap roles (StageArgs::swap).
+// holds the FP32 stage increments Δ_1 = Y2 - y_n and Δ_2 = Y3 - y_n (two sets
+// of 4 planes, the second at float offset 4·ps), YB holds Δ_3 = Y4 - y_n in
+// its first half; the last stage writes y_{n+1} into the ACC buffer, after
+// which y_n and ACC swap roles (StageArgs::swap).
 enum BufRole : int { BUF_YN = 0, BUF_ACC = 1, BUF_YA = 2, BUF_YB = 3 };
 
 // Boundary-data source of one block edge inside a kernel.

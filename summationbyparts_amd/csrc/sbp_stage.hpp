@@ -18,9 +18,10 @@
 //     (J·ηx, J·ηy), and η-stencils.
 //   finish: k = (idx_ξ·Sξ + idx_η·Sη)/J, RK4 combination, store.
 // Mixed form (MIX): the face terms are FP32 deviations from a per-tile
-// reference state (face_terms_mix), LDS and stencil sums are FP32, and the
-// stage combination, accumulator and y_n stay FP64; the next stage input is
-// stored as the FP32 increment Δ = a·dt·k.
+// reference state (face_terms_mix), LDS and stencil sums are FP32, y_n and
+// the RK4 sum stay FP64; each stage input is y_n + Δ with the FP32 increment
+// Δ = a·dt·k stored by the previous stage, and the last stage sums y_n and
+// the increments in FP64.
 // The characteristic SATs of the block-edge points are added to k by the
 // EDGE instantiation, which the host launches for the tiles on block edges
 // only; the interior tiles' instantiation carries none of the 4x4
@@ -351,11 +352,11 @@ __global__ __launch_bounds__((TX / RX) * (TY / RY)) void stage_kernel(const Stag
   // Finish operands at the output points, all loaded before any store.
   // FP64 (see the finish), with q the stage input held from the η phase:
   //   S0 q = y_n;  S1 fa = y_n, q = Y2;  S2 fa = y_n, fb = Y2, q = Y3;  S3 fa = W, q = Y4
-  // mixed: S0 fa = y_n;  S1, S2 fb = acc;  S3 fa = acc
+  // mixed: S3 fa = y_n, fd = Δ_1, Δ_2, Δ_3 (S0-S2 need no operand)
   double fJ[RY][RX], fa[RY][RX][4], fb[RY][RX][4];
+  float fd[MIX ? RY : 1][MIX ? RX : 1][3][4];
   static_assert(MIX || OWNQ, "the FP64 stage combination takes the stage input from the η phase");
   auto load_finish = [&]() {
-    const double* pacc = MIX ? acc_buf(B, A) : B.buf[BUF_ACC];
 #pragma unroll
     for (int s = 0; s < RY; ++s)
 #pragma unroll
@@ -368,14 +369,13 @@ __global__ __launch_bounds__((TX / RX) * (TY / RY)) void stage_kernel(const Stag
         }
         if (mode == MODE_RHS) continue;
         if constexpr (MIX) {
-          if (mode == MODE_S0 || mode == MODE_S3) {
-            const double* pa = mode == MODE_S0 ? in : pacc;
+          if (mode == MODE_S3) {
 #pragma unroll
-            for (int q = 0; q < 4; ++q) fa[s][i][q] = ldg_nt(pa + q * ps, o);
-          }
-          if (mode == MODE_S1 || mode == MODE_S2) {
+            for (int q = 0; q < 4; ++q) fa[s][i][q] = ldg_nt(in + q * ps, o);
 #pragma unroll
-            for (int q = 0; q < 4; ++q) fb[s][i][q] = ldg_nt(pacc + q * ps, o);
+            for (int t = 0; t < 3; ++t)
+#pragma unroll
+              for (int q = 0; q < 4; ++q) fd[MIX ? s : 0][MIX ? i : 0][t][q] = ldg_nt(delta_set(B, t) + q * ps, o);
           }
         } else if (mode != MODE_S0) {
           // S1, S2: fa = y_n; S2 also fb = Y2; S3: fa = W (in the Y2 buffer)
@@ -615,19 +615,25 @@ __global__ __launch_bounds__((TX / RX) * (TY / RY)) void stage_kernel(const Stag
 #pragma unroll
       for (int q = 0; q < 4; ++q) stg_nt((A.rhs_out_sel ? B.rk[A.rhs_out_sel - 1] : B.rhs_out) + q * ps, o, k[q]);
     } else if constexpr (MIX) {
-      double* accb = acc_buf(B, A);
+      // Mixed stage combination: stages 1-3 store only their FP32 increment
+      // Δ_{s+1} = a_s·dt·k_s (the next stage input is y_n + Δ_{s+1}); the
+      // last stage sums the increments in FP64,
+      // y_{n+1} = y_n + (Δ_1 + 2Δ_2 + Δ_3)/3 + dt/6·k_4, the classical RK4
+      // sum with dt·k_1 = 2Δ_1, dt·k_2 = 2Δ_2, dt·k_3 = Δ_3 (the increments
+      // the stage inputs used), written into the other FP64 buffer (the
+      // roles swap after the step).
       if (mode == MODE_S3) {
-        // y_{n+1} over the accumulator (the roles swap after the step)
+        double* yo = acc_buf(B, A);
+        const float(&d)[3][4] = fd[MIX ? s : 0][MIX ? i : 0];
 #pragma unroll
-        for (int q = 0; q < 4; ++q) stg_nt(accb + q * ps, o, fa[s][i][q] + sacc * k[q]);
+        for (int q = 0; q < 4; ++q) {
+          const double inc = ((double)d[0][q] + 2.0 * (double)d[1][q] + (double)d[2][q]) * (1.0 / 3.0);
+          stg_nt(yo + q * ps, o, fa[s][i][q] + (inc + sacc * k[q]));
+        }
       } else {
         float* dn = delta_out(B, mode);
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
-          const double a0 = mode == MODE_S0 ? fa[s][i][q] : fb[s][i][q];
-          stg_nt(accb + q * ps, o, a0 + sacc * k[q]);
-          stg_nt(dn + q * ps, o, (float)(snext * k[q]));
-        }
+        for (int q = 0; q < 4; ++q) stg_nt(dn + q * ps, o, (float)(snext * k[q]));
       }
     } else {
       // FP64 stage combination, classical RK4 (utils/integrate/src/lib.rs:30-35,176-222):

@@ -42,16 +42,20 @@ __device__ __forceinline__ float fma_t(float a, float b, float c) { return __bui
 // from device descriptors, so the cast to the global address space is what
 // keeps the compiler from emitting flat loads (which also wait on LDS).
 template <typename T>
+__device__ __forceinline__ constexpr uint32_t elt_shift() {
+  return sizeof(T) == 16 ? 4 : sizeof(T) == 8 ? 3 : 2;
+}
+template <typename T>
 __device__ __forceinline__ T ldg(const T* base, uint32_t idx) {
   using GP = const __attribute__((address_space(1))) T*;
   using GC = const __attribute__((address_space(1))) char*;
-  return *(GP)((GC)base + (idx << (sizeof(T) == 8 ? 3 : 2)));
+  return *(GP)((GC)base + (idx << elt_shift<T>()));
 }
 template <typename T>
 __device__ __forceinline__ void stg(T* base, uint32_t idx, T v) {
   using GP = __attribute__((address_space(1))) T*;
   using GC = __attribute__((address_space(1))) char*;
-  *(GP)((GC)base + (idx << (sizeof(T) == 8 ? 3 : 2))) = v;
+  *(GP)((GC)base + (idx << elt_shift<T>())) = v;
 }
 // Streaming store/load (non-temporal hint): for data written or read once per
 // stage, so that it neither allocates nor evicts L2 lines holding halos.
@@ -59,13 +63,13 @@ template <typename T>
 __device__ __forceinline__ void stg_nt(T* base, uint32_t idx, T v) {
   using GP = __attribute__((address_space(1))) T*;
   using GC = __attribute__((address_space(1))) char*;
-  __builtin_nontemporal_store(v, (GP)((GC)base + (idx << (sizeof(T) == 8 ? 3 : 2))));
+  __builtin_nontemporal_store(v, (GP)((GC)base + (idx << elt_shift<T>())));
 }
 template <typename T>
 __device__ __forceinline__ T ldg_nt(const T* base, uint32_t idx) {
   using GP = const __attribute__((address_space(1))) T*;
   using GC = const __attribute__((address_space(1))) char*;
-  return __builtin_nontemporal_load((GP)((GC)base + (idx << (sizeof(T) == 8 ? 3 : 2))));
+  return __builtin_nontemporal_load((GP)((GC)base + (idx << elt_shift<T>())));
 }
 
 // The sticky device error word keeps the first error raised (the reference
@@ -409,14 +413,16 @@ __device__ __forceinline__ double* acc_buf(const BlockDev& B, const StageArgs& A
 }
 // Stage-increment sets of the mixed form: Δ_1 (set 0) and Δ_2 (set 1) in the
 // YA buffer, Δ_3 (set 2) in the first half of YB. Stage s (S0..S2) writes
-// set s; stage s (S1..S3) reads set s - 1 for its input y_n + Δ_s.
-__device__ __forceinline__ const float* delta_set(const BlockDev& B, int set) {
-  return reinterpret_cast<const float*>(B.buf[set == 2 ? BUF_YB : BUF_YA]) + (set == 1 ? 4 * B.ps : 0);
+// set s; stage s (S1..S3) reads set s - 1 for its input y_n + Δ_s. A set
+// holds the four components of a point together (one 16-byte load or store
+// per point), at the point's element offset o of the plane layout.
+using f4 = float __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ const f4* delta_set(const BlockDev& B, int set) {
+  return reinterpret_cast<const f4*>(reinterpret_cast<const float*>(B.buf[set == 2 ? BUF_YB : BUF_YA]) +
+                                     (set == 1 ? 4 * B.ps : 0));
 }
-__device__ __forceinline__ const float* delta_in(const BlockDev& B, int mode) { return delta_set(B, mode - 1); }
-__device__ __forceinline__ float* delta_out(const BlockDev& B, int mode) {
-  return const_cast<float*>(delta_set(B, mode));
-}
+__device__ __forceinline__ const f4* delta_in(const BlockDev& B, int mode) { return delta_set(B, mode - 1); }
+__device__ __forceinline__ f4* delta_out(const BlockDev& B, int mode) { return const_cast<f4*>(delta_set(B, mode)); }
 
 // The state stage `mode` reads at element offset o.
 __device__ __forceinline__ void load_stage_state(const StageArgs& A, const BlockDev& B, int mode, uint32_t o, double q[4]) {
@@ -426,9 +432,9 @@ __device__ __forceinline__ void load_stage_state(const StageArgs& A, const Block
 #pragma unroll
     for (int c = 0; c < 4; ++c) q[c] = ldg(yn + c * ps, o);
     if (mode >= MODE_S1) {
-      const float* d = delta_in(B, mode);
+      const f4 d = ldg(delta_in(B, mode), o);
 #pragma unroll
-      for (int c = 0; c < 4; ++c) q[c] += (double)ldg(d + c * ps, o);
+      for (int c = 0; c < 4; ++c) q[c] += (double)d[c];
     }
   } else {
     const double* in = stage_input(A, B, mode);

@@ -12,8 +12,8 @@ namespace sbp {
 // forms y_{n+1} (see the stage kernel's finish).
 // Mixed form: YA
 // holds the FP32 stage increments Δ_1 = Y2 - y_n and Δ_2 = Y3 - y_n (two sets
-// of 4 planes, the second at float offset 4·ps), YB holds Δ_3 = Y4 - y_n in
-// its first half; the last stage writes y_{n+1} into the ACC buffer, after
+// of 4·ps floats, the second at float offset 4·ps; the four components of a
+// point are adjacent), YB holds Δ_3 = Y4 - y_n in its first half; the last stage writes y_{n+1} into the ACC buffer, after
 // which y_n and ACC swap roles (StageArgs::swap).
 enum BufRole : int { BUF_YN = 0, BUF_ACC = 1, BUF_YA = 2, BUF_YB = 3 };
 

@@ -150,7 +150,7 @@ __global__ __launch_bounds__((TX / RX) * (TY / RY)) void stage_kernel(const Stag
   // stage input: FP64 planes, plus the FP32 increment planes in the mixed form
   const bool mixs = MIX && mode >= MODE_S0;
   const double* __restrict__ in = mixs ? yn_buf(B, A) : stage_input(A, B, mode);
-  const float* __restrict__ din = (MIX && mode >= MODE_S1) ? delta_in(B, mode) : nullptr;
+  const f4* __restrict__ din = (MIX && mode >= MODE_S1) ? delta_in(B, mode) : nullptr;
   const double* __restrict__ met = B.metrics;
   const double gamma = A.gamma;
   const double idx_xi = B.idx_xi, idx_eta = B.idx_eta;
@@ -180,8 +180,9 @@ __global__ __launch_bounds__((TX / RX) * (TY / RY)) void stage_kernel(const Stag
     for (int c = 0; c < 4; ++c) q[c] = ldg(in + c * ps, o);
     if constexpr (MIX) {
       if (din) {
+        const f4 d = ldg(din, o);
 #pragma unroll
-        for (int c = 0; c < 4; ++c) q[c] += (double)ldg(din + c * ps, o);
+        for (int c = 0; c < 4; ++c) q[c] += (double)d[c];
       }
     }
   };
@@ -354,7 +355,7 @@ __global__ __launch_bounds__((TX / RX) * (TY / RY)) void stage_kernel(const Stag
   //   S0 q = y_n;  S1 fa = y_n, q = Y2;  S2 fa = y_n, fb = Y2, q = Y3;  S3 fa = W, q = Y4
   // mixed: S3 fa = y_n, fd = Δ_1, Δ_2, Δ_3 (S0-S2 need no operand)
   double fJ[RY][RX], fa[RY][RX][4], fb[RY][RX][4];
-  float fd[MIX ? RY : 1][MIX ? RX : 1][3][4];
+  f4 fd[MIX ? RY : 1][MIX ? RX : 1][3];
   static_assert(MIX || OWNQ, "the FP64 stage combination takes the stage input from the η phase");
   auto load_finish = [&]() {
 #pragma unroll
@@ -373,9 +374,7 @@ __global__ __launch_bounds__((TX / RX) * (TY / RY)) void stage_kernel(const Stag
 #pragma unroll
             for (int q = 0; q < 4; ++q) fa[s][i][q] = ldg_nt(in + q * ps, o);
 #pragma unroll
-            for (int t = 0; t < 3; ++t)
-#pragma unroll
-              for (int q = 0; q < 4; ++q) fd[MIX ? s : 0][MIX ? i : 0][t][q] = ldg_nt(delta_set(B, t) + q * ps, o);
+            for (int t = 0; t < 3; ++t) fd[MIX ? s : 0][MIX ? i : 0][t] = ldg_nt(delta_set(B, t), o);
           }
         } else if (mode != MODE_S0) {
           // S1, S2: fa = y_n; S2 also fb = Y2; S3: fa = W (in the Y2 buffer)
@@ -624,16 +623,17 @@ __global__ __launch_bounds__((TX / RX) * (TY / RY)) void stage_kernel(const Stag
       // roles swap after the step).
       if (mode == MODE_S3) {
         double* yo = acc_buf(B, A);
-        const float(&d)[3][4] = fd[MIX ? s : 0][MIX ? i : 0];
+        const f4(&d)[3] = fd[MIX ? s : 0][MIX ? i : 0];
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
           const double inc = ((double)d[0][q] + 2.0 * (double)d[1][q] + (double)d[2][q]) * (1.0 / 3.0);
           stg_nt(yo + q * ps, o, fa[s][i][q] + (inc + sacc * k[q]));
         }
       } else {
-        float* dn = delta_out(B, mode);
+        f4 dv;
 #pragma unroll
-        for (int q = 0; q < 4; ++q) stg_nt(dn + q * ps, o, (float)(snext * k[q]));
+        for (int q = 0; q < 4; ++q) dv[q] = (float)(snext * k[q]);
+        stg_nt(delta_out(B, mode), o, dv);
       }
     } else {
       // FP64 stage combination, classical RK4 (utils/integrate/src/lib.rs:30-35,176-222):

@@ -41,6 +41,9 @@ __device__ __forceinline__ float fma_t(float a, float b, float c) { return __bui
 // the SGPR-base + VGPR-offset addressing of global loads). The pointers come
 // from device descriptors, so the cast to the global address space is what
 // keeps the compiler from emitting flat loads (which also wait on LDS).
+using f4 = float __attribute__((ext_vector_type(4)));
+using d2 = double __attribute__((ext_vector_type(2)));
+
 template <typename T>
 __device__ __forceinline__ constexpr uint32_t elt_shift() {
   return sizeof(T) == 16 ? 4 : sizeof(T) == 8 ? 3 : 2;
@@ -416,7 +419,7 @@ __device__ __forceinline__ double* acc_buf(const BlockDev& B, const StageArgs& A
 // set s; stage s (S1..S3) reads set s - 1 for its input y_n + Δ_s. A set
 // holds the four components of a point together (one 16-byte load or store
 // per point), at the point's element offset o of the plane layout.
-using f4 = float __attribute__((ext_vector_type(4)));
+
 __device__ __forceinline__ const f4* delta_set(const BlockDev& B, int set) {
   return reinterpret_cast<const f4*>(reinterpret_cast<const float*>(B.buf[set == 2 ? BUF_YB : BUF_YA]) +
                                      (set == 1 ? 4 * B.ps : 0));

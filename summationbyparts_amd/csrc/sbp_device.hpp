@@ -45,6 +45,9 @@ struct BlockDev {
   int ntx, nty;  // tiles along ξ and η for the active tile shape
   int64_t ps;
   const double* metrics;
+  // metric pairs as the stage kernel reads them: (J·ξx, J·ξy) of point o at
+  // doubles 2o, 2o + 1, (J·ηx, J·ηy) at 2ps + 2o, 2ps + 2o + 1
+  const double* mpair;
   const double* xc;  // ξ/η grid coordinates x(η, ξ), y(η, ξ)
   const double* yc;
   double* buf[4];  // BufRole

@@ -31,6 +31,9 @@ int stage_threads(int shape);
 // Pointwise metrics from the four derivative planes (grid.rs:110-131).
 hipError_t launch_metrics_finish(const double* dx_dxi, const double* dx_deta, const double* dy_dxi, const double* dy_deta,
                                  double* metrics, int64_t ps, int64_t ld, int ny, int nx, unsigned* err, hipStream_t stream);
+// Metric pairs (BlockDev::mpair, 4·ps doubles) from the 5 metric planes.
+hipError_t launch_metric_pairs(const double* metrics, int64_t ps, int64_t ld, int ny, int nx, double* pairs,
+                               hipStream_t stream);
 // Vortex initial state on (ny, nx) coordinates into 4 planes.
 hipError_t launch_vortex(const VortexDev& vp, double gamma, double t, const double* xc, const double* yc, double* out,
                          int64_t ps, int64_t ld, int ny, int nx, unsigned* err, hipStream_t stream);

@@ -98,6 +98,27 @@ hipError_t launch_metrics_finish(const double* dx_dxi, const double* dx_deta, co
   return hipGetLastError();
 }
 
+// Metric pairs for the stage kernel (BlockDev::mpair) from the 5 metric
+// planes: (J·ξx, J·ξy) and (J·ηx, J·ηy) of a point side by side.
+__global__ __launch_bounds__(256) void metric_pairs_kernel(const double* __restrict__ met, int64_t ps, int64_t ld, int ny,
+                                                           int nx, double* __restrict__ pairs) {
+  const int x = blockIdx.x * 64 + (threadIdx.x & 63);
+  const int y = blockIdx.y * 4 + (threadIdx.x >> 6);
+  if (x >= nx || y >= ny) return;
+  const int64_t o = (int64_t)y * ld + x;
+  pairs[2 * o] = met[ps + o];
+  pairs[2 * o + 1] = met[2 * ps + o];
+  pairs[2 * ps + 2 * o] = met[3 * ps + o];
+  pairs[2 * ps + 2 * o + 1] = met[4 * ps + o];
+}
+
+hipError_t launch_metric_pairs(const double* metrics, int64_t ps, int64_t ld, int ny, int nx, double* pairs,
+                               hipStream_t stream) {
+  dim3 grid((nx + 63) / 64, (ny + 3) / 4);
+  metric_pairs_kernel<<<grid, 256, 0, stream>>>(metrics, ps, ld, ny, nx, pairs);
+  return hipGetLastError();
+}
+
 // ---------------------------------------------------------------------------
 // Vortex initial state (Field::vortex, lib.rs:319-328).
 __global__ __launch_bounds__(256) void vortex_kernel(const VortexDev vp, double gamma, double t, const double* __restrict__ xc,

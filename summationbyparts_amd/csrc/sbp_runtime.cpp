@@ -172,6 +172,7 @@ struct HostBlock {
   int nx, ny, op_eta, op_xi;
   int64_t ld, ps;
   double* d_metrics = nullptr;
+  double* d_mpair = nullptr;  // metric pairs of the stage kernel (BlockDev::mpair)
   double* d_x = nullptr;
   double* d_y = nullptr;
   double* d_buf[4] = {nullptr, nullptr, nullptr, nullptr};
@@ -263,6 +264,7 @@ BlockDev make_dev(const sbp_system_s* s, int i, int shape) {
   b.ntx = ntiles_along(h.nx, ts.tx);
   b.nty = ntiles_along(h.ny, ts.ty);
   b.metrics = h.d_metrics;
+  b.mpair = h.d_mpair;
   b.xc = h.d_x;
   b.yc = h.d_y;
   for (int r = 0; r < 4; ++r) b.buf[r] = h.d_buf[r];
@@ -903,6 +905,15 @@ int sbp_rhs(sbp_context ctx, int kind, int op_eta, int op_xi, int ny, int nx, do
     for (int tx = 0; tx < b.ntx; ++tx) tiles.push_back({0, (int16_t)tx, (int16_t)ty});
   hipStream_t st = ctx->stream;
   SBP_TRY_HIP(hipStreamSynchronize(st));
+  // the stage kernel reads the metric pairs; build them from the caller's planes
+  double* pairs = nullptr;
+  SBP_TRY_HIP(hipMalloc((void**)&pairs, sizeof(double) * 4 * b.ps));
+  struct FreeOnExit {
+    double* p;
+    ~FreeOnExit() { hipFree(p); }
+  } free_pairs{pairs};
+  SBP_TRY_HIP(launch_metric_pairs(metrics, b.ps, b.ld, ny, nx, pairs, st));
+  b.mpair = pairs;
   if (ctx->tmptiles_cap < (int)tiles.size()) {
     if (ctx->d_tmptiles) hipFree(ctx->d_tmptiles);
     ctx->d_tmptiles = nullptr;
@@ -958,6 +969,7 @@ int sbp_system_create(sbp_context ctx, int nblocks, const int* ny, const int* nx
     HostBlock& h = s->blocks[i];
     const size_t plane = sizeof(double) * (size_t)h.ps;
     if ((rc = from_hip(hipMalloc((void**)&h.d_metrics, 5 * plane))) != SBP_OK) break;
+    if ((rc = from_hip(hipMalloc((void**)&h.d_mpair, 4 * plane))) != SBP_OK) break;
     if ((rc = from_hip(hipMalloc((void**)&h.d_x, plane))) != SBP_OK) break;
     if ((rc = from_hip(hipMalloc((void**)&h.d_y, plane))) != SBP_OK) break;
     for (int r = 0; r < 4 && rc == SBP_OK; ++r) rc = from_hip(hipMalloc((void**)&h.d_buf[r], 4 * plane));
@@ -965,6 +977,7 @@ int sbp_system_create(sbp_context ctx, int nblocks, const int* ny, const int* nx
     // zero everything once so that pitch padding never holds garbage
     for (int r = 0; r < 4 && rc == SBP_OK; ++r) rc = from_hip(hipMemsetAsync(h.d_buf[r], 0, 4 * plane, st));
     if (rc == SBP_OK) rc = from_hip(hipMemsetAsync(h.d_metrics, 0, 5 * plane, st));
+    if (rc == SBP_OK) rc = from_hip(hipMemsetAsync(h.d_mpair, 0, 4 * plane, st));
     if (rc == SBP_OK)
       rc = from_hip(hipMemcpy2DAsync(h.d_x, h.ld * 8, x[i], (size_t)h.nx * 8, (size_t)h.nx * 8, h.ny, hipMemcpyHostToDevice, st));
     if (rc == SBP_OK)
@@ -978,6 +991,7 @@ int sbp_system_create(sbp_context ctx, int nblocks, const int* ny, const int* nx
     if (rc == SBP_OK)
       rc = from_hip(launch_metrics_finish(t, t + h.ps, t + 2 * h.ps, t + 3 * h.ps, h.d_metrics, h.ps, h.ld, h.ny, h.nx, ctx->d_err,
                                           st));
+    if (rc == SBP_OK) rc = from_hip(launch_metric_pairs(h.d_metrics, h.ps, h.ld, h.ny, h.nx, h.d_mpair, st));
   }
   if (rc != SBP_OK) return fail(rc);
   if ((rc = from_hip(hipMalloc(&s->d_time, sizeof(double) * 2))) != SBP_OK) return fail(rc);
@@ -1003,6 +1017,7 @@ int sbp_system_destroy(sbp_system s) {
   destroy_graph(s);
   for (auto& h : s->blocks) {
     hipFree(h.d_metrics);
+    hipFree(h.d_mpair);
     hipFree(h.d_x);
     hipFree(h.d_y);
     for (int r = 0; r < 4; ++r) hipFree(h.d_buf[r]);

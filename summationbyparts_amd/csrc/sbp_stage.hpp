@@ -152,6 +152,9 @@ __global__ __launch_bounds__((TX / RX) * (TY / RY)) void stage_kernel(const Stag
   const double* __restrict__ in = mixs ? yn_buf(B, A) : stage_input(A, B, mode);
   const f4* __restrict__ din = (MIX && mode >= MODE_S1) ? delta_in(B, mode) : nullptr;
   const double* __restrict__ met = B.metrics;
+  // metric pairs: one 16-byte load per region point and phase
+  const d2* __restrict__ mxi = reinterpret_cast<const d2*>(B.mpair);
+  const d2* __restrict__ meta = reinterpret_cast<const d2*>(B.mpair + 2 * B.ps);
   const double gamma = A.gamma;
   const double idx_xi = B.idx_xi, idx_eta = B.idx_eta;
   unsigned err = 0;
@@ -249,12 +252,14 @@ __global__ __launch_bounds__((TX / RX) * (TY / RY)) void stage_kernel(const Stag
         const bool ok = i0 + j < ITXE && xi_pt(i0 + j, r, c);
         const uint32_t o = ok ? (uint32_t)((y0 + r) * ld + rx0 + c) : (uint32_t)(y0 * ld + rx0);
         load_q(o, v[j]);
-        v[j][4] = ldg(met + ps, o);
-        v[j][5] = ldg(met + 2 * ps, o);
+        const d2 mx = ldg(mxi, o);
+        v[j][4] = mx.x;
+        v[j][5] = mx.y;
         // the own points' η metrics are issued with them (used by the η phase)
         if (OWNQ && i0 + j < RY) {
-          ownm[OWNQ ? i0 + j : 0][0] = ldg(met + 3 * ps, o);
-          ownm[OWNQ ? i0 + j : 0][1] = ldg(met + 4 * ps, o);
+          const d2 me = ldg(meta, o);
+          ownm[OWNQ ? i0 + j : 0][0] = me.x;
+          ownm[OWNQ ? i0 + j : 0][1] = me.y;
         }
       }
       if constexpr (OWNQ) {
@@ -449,8 +454,9 @@ __global__ __launch_bounds__((TX / RX) * (TY / RY)) void stage_kernel(const Stag
           v[j][5] = ownm[OWNQ ? i0 + j : 0][1];
         } else {
           load_q(o, v[j]);
-          v[j][4] = ldg(met + 3 * ps, o);
-          v[j][5] = ldg(met + 4 * ps, o);
+          const d2 me = ldg(meta, o);
+          v[j][4] = me.x;
+          v[j][5] = me.y;
         }
       }
 #pragma unroll

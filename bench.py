@@ -52,7 +52,9 @@ BYTES_PER_POINT_STEP = {"fp64": 544.0, "mixed": 416.0}
 # accepted step (64 B): 1216 B per point per accepted step.
 BYTES_PER_POINT_ADAPTIVE_STEP = 1216.0
 REFERENCE_SCHEDULE_BYTES = 672.0
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "stage_kernel_upwind9_4096_traffic.json")
+# HBM bytes per stage launch from rocprofv3 counters (bench.py --counters)
+TRAFFIC_FILES = {"fp64": os.path.join(ROOT, "profiles", "stage_kernel_upwind9_4096_traffic.json"),
+                 "mixed": os.path.join(ROOT, "profiles", "stage_kernel_upwind9_4096_mixed_traffic.json")}
 
 
 def _counter_rows(path, counter):
@@ -466,12 +468,16 @@ def main():
     # HBM bytes per launch measured with rocprofv3 counters for this
     # workload and tile shape, if recorded
     traffic = None
-    if os.path.exists(TRAFFIC_FILE) and args.workload == "c2":
-        with open(TRAFFIC_FILE) as f:
+    tfile = TRAFFIC_FILES[args.precision]
+    if os.path.exists(tfile) and args.workload == "c2":
+        with open(tfile) as f:
             tr = json.load(f)
+        # the mixed form runs shapes 0 and 1 as the same kernel (it always
+        # caches α·q in registers)
+        ti = 1 if (args.precision == "mixed" and tile_index == 0) else tile_index
         if tr.get("n") == args.n and tr.get("precision", "fp64") == args.precision:
             for e in tr.get("tiles", []):
-                if e.get("tile_index") == tile_index:
+                if e.get("tile_index") == ti:
                     traffic = e.get("hbm_bytes_per_launch")
     cfg = {"workload": w.desc, **w.cfg, "precision": args.precision, "tile": list(tile),
            "tile_index": tile_index, "event_ms_per_step": ms_timed / args.steps, "stage_launch_ms": ms_stage}

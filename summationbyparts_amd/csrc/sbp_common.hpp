@@ -397,6 +397,36 @@ __device__ __forceinline__ void vortex_point(const VortexDev& vp, double gamma, 
 
 // FP64 stage inputs: y_n, Y2 (BUF_YA), Y3 (BUF_YB), Y4 (BUF_ACC); the
 // right-hand-side mode reads rhs_in or the selected work set.
+// Y2, Y3, Y4 (and W over Y2) are internal to a step and stored point-major:
+// the four components of point o at doubles 4o .. 4o + 3, two 16-byte loads
+// per point; y_n and every buffer a caller sees keep the plane layout.
+__device__ __forceinline__ bool stage_input_pm(const StageArgs& A, int mode) { return !A.mixed && mode >= MODE_S1; }
+__device__ __forceinline__ void load_pm(const double* base, uint32_t o, double q[4]) {
+  const d2* p = reinterpret_cast<const d2*>(base);
+  const d2 a = ldg(p, 2 * o), b = ldg(p, 2 * o + 1);
+  q[0] = a.x;
+  q[1] = a.y;
+  q[2] = b.x;
+  q[3] = b.y;
+}
+__device__ __forceinline__ void load_pm_nt(const double* base, uint32_t o, double q[4]) {
+  const d2* p = reinterpret_cast<const d2*>(base);
+  const d2 a = ldg_nt(p, 2 * o), b = ldg_nt(p, 2 * o + 1);
+  q[0] = a.x;
+  q[1] = a.y;
+  q[2] = b.x;
+  q[3] = b.y;
+}
+__device__ __forceinline__ void store_pm_nt(double* base, uint32_t o, double q0, double q1, double q2, double q3) {
+  d2* p = reinterpret_cast<d2*>(base);
+  d2 a, b;
+  a.x = q0;
+  a.y = q1;
+  b.x = q2;
+  b.y = q3;
+  stg_nt(p, 2 * o, a);
+  stg_nt(p, 2 * o + 1, b);
+}
 __device__ __forceinline__ const double* stage_input(const StageArgs& A, const BlockDev& B, int mode) {
   switch (mode) {
     case MODE_S0: return B.buf[BUF_YN];
@@ -439,6 +469,8 @@ __device__ __forceinline__ void load_stage_state(const StageArgs& A, const Block
 #pragma unroll
       for (int c = 0; c < 4; ++c) q[c] += (double)d[c];
     }
+  } else if (stage_input_pm(A, mode)) {
+    load_pm(stage_input(A, B, mode), o, q);
   } else {
     const double* in = stage_input(A, B, mode);
 #pragma unroll

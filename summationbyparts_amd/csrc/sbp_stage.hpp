@@ -178,7 +178,12 @@ __global__ __launch_bounds__((TX / RX) * (TY / RY)) void stage_kernel(const Stag
     for (int s = 0; s < RY; ++s) any = any || (xin[i] && yin[s]);
 
   // state of the stage input at offset o (FP64; the mixed form adds Δ)
+  const bool pm = !MIX && stage_input_pm(A, mode);  // point-major FP64 stage input
   auto load_q = [&](uint32_t o, double q[4]) {
+    if (!MIX && pm) {
+      load_pm(in, o, q);
+      return;
+    }
 #pragma unroll
     for (int c = 0; c < 4; ++c) q[c] = ldg(in + c * ps, o);
     if constexpr (MIX) {
@@ -340,9 +345,10 @@ __global__ __launch_bounds__((TX / RX) * (TY / RY)) void stage_kernel(const Stag
 #pragma unroll
               for (int q = 0; q < 4; ++q) lds[q * NP + l] = cache[it][q];
             } else {
-              const uint32_t o = (uint32_t)((y0 + r) * ld + rx0 + c);
+              double qq[4];
+              load_q((uint32_t)((y0 + r) * ld + rx0 + c), qq);
 #pragma unroll
-              for (int q = 0; q < 4; ++q) lds[q * NP + l] = cache[it][0] * ldg(in + q * ps, o);
+              for (int q = 0; q < 4; ++q) lds[q * NP + l] = cache[it][0] * qq[q];
             }
           }
         }
@@ -381,15 +387,13 @@ __global__ __launch_bounds__((TX / RX) * (TY / RY)) void stage_kernel(const Stag
 #pragma unroll
             for (int t = 0; t < 3; ++t) fd[MIX ? s : 0][MIX ? i : 0][t] = ldg_nt(delta_set(B, t), o);
           }
+        } else if (mode == MODE_S3) {
+          load_pm_nt(B.buf[BUF_YA], o, fa[s][i]);  // W, point-major in the Y2 buffer
         } else if (mode != MODE_S0) {
-          // S1, S2: fa = y_n; S2 also fb = Y2; S3: fa = W (in the Y2 buffer)
-          const double* pa = B.buf[mode == MODE_S3 ? BUF_YA : BUF_YN];
+          // S1, S2: fa = y_n (planes); S2 also fb = Y2 (point-major)
 #pragma unroll
-          for (int q = 0; q < 4; ++q) fa[s][i][q] = ldg_nt(pa + q * ps, o);
-          if (mode == MODE_S2) {
-#pragma unroll
-            for (int q = 0; q < 4; ++q) fb[s][i][q] = ldg_nt((const double*)B.buf[BUF_YA] + q * ps, o);
-          }
+          for (int q = 0; q < 4; ++q) fa[s][i][q] = ldg_nt((const double*)B.buf[BUF_YN] + q * ps, o);
+          if (mode == MODE_S2) load_pm_nt(B.buf[BUF_YA], o, fb[s][i]);
         }
       }
   };
@@ -526,9 +530,10 @@ __global__ __launch_bounds__((TX / RX) * (TY / RY)) void stage_kernel(const Stag
 #pragma unroll
               for (int q = 0; q < 4; ++q) lds[q * NP + l] = cache[it][q];
             } else {
-              const uint32_t o = (uint32_t)((ry0 + r) * ld + x0 + c);
+              double qq[4];
+              load_q((uint32_t)((ry0 + r) * ld + x0 + c), qq);
 #pragma unroll
-              for (int q = 0; q < 4; ++q) lds[q * NP + l] = cache[it][0] * ldg(in + q * ps, o);
+              for (int q = 0; q < 4; ++q) lds[q * NP + l] = cache[it][0] * qq[q];
             }
           }
         }
@@ -652,18 +657,20 @@ __global__ __launch_bounds__((TX / RX) * (TY / RY)) void stage_kernel(const Stag
       // accumulator is kept between stages; the rounding differs from the
       // reference's running sum by a few ulps of y.
       const double* qs = ownq[OWNQ ? s : 0];  // the stage input here
+      // Y2, Y3, Y4 and W point-major, y_{n+1} in planes
+      const double* fs = fa[s][i];
+      const double* fw = fb[s][i];
       if (mode == MODE_S0) {
-#pragma unroll
-        for (int q = 0; q < 4; ++q) stg_nt(B.buf[BUF_YA] + q * ps, o, qs[q] + snext * k[q]);
+        store_pm_nt(B.buf[BUF_YA], o, qs[0] + snext * k[0], qs[1] + snext * k[1], qs[2] + snext * k[2],
+                    qs[3] + snext * k[3]);
       } else if (mode == MODE_S1) {
-#pragma unroll
-        for (int q = 0; q < 4; ++q) stg_nt(B.buf[BUF_YB] + q * ps, o, fa[s][i][q] + snext * k[q]);
+        store_pm_nt(B.buf[BUF_YB], o, fs[0] + snext * k[0], fs[1] + snext * k[1], fs[2] + snext * k[2],
+                    fs[3] + snext * k[3]);
       } else if (mode == MODE_S2) {
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-          stg_nt(B.buf[BUF_ACC] + q * ps, o, fa[s][i][q] + snext * k[q]);
-          stg_nt(B.buf[BUF_YA] + q * ps, o, fma_t(2.0, qs[q], fb[s][i][q]) - fa[s][i][q]);
-        }
+        store_pm_nt(B.buf[BUF_ACC], o, fs[0] + snext * k[0], fs[1] + snext * k[1], fs[2] + snext * k[2],
+                    fs[3] + snext * k[3]);
+        store_pm_nt(B.buf[BUF_YA], o, fma_t(2.0, qs[0], fw[0]) - fs[0], fma_t(2.0, qs[1], fw[1]) - fs[1],
+                    fma_t(2.0, qs[2], fw[2]) - fs[2], fma_t(2.0, qs[3], fw[3]) - fs[3]);
       } else {
 #pragma unroll
         for (int q = 0; q < 4; ++q) stg_nt(B.buf[BUF_YN] + q * ps, o, (fa[s][i][q] + qs[q]) * (1.0 / 3.0) + sacc * k[q]);

@@ -179,7 +179,8 @@ __global__ __launch_bounds__((TX / RX) * (TY / RY)) void stage_kernel(const Stag
 
   // state of the stage input at offset o (FP64; the mixed form adds Δ)
   const bool pm = !MIX && stage_input_pm(A, mode);  // point-major FP64 stage input
-  auto load_q = [&](uint32_t o, double q[4]) {
+  // (dk: where to keep the mixed form's increment Δ of the point as well)
+  auto load_q = [&](uint32_t o, double q[4], f4* dk = nullptr) {
     if (!MIX && pm) {
       load_pm(in, o, q);
       return;
@@ -189,6 +190,7 @@ __global__ __launch_bounds__((TX / RX) * (TY / RY)) void stage_kernel(const Stag
     if constexpr (MIX) {
       if (din) {
         const f4 d = ldg(din, o);
+        if (dk) *dk = d;
 #pragma unroll
         for (int c = 0; c < 4; ++c) q[c] += (double)d[c];
       }
@@ -208,6 +210,9 @@ __global__ __launch_bounds__((TX / RX) * (TY / RY)) void stage_kernel(const Stag
   // stage input at the own output points (OWNQ): loaded in the ξ phase, used
   // again by the η phase (which loads only J·ηx, J·ηy there) and the finish
   double ownq[OWNQ ? RY : 1][4];
+  // mixed form: the increment Δ_s of the stage input at the own points (the
+  // last stage's sum needs Δ_3 apart from y_n + Δ_3)
+  f4 ownd[MIX ? RY : 1];
 
   // ------------------------------------------------------------ ξ-phase
   {
@@ -256,7 +261,7 @@ __global__ __launch_bounds__((TX / RX) * (TY / RY)) void stage_kernel(const Stag
         int r, c;
         const bool ok = i0 + j < ITXE && xi_pt(i0 + j, r, c);
         const uint32_t o = ok ? (uint32_t)((y0 + r) * ld + rx0 + c) : (uint32_t)(y0 * ld + rx0);
-        load_q(o, v[j]);
+        load_q(o, v[j], (MIX && i0 + j < RY) ? &ownd[MIX && i0 + j < RY ? i0 + j : 0] : nullptr);
         const d2 mx = ldg(mxi, o);
         v[j][4] = mx.x;
         v[j][5] = mx.y;
@@ -364,10 +369,10 @@ __global__ __launch_bounds__((TX / RX) * (TY / RY)) void stage_kernel(const Stag
   // Finish operands at the output points, all loaded before any store.
   // FP64 (see the finish), with q the stage input held from the η phase:
   //   S0 q = y_n;  S1 fa = y_n, q = Y2;  S2 fa = y_n, fb = Y2, q = Y3;  S3 fa = W, q = Y4
-  // mixed: S3 fa = y_n, fd = Δ_1, Δ_2, Δ_3 (S0-S2 need no operand)
+  // mixed: S3 fd = Δ_1, Δ_2 (S0-S2 need no operand)
   double fJ[RY][RX], fa[RY][RX][4], fb[RY][RX][4];
-  f4 fd[MIX ? RY : 1][MIX ? RX : 1][3];
-  static_assert(MIX || OWNQ, "the FP64 stage combination takes the stage input from the η phase");
+  f4 fd[MIX ? RY : 1][MIX ? RX : 1][2];
+  static_assert(OWNQ, "the stage combination takes the stage input from the η phase");
   auto load_finish = [&]() {
 #pragma unroll
     for (int s = 0; s < RY; ++s)
@@ -383,9 +388,7 @@ __global__ __launch_bounds__((TX / RX) * (TY / RY)) void stage_kernel(const Stag
         if constexpr (MIX) {
           if (mode == MODE_S3) {
 #pragma unroll
-            for (int q = 0; q < 4; ++q) fa[s][i][q] = ldg_nt(in + q * ps, o);
-#pragma unroll
-            for (int t = 0; t < 3; ++t) fd[MIX ? s : 0][MIX ? i : 0][t] = ldg_nt(delta_set(B, t), o);
+            for (int t = 0; t < 2; ++t) fd[MIX ? s : 0][MIX ? i : 0][t] = ldg_nt(delta_set(B, t), o);
           }
         } else if (mode == MODE_S3) {
           load_pm_nt(B.buf[BUF_YA], o, fa[s][i]);  // W, point-major in the Y2 buffer
@@ -630,15 +633,19 @@ __global__ __launch_bounds__((TX / RX) * (TY / RY)) void stage_kernel(const Stag
       // last stage sums the increments in FP64,
       // y_{n+1} = y_n + (Δ_1 + 2Δ_2 + Δ_3)/3 + dt/6·k_4, the classical RK4
       // sum with dt·k_1 = 2Δ_1, dt·k_2 = 2Δ_2, dt·k_3 = Δ_3 (the increments
-      // the stage inputs used), written into the other FP64 buffer (the
-      // roles swap after the step).
+      // the stage inputs used), formed from the stage input y_n + Δ_3 held
+      // since the ξ phase as (y_n + Δ_3) + (Δ_1 + 2Δ_2 - 2Δ_3)/3 + dt/6·k_4
+      // and written into the other FP64 buffer (the roles swap after the
+      // step).
       if (mode == MODE_S3) {
         double* yo = acc_buf(B, A);
-        const f4(&d)[3] = fd[MIX ? s : 0][MIX ? i : 0];
+        const f4(&d)[2] = fd[MIX ? s : 0][MIX ? i : 0];
+        const f4 d3 = ownd[MIX ? s : 0];
+        const double* qs = ownq[OWNQ ? s : 0];
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
-          const double inc = ((double)d[0][q] + 2.0 * (double)d[1][q] + (double)d[2][q]) * (1.0 / 3.0);
-          stg_nt(yo + q * ps, o, fa[s][i][q] + (inc + sacc * k[q]));
+          const double inc = ((double)d[0][q] + 2.0 * ((double)d[1][q] - (double)d3[q])) * (1.0 / 3.0);
+          stg_nt(yo + q * ps, o, qs[q] + (inc + sacc * k[q]));
         }
       } else {
         f4 dv;

@@ -70,7 +70,7 @@ typedef enum { SBP_INTERP4 = 0, SBP_INTERP8 = 1, SBP_INTERP9 = 2, SBP_INTERP9H2 
 typedef enum { SBP_RHS_AUTO = 0, SBP_RHS_TRAD = 1, SBP_RHS_UPWIND = 2 } sbp_rhs_kind;
 
 /* FP64: the drop-in. MIXED: FP32 flux/stencil/dissipation arithmetic with
- * FP64 state, stage sums and metrics. */
+ * FP64 state, metrics and RK4 sum (the stage increments are stored in FP32). */
 typedef enum { SBP_FP64 = 0, SBP_MIXED = 1 } sbp_precision;
 
 /* euler/src/vortex.rs:3-21 (Vortice, VortexParameters; at most 5 vortices) */

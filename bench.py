@@ -473,7 +473,7 @@ def main():
         with open(tfile) as f:
             tr = json.load(f)
         # the mixed form runs shapes 0 and 1 as the same kernel (it always
-        # caches α·q in registers)
+        # keeps flux and α·q together in LDS)
         ti = 1 if (args.precision == "mixed" and tile_index == 0) else tile_index
         if tr.get("n") == args.n and tr.get("precision", "fp64") == args.precision:
             for e in tr.get("tiles", []):

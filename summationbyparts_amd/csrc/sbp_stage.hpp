@@ -114,13 +114,15 @@ __device__ __forceinline__ void col_window(const T* p, int stride, T out[RS][RX]
 // region points in registers between the flux and dissipation passes; VAR 1
 // keeps only α and re-reads q (cache hits); VAR 2 stores flux and α·q
 // together (8 values per point in LDS, one pass per direction). The mixed
-// form uses VAR 0. (VAR 1 at 16x24 and 16x16 fits 5 waves per SIMD, at most
+// form uses VAR 2: its FP32 values fit both terms in the LDS the FP64 form
+// needs for one, and it saves two barriers per phase and the register cache
+// of α·q. (VAR 1 at 16x24 and 16x16 fits 5 waves per SIMD, at most
 // 96 VGPRs, but its re-reads cost more than the occupancy gains: 6.2 and 5.8
 // against 7.2 Gpoint-steps/s at 4096².)
 // Threads own RX adjacent ξ-columns and RY consecutive η-rows of the tile.
 template <class OX, class OY, bool UP, int TX, int TY, int RX, int RY, int VAR, bool MIX = false, bool EDGE = false>
 __global__ __launch_bounds__((TX / RX) * (TY / RY)) void stage_kernel(const StageArgs A, const int mode) {
-  static_assert(!MIX || VAR == 0, "the mixed form caches α·q in registers");
+  static_assert(!MIX || VAR == 2, "the mixed form keeps flux and α·q together in LDS");
   using CT = std::conditional_t<MIX, float, double>;  // face terms, LDS, stencil sums
   using G = StageGeom<OX, OY, TX, TY, RX, RY, VAR, sizeof(CT)>;
   constexpr int NT = G::NT, HX = G::HX, HY = G::HY, WX = G::WX, NP = G::NP;

@@ -19,10 +19,10 @@ namespace sbp {
 
 // Kernel configurations (tile TX x TY, RY points per thread, caching
 // variant); the list matches kTileShapes in sbp_misc.hip.
-// The mixed form always caches α·q in registers (VAR 0).
+// The mixed form always keeps flux and α·q together in LDS (VAR 2).
 template <class OX, class OY, bool UP, bool MIX, bool EDGE, int TX, int TY, int RX, int RY, int VAR>
 static void launch_one(int mode, const StageArgs& args, int ntiles, hipStream_t stream) {
-  stage_kernel<OX, OY, UP, TX, TY, RX, RY, (MIX ? 0 : VAR), MIX, EDGE>
+  stage_kernel<OX, OY, UP, TX, TY, RX, RY, (MIX ? 2 : VAR), MIX, EDGE>
       <<<ntiles, (TX / RX) * (TY / RY), 0, stream>>>(args, mode);
 }
 

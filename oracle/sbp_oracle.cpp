@@ -1,8 +1,9 @@
 // SummationByParts Euler RK4 oracle — TEST INFRASTRUCTURE ONLY.
 //
-// A plain-C++ CPU restatement of the reference's Euler time-step path. It is
-// the checker for the HIP library and the CPU baseline timed beside it; only
-// tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may load it.
+// A plain-C++ CPU restatement of the reference's Euler time-step path. The
+// parity tests compare the HIP library against it, and bench.py times it as
+// the CPU baseline; only tests/, __graft_entry__.smoke() and bench.py's
+// cpu_baseline leg may load it.
 // The product (summationbyparts_amd) never links, imports or calls it.
 //
 // Compiled with -ffp-contract=off so every a*b+c is two roundings, exactly

@@ -146,6 +146,15 @@ def check(status, what):
         raise SbpError(status, what)
 
 
+def wait_for_torch():
+    """Wait for the work queued on torch's current stream. The library runs on
+    its context's own stream, so torch tensors handed over by pointer must be
+    finished (filled, copied) before a library call reads or writes them."""
+    import torch
+
+    torch.cuda.current_stream().synchronize()
+
+
 _context = None
 
 

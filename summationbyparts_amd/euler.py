@@ -297,6 +297,7 @@ def rhs(kind, op, k, y, metrics, boundaries):
     oe, ox = _ids(op)
     bp = (ctypes.c_void_p * 4)(*[b.data_ptr() for b in boundaries])
     ctx = _lib.context()
+    _lib.wait_for_torch()
     _lib.check(_lib.lib().sbp_rhs(ctx, kind, oe, ox, ny, nx, ctypes.c_void_p(k.data_ptr()),
                                   ctypes.c_void_p(y.data_ptr()), ctypes.c_void_p(metrics.data_ptr()), bp), "sbp_rhs")
 
@@ -316,6 +317,7 @@ def grid_metrics(op, x, y):
     ny, nx = x.shape
     out = torch.empty((5, ny, nx), dtype=torch.float64, device=x.device)
     oe, ox = _ids(op)
+    _lib.wait_for_torch()
     _lib.check(_lib.lib().sbp_metrics(_lib.context(), oe, ox, ny, nx, ctypes.c_void_p(x.data_ptr()),
                                       ctypes.c_void_p(y.data_ptr()), ctypes.c_void_p(out.data_ptr())), "sbp_metrics")
     return out

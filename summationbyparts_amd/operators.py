@@ -32,6 +32,7 @@ def _device_call(fn, op_id, prev, fut, what):
             raise ValueError(f"{what}: need float64 CUDA tensors with unit column stride")
     ny, nx = prev.shape
     ctx = _lib.context()
+    _lib.wait_for_torch()
     st = fn(ctx, op_id, ctypes.c_void_p(prev.data_ptr()), prev.stride(0), ctypes.c_void_p(fut.data_ptr()),
             fut.stride(0), ny, nx)
     _lib.check(st, what)
@@ -242,6 +243,7 @@ class _Interpolation:
                 if a.dtype != torch.float64 or not a.is_cuda or a.dim() != 1 or a.stride(0) != 1:
                     raise ValueError("interpolation: need contiguous 1-D float64 CUDA tensors")
             ctx = _lib.context()
+            _lib.wait_for_torch()
             _lib.check(_lib.lib().sbp_interpolate(ctx, self.ID, f2c, ctypes.c_void_p(src.data_ptr()), src.numel(),
                                                   ctypes.c_void_p(dst.data_ptr()), dst.numel()), "sbp_interpolate")
             _lib.check(_lib.lib().sbp_context_synchronize(ctx), "sbp_interpolate")

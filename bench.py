@@ -371,7 +371,7 @@ def main():
                     help="c2: one 4096^2 curvilinear block (default); c3: 24-block Grid layout; c5: 256-solve batch; "
                          "interp: 2048^2 block under a 4095^2 block, 2:1 Interpolate interfaces; "
                          "adaptive: advance_adaptive (embedded Bogacki-Shampine) on the 4096^2 block")
-    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--tile", type=int, default=-1, help="force a tile shape index (-1: autotune)")
     ap.add_argument("--calibrate-copy", action="store_true",

@@ -490,16 +490,18 @@ __device__ __forceinline__ uint32_t source_edge_offset(const BlockDev& S, int di
   }
 }
 
-__device__ __forceinline__ void edge_data(const StageArgs& A, const BlockDev& B, int dir, int t, int x, int y, int mode,
-                                          double time, double g[4], unsigned& err) {
+__device__ __forceinline__ void edge_data(const StageArgs& A, const BlockDev& B, int dir, int t, int mode, double g[4]) {
   const int src = B.edge_src[dir];
   if (src == EDGE_THIS || src == EDGE_GRID) {
     const BlockDev& S = src == EDGE_THIS ? B : A.blocks[B.edge_nb[dir]];
     load_stage_state(A, S, mode, source_edge_offset(S, dir, t), g);
   } else if (src == EDGE_VORTEX) {
-    const int64_t o = (int64_t)y * B.ld + x;
-    g[2] = 0.0;
-    vortex_point(B.vortex[dir], A.gamma, time, B.xc[o], B.yc[o], g, err);
+    // evaluated for this stage's time by vortex_edges_kernel before the edge
+    // tiles run (vortex_point on this edge's coordinates)
+    const int len = (dir == SBP_NORTH || dir == SBP_SOUTH) ? B.nx : B.ny;
+    const double* p = B.vedge[dir];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) g[c] = p[c * len + t];
   } else if (src == EDGE_INTERP) {
     // Interpolate (lib.rs:816-828): the neighbour's opposite edge through a
     // 2:1 operator (the host checked the lengths)

@@ -63,10 +63,16 @@ struct BlockDev {
   const double* edge_ptr[4];  // EDGE_EXPLICIT data
   int64_t edge_cs[4];         // component stride of EDGE_EXPLICIT data
   VortexDev vortex[4];        // EDGE_VORTEX parameters
+  double* vedge[4];           // EDGE_VORTEX data at the current stage time: 4 x edge length, component-major
   int edge_iop[4];            // EDGE_INTERP: interpolation operator (InterpId)
   int edge_f2c[4];            // EDGE_INTERP: 1 = fine to coarse
   const int* edge_pieces[4];  // EDGE_MULTIGRID: (block, start, end) triples
   int edge_npieces[4];
+};
+
+// One far-field (EDGE_VORTEX) edge whose boundary data a stage precomputes.
+struct VortexSeg {
+  int32_t block, dir;
 };
 
 struct TileDev {

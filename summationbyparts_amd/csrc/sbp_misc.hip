@@ -119,6 +119,37 @@ hipError_t launch_metric_pairs(const double* metrics, int64_t ps, int64_t ld, in
   return hipGetLastError();
 }
 
+// Far-field (Vortex) boundary data of every vortex edge of a system at the
+// stage's time (t + c_s·dt, times the block's step scale), evaluated once per
+// stage before the edge tiles read it (BlockDev::vedge; boundary_extracts'
+// Vortex case, euler/src/lib.rs:983-998): one y-block of the grid per edge.
+__global__ __launch_bounds__(256) void vortex_edges_kernel(const StageArgs A, const VortexSeg* __restrict__ segs) {
+  const VortexSeg sg = segs[blockIdx.y];
+  const BlockDev& B = A.blocks[sg.block];
+  const int dir = sg.dir;
+  const bool ns = dir == SBP_NORTH || dir == SBP_SOUTH;
+  const int len = ns ? B.nx : B.ny;
+  const int t = blockIdx.x * 256 + threadIdx.x;
+  if (t >= len) return;
+  const int x = ns ? t : (dir == SBP_EAST ? B.nx - 1 : 0);
+  const int y = ns ? (dir == SBP_NORTH ? B.ny - 1 : 0) : t;
+  const int64_t o = (int64_t)y * B.ld + x;
+  const double time = ((A.time ? *A.time : 0.0) + A.time_offset) * B.dt_scale;
+  double g[4] = {0.0, 0.0, 0.0, 0.0};
+  unsigned e = 0;
+  vortex_point(B.vortex[dir], A.gamma, time, B.xc[o], B.yc[o], g, e);
+  if (e) record_error(A.error, e);
+  double* out = B.vedge[dir];
+#pragma unroll
+  for (int c = 0; c < 4; ++c) out[c * len + t] = g[c];
+}
+
+hipError_t launch_vortex_edges(const StageArgs& a, const VortexSeg* segs, int nsegs, int maxlen, hipStream_t stream) {
+  dim3 grid((maxlen + 255) / 256, nsegs);
+  vortex_edges_kernel<<<grid, 256, 0, stream>>>(a, segs);
+  return hipGetLastError();
+}
+
 // ---------------------------------------------------------------------------
 // Vortex initial state (Field::vortex, lib.rs:319-328).
 __global__ __launch_bounds__(256) void vortex_kernel(const VortexDev vp, double gamma, double t, const double* __restrict__ xc,

@@ -204,6 +204,11 @@ struct sbp_system_s {
   BlockDev* d_blocks = nullptr;
   TileDev* d_tiles = nullptr;
   int* d_pieces = nullptr;  // MultiGrid piece lists of all edges
+  // far-field (Vortex) edges: boundary data buffers and the edge list the
+  // per-stage vortex_edges_kernel walks
+  double* d_vedge = nullptr;
+  VortexSeg* d_vsegs = nullptr;
+  int n_vsegs = 0, max_vlen = 0;
   double* d_time = nullptr;      // system time
   double* d_time_rhs = nullptr;  // explicit time for sbp_system_rhs
   unsigned* d_err = nullptr;
@@ -373,6 +378,8 @@ int normalize(sbp_system_s* s) {
 hipError_t launch_group(const sbp_system_s* s, const Group& g, int shape, int mode, const StageArgs& a, hipStream_t st) {
   const sbp_context_s* c = s->ctx;
   hipError_t e;
+  // far-field boundary data of this stage's time, read by the edge tiles
+  if (s->n_vsegs && (e = launch_vortex_edges(a, s->d_vsegs, s->n_vsegs, s->max_vlen, st)) != hipSuccess) return e;
   if ((e = hipEventRecord(c->ev_fork, st)) != hipSuccess) return e;
   if ((e = hipStreamWaitEvent(c->aux, c->ev_fork, 0)) != hipSuccess) return e;
   StageArgs ae = a;
@@ -438,9 +445,34 @@ int rebuild(sbp_system_s* s) {
     if (s->d_blocks) hipFree(s->d_blocks);
     if (s->d_tiles) hipFree(s->d_tiles);
     if (s->d_pieces) hipFree(s->d_pieces);
+    if (s->d_vedge) hipFree(s->d_vedge);
+    if (s->d_vsegs) hipFree(s->d_vsegs);
     s->d_blocks = nullptr;
     s->d_tiles = nullptr;
     s->d_pieces = nullptr;
+    s->d_vedge = nullptr;
+    s->d_vsegs = nullptr;
+    s->n_vsegs = s->max_vlen = 0;
+    // far-field edges: 4 x edge length doubles each in one device array
+    std::vector<VortexSeg> vsegs;
+    std::vector<size_t> voff;
+    size_t vtotal = 0;
+    for (size_t bi = 0; bi < s->blocks.size(); ++bi)
+      for (int d = 0; d < 4; ++d)
+        if (s->blocks[bi].bc_kind[d] == SBP_BC_VORTEX) {
+          const int len = (d == SBP_NORTH || d == SBP_SOUTH) ? s->blocks[bi].nx : s->blocks[bi].ny;
+          vsegs.push_back({(int32_t)bi, d});
+          voff.push_back(vtotal);
+          vtotal += 4 * (size_t)len;
+          s->max_vlen = std::max(s->max_vlen, len);
+        }
+    if (!vsegs.empty()) {
+      SBP_TRY_HIP(hipMalloc(&s->d_vedge, sizeof(double) * vtotal));
+      SBP_TRY_HIP(hipMalloc(&s->d_vsegs, sizeof(VortexSeg) * vsegs.size()));
+      SBP_TRY_HIP(hipMemcpyAsync(s->d_vsegs, vsegs.data(), sizeof(VortexSeg) * vsegs.size(), hipMemcpyHostToDevice, st));
+      for (size_t k = 0; k < vsegs.size(); ++k) devs[vsegs[k].block].vedge[vsegs[k].dir] = s->d_vedge + voff[k];
+      s->n_vsegs = (int)vsegs.size();
+    }
     // MultiGrid piece lists: one device array, each edge pointing into it
     std::vector<int> pieces;
     std::vector<size_t> poff(4 * s->blocks.size(), 0);
@@ -1027,6 +1059,8 @@ int sbp_system_destroy(sbp_system s) {
   if (s->d_blocks) hipFree(s->d_blocks);
   if (s->d_tiles) hipFree(s->d_tiles);
   if (s->d_pieces) hipFree(s->d_pieces);
+  if (s->d_vedge) hipFree(s->d_vedge);
+  if (s->d_vsegs) hipFree(s->d_vsegs);
   if (s->d_time) hipFree(s->d_time);
   if (s->d_err) hipFree(s->d_err);
   if (s->d_partials) hipFree(s->d_partials);

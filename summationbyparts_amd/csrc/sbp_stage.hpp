@@ -149,7 +149,8 @@ __global__ __launch_bounds__((TX / RX) * (TY / RY)) void stage_kernel(const Stag
   int x0, wx0, wx1, y0, wy0, wy1;
   tile_place(T.tx, B.ntx, nx, TX, OX::M, x0, wx0, wx1);
   tile_place(T.ty, B.nty, ny, TY, OY::M, y0, wy0, wy1);
-  // stage input: FP64 planes, plus the FP32 increment planes in the mixed form
+  // stage input: y_n (FP64 planes), a point-major FP64 stage input, or in the
+  // mixed form y_n plus the FP32 increment set (4 floats per point)
   const bool mixs = MIX && mode >= MODE_S0;
   const double* __restrict__ in = mixs ? yn_buf(B, A) : stage_input(A, B, mode);
   const f4* __restrict__ din = (MIX && mode >= MODE_S1) ? delta_in(B, mode) : nullptr;
@@ -584,21 +585,20 @@ __global__ __launch_bounds__((TX / RX) * (TY / RY)) void stage_kernel(const Stag
             meta1 = ldg(met + 4 * ps, o);
           }
           const double dj = ldg(met, o);
-          const double time = ((A.time ? *A.time : 0.0) + A.time_offset) * B.dt_scale;
           if (gy == ny - 1) {
-            edge_data(A, B, SBP_NORTH, gx, gx, gy, mode, time, g, err);
+            edge_data(A, B, SBP_NORTH, gx, mode, g);
             sat_point(sat, q4, g, B.hi_eta, -1.0, 1.0, dj, meta0, meta1, gamma);
           }
           if (gy == 0) {
-            edge_data(A, B, SBP_SOUTH, gx, gx, gy, mode, time, g, err);
+            edge_data(A, B, SBP_SOUTH, gx, mode, g);
             sat_point(sat, q4, g, B.hi_eta, 1.0, -1.0, dj, meta0, meta1, gamma);
           }
           if (gx == nx - 1) {
-            edge_data(A, B, SBP_EAST, gy, gx, gy, mode, time, g, err);
+            edge_data(A, B, SBP_EAST, gy, mode, g);
             sat_point(sat, q4, g, B.hi_xi, -1.0, 1.0, dj, mxi0, mxi1, gamma);
           }
           if (gx == 0) {
-            edge_data(A, B, SBP_WEST, gy, gx, gy, mode, time, g, err);
+            edge_data(A, B, SBP_WEST, gy, mode, g);
             sat_point(sat, q4, g, B.hi_xi, 1.0, -1.0, dj, mxi0, mxi1, gamma);
           }
         }

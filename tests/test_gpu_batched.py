@@ -100,6 +100,37 @@ def test_c3_full_size(gpu):
     assert np.isfinite(e) and e < 5e-2
 
 
+def test_operator_groups_farfield_and_grid(gpu):
+    """Two blocks with different operators (two launch groups, each with its
+    own stage launches) coupled by a Grid interface, far-field vortex data
+    on their other six edges (evaluated once per stage for all far-field
+    edges of the system): 3 steps, every block per step against the oracle."""
+    from summationbyparts_amd import _lib, multigrid, operators
+    ny = 41
+    grids = []
+    for x0, x1, nx in ((-5.0, 0.0, 33), (0.0, 5.0, 45)):
+        x, y = np.linspace(x0, x1, nx), np.linspace(-5.0, 5.0, ny)
+        grids.append((np.broadcast_to(x[None, :], (ny, nx)).copy(), np.broadcast_to(y[:, None], (ny, nx)).copy()))
+    vg, vo = _vortex_params(0.5, 1.0)
+    sys_ = multigrid.System(grids, [operators.Upwind4(), operators.Upwind9()])
+    ref = O.System(grids, [(O.UPWIND4, O.UPWIND4), (O.UPWIND9, O.UPWIND9)])
+    for i in range(2):
+        for d in range(4):
+            sys_.set_boundary(i, d, multigrid.Vortex(vg))
+            ref.set_bc(i, d, O.BC_VORTEX, 0, vo)
+    for i, d, j in ((0, _lib.EAST, 1), (1, _lib.WEST, 0)):
+        sys_.set_boundary(i, d, multigrid.Grid(j))
+        ref.set_bc(i, d, O.BC_GRID, j)
+    sys_.vortex(0.0, vg)
+    ref.vortex(0.0, vo)
+    dt = 0.2 / 44
+    sys_.set_dt(dt)
+    for _ in range(3):
+        ref.advance(dt, 1, 2)
+        sys_.advance(1)
+        assert rel_maxnorm_blocks([sys_.get_state(i) for i in range(2)], [ref.get_state(i) for i in range(2)]) < 1e-12
+
+
 def _c5_system(batch, precision, nblocks=None):
     from summationbyparts_amd import _lib, euler, operators
     sel = batch if nblocks is None else batch[:nblocks]

@@ -465,3 +465,62 @@ def test_rk4_step_is_classical_rk4():
     got = S.get_state(0)
     assert rel_maxnorm(got - q0, expect - q0) < 1e-12
     assert abs(S.time - (t0 + dt)) < 1e-15
+
+
+def _smooth_state(X, Y):
+    """A smooth state whose velocities change sign over the grid (so û and v̂
+    take both signs) with positive density and pressure."""
+    L = np.max(X) - np.min(X)
+    s, c = np.sin(2 * np.pi * X / L), np.cos(2 * np.pi * Y / L)
+    rho = 1.0 + 0.2 * s * c
+    u = 0.8 * np.sin(2 * np.pi * (X + 0.3 * Y) / L)
+    v = 0.6 * np.cos(2 * np.pi * (Y - 0.2 * X) / L)
+    p = 1.0 + 0.1 * np.cos(2 * np.pi * (X + Y) / L)
+    e = p / (GAMMA - 1.0) + 0.5 * rho * (u * u + v * v)
+    return np.array([rho, rho * u, rho * v, e])
+
+
+def _spectral_radius(q, kx, ky):
+    """max |λ| of the normal flux Jacobian kx·∂E/∂q + ky·∂F/∂q at every point
+    of q (4, ny, nx): the Jacobian's columns by complex step, its eigenvalues
+    by LAPACK."""
+    cols = [_flux_jacobian_times(q, np.eye(4)[i][:, None, None], kx, ky) for i in range(4)]
+    A = np.moveaxis(np.stack(cols, axis=1), (0, 1), (-2, -1))  # (ny, nx, 4, 4)
+    return np.max(np.abs(np.linalg.eigvals(A)), axis=-1)
+
+
+@pytest.mark.parametrize("ops", [(O.UPWIND9, O.UPWIND9), (O.UPWIND4, O.UPWIND9), (O.UPWIND9H2, O.UPWIND4)])
+def test_upwind_dissipation_weight_is_spectral_radius(ops):
+    """RHS_upwind - RHS_trad = (DISS_ξ(α_ξ q) + DISS_η(α_η q))/J
+    (euler/src/lib.rs:571-616, :619-674), where the dissipation weight of
+    each direction is the spectral radius of that direction's normal flux
+    Jacobian, J·ξx·∂E/∂q + J·ξy·∂F/∂q and J·ηx·∂E/∂q + J·ηy·∂F/∂q (the
+    eigenvalues û, û, û ± c·|J·∇ξ|), derived here from the textbook fluxes by
+    complex step and an eigenvalue solver, independently of the oracle's
+    formula α = |û| + c·√(J·ξx² + J·ξy²). The state is smooth and non-uniform
+    on a curvilinear grid, with velocities of both signs; the SATs (periodic)
+    are the same in both right-hand sides and cancel. A halved sound-speed
+    term, swapped ξ/η metrics or a dropped absolute value fails."""
+    op_eta, op_xi = ops
+    ny, nx = 34, 30
+    X, Y = curvilinear(nx, ny, amp=0.08)
+    q = _smooth_state(X, Y)
+    S = O.System([(X, Y)], [(op_eta, op_xi)])
+    S.set_state(0, q)
+    S.set_rhs_kind(O.RHS_UPWIND)
+    k_up = S.rhs(0.0)[0]
+    S.set_rhs_kind(O.RHS_TRAD)
+    k_trad = S.rhs(0.0)[0]
+    J, jxx, jxy, jyx, jyy = S.metrics(0)
+    a_xi = _spectral_radius(q, jxx, jxy)
+    a_eta = _spectral_radius(q, jyx, jyy)
+    expect = np.stack([(O.diff(op_xi, O.DISS, O.XI, a_xi * q[c]) + O.diff(op_eta, O.DISS, O.ETA, a_eta * q[c])) / J
+                       for c in range(4)])
+    got = k_up - k_trad
+    scale = np.max(np.abs(expect))
+    assert scale > 1e-3
+    assert np.max(np.abs(got - expect)) < 1e-10 * scale
+    # both signs of the contravariant velocities occur (a dropped |û| matters)
+    uh = jxx * q[1] / q[0] + jxy * q[2] / q[0]
+    vh = jyx * q[1] / q[0] + jyy * q[2] / q[0]
+    assert np.min(uh) < 0 < np.max(uh) and np.min(vh) < 0 < np.max(vh)

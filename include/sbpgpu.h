@@ -131,7 +131,10 @@ int sbp_disseta(sbp_context ctx, int op, const double* prev, int64_t ld_prev, do
 /* InterpolationOperator::fine2coarse (fine2coarse = 1) / coarse2fine
  * (sbp/src/operators.rs:108-113; interpolation.rs:15-50) of a device vector
  * of nin points into nout points: fine = 2 coarse - 1 (2 (coarse - 1) for
- * SBP_INTERP9H2), else SBP_ERR_SHAPE. out is fully overwritten. */
+ * SBP_INTERP9H2), and every tap of the operator must fall inside the nin
+ * input points (short edges make the reference's indexing panic), else
+ * SBP_ERR_SHAPE; the lengths are checked before the handle, so a NULL ctx
+ * validates a coupling without device work. out is fully overwritten. */
 int sbp_interpolate(sbp_context ctx, int interp_op, int fine2coarse, const double* in, int nin, double* out, int nout);
 /* Table shapes {block rows, block cols, diag rows, diag cols, jump0, jump1}
  * and coefficients (block row-major, then diag row-major) of one direction of
@@ -172,7 +175,8 @@ int sbp_system_set_bc(sbp_system sys, int block, int dir, int kind, int neighbou
  * :816-828): the neighbour's opposite edge through a 2:1 interpolation
  * operator, fine to coarse when this block has fewer points along ξ than the
  * neighbour (the reference compares nx for every edge). SBP_ERR_EDGE unless
- * the edge lengths are the operator's coarse/fine pair. */
+ * the edge lengths are the operator's coarse/fine pair and long enough for
+ * every tap of its stencil rows (as sbp_interpolate). */
 int sbp_system_set_bc_interpolate(sbp_system sys, int block, int dir, int neighbour, int interp_op);
 /* BoundaryCharacteristic::MultiGrid(pieces) (euler/src/lib.rs:744, :829-849):
  * npieces triples (block, start, end) of the sources' opposite edges, copied
@@ -230,7 +234,9 @@ int sbp_system_energy(sbp_system sys, double* out);
  * in the reference, the loop time grows by the trial step on every attempt
  * and once more on acceptance, and boundary data sees the loop's stage times
  * (the reference applies This on every edge); the system time is not
- * advanced. FP64 only (SBP_ERR_INVALID in the mixed form). */
+ * advanced. FP64 only (SBP_ERR_INVALID in the mixed form), and every block
+ * must step with the system's dt (SBP_ERR_INVALID when a block has a step
+ * scale other than 1, see sbp_system_set_dt_scale). */
 int sbp_system_advance_adaptive(sbp_system sys, double dt, double* guess_dt, double maxerr);
 /* Sticky device error word of the last advance (0 = none); cleared by reading. */
 int sbp_system_check(sbp_system sys);

@@ -84,9 +84,20 @@ bool interp_def(int op, bool f2c, InterpDef& d) {
 
 // fine2coarse / coarse2fine length asserts (interpolation4.rs:75,85;
 // interpolation9h2.rs:53,64): fine = 2 coarse - 1, or 2 (coarse - 1) on h2 grids
+// and the input indices the loop below reads (interpolation.rs:29-46 index
+// `input` directly, so a short edge panics there): start and end rows read
+// bcols points from either end, the last interior row dcols points from
+// j0 + j1·(its row group)
 bool interp_lengths(const InterpDef& d, bool f2c, int nin, int nout) {
   const int coarse = f2c ? nout : nin, fine = f2c ? nin : nout;
-  return d.h2 ? fine == 2 * (coarse - 1) : fine == 2 * coarse - 1;
+  if (!(d.h2 ? fine == 2 * (coarse - 1) : fine == 2 * coarse - 1)) return false;
+  if (nin <= 0 || nout <= 0 || d.bcols > nin) return false;
+  const int last = nout - d.brows - 1;
+  if (last >= d.brows) {
+    const int index = d.j0 + d.j1 * ((last - d.brows) / d.drows);
+    if (index + d.dcols > nin) return false;
+  }
+  return true;
 }
 
 // interpolation.rs:15-50: output rows below the block height take the block,

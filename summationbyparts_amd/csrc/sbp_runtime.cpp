@@ -836,9 +836,11 @@ int sbp_disseta(sbp_context ctx, int op, const double* prev, int64_t ld_prev, do
 }
 
 int sbp_interpolate(sbp_context ctx, int interp_op, int fine2coarse, const double* in, int nin, double* out, int nout) {
-  if (!ctx || !in || !out || interp_op < 0 || interp_op >= INTERP_COUNT) return SBP_ERR_INVALID;
+  if (interp_op < 0 || interp_op >= INTERP_COUNT) return SBP_ERR_INVALID;
+  // lengths first (a host-only check, also without a context)
   const bool f2c = fine2coarse != 0;
-  if (!interp::lengths_match(interp_op, f2c ? nout : nin, f2c ? nin : nout)) return SBP_ERR_SHAPE;
+  if (!interp::coupling_valid(interp_op, f2c, f2c ? nout : nin, f2c ? nin : nout)) return SBP_ERR_SHAPE;
+  if (!ctx || !in || !out) return SBP_ERR_INVALID;
   SBP_TRY_HIP(hipSetDevice(ctx->device));
   SBP_TRY_HIP(launch_interpolate(interp_op, f2c, in, nin, out, nout, ctx->stream));
   return SBP_OK;
@@ -1100,7 +1102,7 @@ int sbp_system_set_bc_interpolate(sbp_system s, int block, int dir, int neighbou
   // fine to coarse iff this block has fewer points along ξ (lib.rs:818); the
   // operator's length assert then decides (interpolation4.rs:75,85)
   const bool f2c = h.nx < o.nx;
-  if (!interp::lengths_match(interp_op, f2c ? own : src, f2c ? src : own)) return SBP_ERR_EDGE;
+  if (!interp::coupling_valid(interp_op, f2c, f2c ? own : src, f2c ? src : own)) return SBP_ERR_EDGE;
   h.bc_kind[dir] = SBP_BC_INTERPOLATE;
   h.bc_nb[dir] = neighbour;
   h.bc_iop[dir] = interp_op;
@@ -1434,6 +1436,10 @@ int sbp_system_advance_adaptive(sbp_system s, double dt, double* guess_dt, doubl
   if (!s || !guess_dt || !std::isfinite(dt) || !std::isfinite(*guess_dt) || !(*guess_dt > 0.0) || std::isnan(maxerr))
     return SBP_ERR_INVALID;
   if (s->precision != SBP_FP64) return SBP_ERR_INVALID;
+  // one trial step for the whole system: per-block step scales (batched
+  // independent solves) have no meaning here
+  for (auto& h : s->blocks)
+    if (h.dt_scale != 1.0) return SBP_ERR_INVALID;
   double gamma;
   if (!gamma_get(gamma)) return SBP_ERR_GAMMA;
   SBP_TRY_HIP(hipSetDevice(s->ctx->device));

@@ -78,3 +78,21 @@ def test_advance_adaptive_rejects_mixed_and_bad_args(gpu):
     t = euler.System(X, Y, operators.SBP8())
     with pytest.raises(_lib.SbpError):
         t.advance_adaptive(0.1, 0.01, 1.0)
+
+
+def test_advance_adaptive_rejects_block_step_scales(gpu):
+    """One trial step drives the whole system, so a block with its own step
+    scale (batched independent solves) is rejected before any work."""
+    from summationbyparts_amd import _lib, euler, operators
+    X, Y = curvilinear(40, 40)
+    s = euler.BlockSystem([(X, Y), (X, Y)], [operators.Upwind9()] * 2)
+    s.set_rhs_kind(_lib.RHS_UPWIND)
+    s.vortex(0.0, _vp()[0])
+    q0 = s.get_state(1)
+    s.set_dt_scale(1, 0.5)
+    with pytest.raises(_lib.SbpError) as e:
+        s.advance_adaptive_steps(0.01, 0.002, np.inf)
+    assert e.value.status == _lib.ERR_INVALID
+    assert np.array_equal(s.get_state(1), q0)
+    s.set_dt_scale(1, 1.0)
+    s.advance_adaptive_steps(0.01, 0.002, np.inf)

@@ -91,3 +91,54 @@ def test_product_does_not_touch_oracle():
             if f.endswith((".py", ".cpp", ".hpp", ".hip", ".h")) or f == "Makefile":
                 text = open(os.path.join(dirpath, f), errors="ignore").read()
                 assert "oracle" not in text.lower(), os.path.join(dirpath, f)
+
+
+def _taps_inside(block, diag, jump, nin, nout):
+    """Whether every input index the reference's interpolation loop
+    (sbp/src/operators/interpolation.rs:26-47) reads for nout outputs lies in
+    [0, nin): the loop itself, walked row by row."""
+    brows, bcols = block.shape
+    drows, dcols = diag.shape
+    for i in range(nout):
+        if i < brows:
+            idx = range(bcols)
+        elif nout - i <= brows:
+            idx = range(nin - bcols, nin)
+        else:
+            n = i - brows
+            base = jump[0] + jump[1] * (n // drows)
+            idx = range(base, base + dcols)
+        if idx.start < 0 or idx.stop > nin:
+            return False
+    return True
+
+
+@pytest.mark.parametrize("iop", [O.INTERP4, O.INTERP8, O.INTERP9, O.INTERP9H2])
+def test_interpolation_rejects_short_edges(iop):
+    """sbp_interpolate (checked before its handle, so no device is needed)
+    and the oracle reject every length pair whose taps would leave the input,
+    and accept every other pair the operator couples. Short coarse edges of
+    the wide operators (a 23-column block) are the cases that pass the
+    2:1 length relation but not the taps."""
+    from summationbyparts_amd import _lib
+    L = _lib.lib()
+    h2 = iop == O.INTERP9H2
+    rejected = 0
+    for coarse in range(2, 60):
+        fine = 2 * (coarse - 1) if h2 else 2 * coarse - 1
+        for f2c in (True, False):
+            block, diag, jump = O.interp_tables(iop, f2c)
+            nin, nout = (fine, coarse) if f2c else (coarse, fine)
+            ok = _taps_inside(block, diag, jump, nin, nout)
+            st = L.sbp_interpolate(None, iop, int(f2c), None, nin, None, nout)
+            assert st == (_lib.ERR_INVALID if ok else _lib.ERR_SHAPE), (coarse, f2c, st)
+            if ok:
+                out = O.interpolate(iop, f2c, np.linspace(0.0, 1.0, nin), nout)
+                assert out.shape == (nout,)
+            else:
+                rejected += 1
+                with pytest.raises(O.OracleError):
+                    O.interpolate(iop, f2c, np.zeros(nin), nout)
+        # lengths the operator does not couple are rejected either way
+        assert L.sbp_interpolate(None, iop, 1, None, fine + 1, None, coarse) == _lib.ERR_SHAPE
+    assert rejected > 0

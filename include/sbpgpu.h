@@ -152,6 +152,36 @@ int sbp_interp_coefficients(int interp_op, int fine2coarse, double* block, doubl
 int sbp_rhs(sbp_context ctx, int kind, int op_eta, int op_xi, int ny, int nx, double* k, const double* y,
             const double* metrics, const double* const* boundaries);
 
+/* euler::RHS_no_SAT (euler/src/lib.rs:518-568): the right-hand side without
+ * boundary penalties, with the upwind dissipation iff both operators are
+ * upwind (op.upwind().is_some(), sbp/src/operators.rs:137-150). Buffers as
+ * sbp_rhs. */
+int sbp_rhs_no_sat(sbp_context ctx, int op_eta, int op_xi, int ny, int nx, double* k, const double* y, const double* metrics);
+
+/* euler::SAT_characteristics (euler/src/lib.rs:1002-1013): k += the
+ * characteristic SATs of the north, south, east and west edges in that
+ * order (corners get two), SAT_characteristic :1150-1252. k, y (4, ny, nx),
+ * metrics (5, ny, nx) contiguous device buffers; boundaries[dir] as sbp_rhs.
+ * k accumulates (the reference adds into its edge views of k). */
+int sbp_sat_characteristics(sbp_context ctx, int op_eta, int op_xi, int ny, int nx, double* k, const double* y,
+                            const double* metrics, const double* const* boundaries);
+/* One of euler::SAT_FUNCTIONS / SAT_north, SAT_south, SAT_east, SAT_west
+ * (euler/src/lib.rs:1015-1144): k += the SAT of edge `dir` with boundary
+ * data `boundary` (4 x n contiguous device array). */
+int sbp_sat_edge(sbp_context ctx, int dir, int op_eta, int op_xi, int ny, int nx, double* k, const double* y,
+                 const double* metrics, const double* boundary);
+
+/* Field::h2_err (euler/src/lib.rs:377-432) of two (4, ny, nx) contiguous
+ * device fields: sqrt(Σ_c Σ_ij hx_i hy_j (a - b)²) with the operators' H
+ * weights, summed compensated in FP64 on the device; *out is a host double.
+ * Allocates (and frees) a small device buffer of partial sums per call.
+ * SBP_ERR_SHAPE when n < 2M along an axis; at most 2^31 - 1 points. */
+int sbp_h2_err(sbp_context ctx, int op_eta, int op_xi, int ny, int nx, const double* a, const double* b, double* out);
+
+/* sbp::utils::h2linspace (sbp/src/utils.rs:133-140): n >= 2 host values,
+ * start, start + h/2, start + 3h/2, ..., end with h = (end - start)/(n - 2). */
+int sbp_h2linspace(double start, double end, int n, double* out);
+
 /* Grid::metrics (sbp/src/grid.rs:64-66,95-139) on device buffers: x, y (ny, nx)
  * contiguous in, metrics (5, ny, nx) out. */
 int sbp_metrics(sbp_context ctx, int op_eta, int op_xi, int ny, int nx, const double* x, const double* y, double* metrics);
@@ -238,6 +268,18 @@ int sbp_system_energy(sbp_system sys, double* out);
  * must step with the system's dt (SBP_ERR_INVALID when a block has a step
  * scale other than 1, see sbp_system_set_dt_scale). */
 int sbp_system_advance_adaptive(sbp_system sys, double dt, double* guess_dt, double maxerr);
+/* integrate::integrate::<B> (utils/integrate/src/lib.rs:17-24, :176-222) of
+ * nsteps steps of size dt for any explicit Butcher tableau given as host
+ * arrays: nstages s in [1, 9]; A: s x s row-major, stage i > 0 takes
+ * y_n + Σ_{j<i} (A[i*s + j]·dt)·k_j (the reference's A[i-1][j]); B: s
+ * weights; C: s - 1 nodes, stage i > 0 evaluated at t + dt·C[i-1] (the
+ * reference's indexing; NULL when s = 1). Zero coefficients are skipped and
+ * every term is added with two roundings, as the reference's scaled_add.
+ * Each stage is one RHS launch per operator group into a work set (s + 1
+ * sets of 4 planes per block, allocated on first use); the system time
+ * advances by dt per step. FP64 only (SBP_ERR_INVALID in the mixed form). */
+int sbp_system_integrate(sbp_system sys, int nstages, const double* A, const double* B, const double* C, double dt,
+                         int64_t nsteps);
 /* Sticky device error word of the last advance (0 = none); cleared by reading. */
 int sbp_system_check(sbp_system sys);
 

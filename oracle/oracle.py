@@ -84,6 +84,10 @@ def _load():
     lib.orc_system_edge.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_double, dp]
     lib.orc_system_embedded_step.argtypes = [ctypes.c_void_p, dp, ctypes.c_double, dp, ctypes.c_int]
     lib.orc_system_advance_adaptive.argtypes = [ctypes.c_void_p, ctypes.c_double, dp, ctypes.c_double, ctypes.c_int]
+    lib.orc_system_set_sat.argtypes = [ctypes.c_void_p, ctypes.c_int]
+    lib.orc_sat_edge.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, dp, dp, dp, dp]
+    lib.orc_system_integrate.argtypes = [ctypes.c_void_p, ctypes.c_int, dp, dp, dp, ctypes.c_double, ctypes.c_long,
+                                         ctypes.c_int]
     return lib
 
 
@@ -161,6 +165,18 @@ def h2_err(op_eta, op_xi, a, b):
     out = ctypes.c_double()
     _check(lib().orc_h2_err(op_eta, op_xi, _ptr(a), _ptr(b), a.shape[1], a.shape[2], ctypes.byref(out)), "h2_err")
     return out.value
+
+
+def sat_edge(op_eta, op_xi, direction, k, y, metrics, g):
+    """SAT_north / south / east / west (euler/src/lib.rs:1025-1144): returns
+    k + SAT of one edge; k, y (4, ny, nx), metrics (5, ny, nx), g (4, n)."""
+    k = np.array(k, dtype=np.float64, order="C")
+    y = np.ascontiguousarray(y, dtype=np.float64)
+    metrics = np.ascontiguousarray(metrics, dtype=np.float64)
+    g = np.ascontiguousarray(g, dtype=np.float64)
+    _check(lib().orc_sat_edge(op_eta, op_xi, direction, y.shape[1], y.shape[2], _ptr(k), _ptr(y), _ptr(metrics), _ptr(g)),
+           "sat_edge")
+    return k
 
 
 def interpolate(iop, fine2coarse, data, nout):
@@ -258,6 +274,18 @@ class System:
             res.append(out[off:off + n].reshape((4,) + s))
             off += n
         return res
+
+    def set_sat(self, on):
+        """False: the right-hand side is RHS_no_SAT (euler/src/lib.rs:518-568)."""
+        _check(lib().orc_system_set_sat(self.h, int(bool(on))), "set_sat")
+
+    def integrate(self, A, B, C, dt, nsteps, nthreads=1):
+        """nsteps of integrate::<tableau> (integrate/src/lib.rs:176-222):
+        A (s, s) with stage i using A[i, :i], B (s), C (s-1 used)."""
+        A = np.ascontiguousarray(A, dtype=np.float64)
+        B = np.ascontiguousarray(B, dtype=np.float64)
+        C = np.ascontiguousarray(np.concatenate([np.asarray(C, dtype=np.float64), [0.0]]))
+        _check(lib().orc_system_integrate(self.h, len(B), _ptr(A), _ptr(B), _ptr(C), dt, nsteps, nthreads), "integrate")
 
     def advance(self, dt, nsteps, nthreads=1):
         _check(lib().orc_system_advance(self.h, dt, nsteps, nthreads), "advance")

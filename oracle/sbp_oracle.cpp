@@ -26,7 +26,10 @@
 //                  Interpolate / MultiGrid)
 //   interpolate    sbp/src/operators/interpolation.rs:15-50 (2:1 interfaces;
 //                  tables in sbp_interp_tables.h)
-//   integrate      utils/integrate/src/lib.rs:30-35, :176-222 (Rk4)
+//   integrate      utils/integrate/src/lib.rs:30-35, :176-222 (Rk4; any
+//                  explicit tableau given as arrays: system_step_tableau)
+//   rhs_no_sat     euler/src/lib.rs:518-568 (block_rhs with the SATs off)
+//   sat_edge       euler/src/lib.rs:1025-1144 (one SAT function on arrays)
 //   embedded RK    utils/integrate/src/lib.rs:138-151, :176-251
 //                  (BogackiShampine, integrate_embedded_rk)
 //   adaptive       euler/src/lib.rs:149-185 (advance_adaptive)
@@ -468,6 +471,7 @@ struct System {
   std::vector<Work> work;
   double time = 0.0;
   int rhs_kind = RHS_AUTO;
+  bool sat = true;  // false: RHS_no_SAT (lib.rs:518-568)
 };
 
 bool block_is_upwind(const Block& b) { return get_op(b.op_eta)->upwind && get_op(b.op_xi)->upwind; }
@@ -641,6 +645,7 @@ int block_rhs(System& S, size_t bi, const std::vector<std::vector<double>>& fiel
     for (int c = 0; c < 4; ++c)
       for (size_t q = 0; q < n; ++q) kout[c * n + q] = (-w.dE[c * n + q] - w.dF[c * n + q]) / b.detj[q];
   }
+  if (!S.sat) return ST_OK;  // RHS_no_SAT, lib.rs:518-568
   // SAT_characteristics, lib.rs:1002-1013 (north, south, east, west)
   const OpDef* oe = get_op(b.op_eta);
   const OpDef* ox = get_op(b.op_xi);
@@ -787,6 +792,43 @@ int system_step_bs(System& S, double dt, double& time, std::vector<std::vector<d
   fut2 = S.fnow;
   for (int j = 0; j < 4; ++j)
     if (Bs[j] != 0.0) scaled_add(fut2, j, Bs[j] * dt);
+  return ST_OK;
+}
+
+// integrate::<B> for an explicit tableau given as arrays (integrate/src/lib.rs:
+// 17-24, 176-222): ns stages, A row-major ns x ns (stage i > 0 uses A[i][j],
+// j < i, the reference's A[i-1][j]), B (ns), C (ns - 1 used: stage i > 0 at
+// time t + dt·C[i-1]); zero coefficients are skipped, as the reference does.
+int system_step_tableau(System& S, int ns, const double* A, const double* B, const double* C, double dt, int nthreads) {
+  size_t nb = S.blocks.size();
+  std::vector<std::vector<std::vector<double>>> k(ns, S.fnow);
+  auto scaled_add = [&](std::vector<std::vector<double>>& f, int j, double scale) {
+    for (size_t bi = 0; bi < nb; ++bi) {
+      double* y = f[bi].data();
+      const double* kk = k[j][bi].data();
+      size_t m = f[bi].size();
+      for (size_t q = 0; q < m; ++q) y[q] = y[q] + scale * kk[q];
+    }
+  };
+  int st;
+  for (int i = 0; i < ns; ++i) {
+    double simtime = S.time;
+    S.fnext = S.fnow;
+    if (i > 0) {
+      for (int j = 0; j < i; ++j) {
+        const double a = A[i * ns + j];
+        if (a == 0.0) continue;
+        scaled_add(S.fnext, j, a * dt);
+      }
+      simtime = S.time + dt * C[i - 1];
+    }
+    if ((st = system_rhs(S, S.fnext, simtime, k[i], nthreads))) return st;
+  }
+  S.fnext = S.fnow;
+  for (int j = 0; j < ns; ++j)
+    if (B[j] != 0.0) scaled_add(S.fnext, j, B[j] * dt);
+  S.time += dt;
+  std::swap(S.fnow, S.fnext);
   return ST_OK;
 }
 
@@ -1062,6 +1104,53 @@ int orc_system_rhs(void* s, double t, double* kout) {
   for (auto& kk : k) {
     std::memcpy(kout + off, kk.data(), kk.size() * 8);
     off += kk.size();
+  }
+  return ST_OK;
+}
+
+// RHS_no_SAT (lib.rs:518-568) for on = 0, the full RHS otherwise
+int orc_system_set_sat(void* s, int on) {
+  ((System*)s)->sat = on != 0;
+  return ST_OK;
+}
+
+// One SAT function (SAT_north / south / east / west, lib.rs:1025-1144) on
+// caller arrays: k, y (4, ny, nx), metrics (5, ny, nx), g (4, n); k += SAT.
+int orc_sat_edge(int op_eta, int op_xi, int dir, int ny, int nx, double* k, const double* y, const double* met, const double* g) {
+  const OpDef* oe = get_op(op_eta);
+  const OpDef* ox = get_op(op_xi);
+  if (!oe || !ox || dir < 0 || dir > 3) return ST_INVALID;
+  if (!g_gamma_set) return ST_GAMMA;
+  const size_t n = (size_t)nx * ny, nxs = (size_t)nx;
+  const double hi_eta = (oe->h2 ? (double)(ny - 2) : (double)(ny - 1)) / oe->h[0];
+  const double hi_xi = (ox->h2 ? (double)(nx - 2) : (double)(nx - 1)) / ox->h[0];
+  const double *J = met, *jxx = met + n, *jxy = met + 2 * n, *jyx = met + 3 * n, *jyy = met + 4 * n;
+  const size_t rN = (size_t)(ny - 1) * nxs, cE = nxs - 1;
+  switch (dir) {
+    case DIR_NORTH:
+      sat_characteristic(k + rN, n, 1, y + rN, n, 1, g, (size_t)nx, 1, nx, hi_eta, -1.0, 1.0, J + rN, jyx + rN, jyy + rN, 1);
+      break;
+    case DIR_SOUTH:
+      sat_characteristic(k, n, 1, y, n, 1, g, (size_t)nx, 1, nx, hi_eta, 1.0, -1.0, J, jyx, jyy, 1);
+      break;
+    case DIR_EAST:
+      sat_characteristic(k + cE, n, nxs, y + cE, n, nxs, g, (size_t)ny, 1, ny, hi_xi, -1.0, 1.0, J + cE, jxx + cE, jxy + cE,
+                         nxs);
+      break;
+    default:
+      sat_characteristic(k, n, nxs, y, n, nxs, g, (size_t)ny, 1, ny, hi_xi, 1.0, -1.0, J, jxx, jxy, nxs);
+  }
+  return ST_OK;
+}
+
+// nsteps of integrate::<B> for the tableau (ns, A, B, C)
+int orc_system_integrate(void* s, int ns, const double* A, const double* B, const double* C, double dt, long nsteps,
+                         int nthreads) {
+  System* S = (System*)s;
+  if (ns < 1) return ST_INVALID;
+  for (long i = 0; i < nsteps; ++i) {
+    int st = system_step_tableau(*S, ns, A, B, C, dt, nthreads);
+    if (st) return st;
   }
   return ST_OK;
 }

@@ -110,6 +110,13 @@ def _declare(L):
         "sbp_interp_info": ([c_int, c_int, ip], c_int),
         "sbp_interp_coefficients": ([c_int, c_int, dp, dp], c_int),
         "sbp_system_advance_timed": ([c_void_p, c_double, c_int64, dp], c_int),
+        "sbp_rhs_no_sat": ([c_void_p, c_int, c_int, c_int, c_int, c_void_p, c_void_p, c_void_p], c_int),
+        "sbp_sat_characteristics": ([c_void_p, c_int, c_int, c_int, c_int, c_void_p, c_void_p, c_void_p,
+                                     ctypes.POINTER(c_void_p)], c_int),
+        "sbp_sat_edge": ([c_void_p, c_int, c_int, c_int, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p], c_int),
+        "sbp_h2_err": ([c_void_p, c_int, c_int, c_int, c_int, c_void_p, c_void_p, dp], c_int),
+        "sbp_h2linspace": ([c_double, c_double, c_int, dp], c_int),
+        "sbp_system_integrate": ([c_void_p, c_int, dp, dp, dp, c_double, c_int64], c_int),
         "sbp_tile_shape_count": ([], c_int),
         "sbp_tile_shape_get": ([c_int, ip, ip, ip], c_int),
     }
@@ -134,6 +141,8 @@ EXPORTED = [
     "sbp_interp_coefficients",
     "sbp_tile_shape_count",
     "sbp_tile_shape_get",
+    "sbp_rhs_no_sat", "sbp_sat_characteristics", "sbp_sat_edge", "sbp_h2_err", "sbp_h2linspace",
+    "sbp_system_integrate",
 ]
 
 

@@ -6,6 +6,9 @@
 
 namespace sbp {
 
+// Work sets of the Runge-Kutta paths that keep their stage derivatives.
+constexpr int kMaxWorkSets = 10;
+
 // Roles of the four state buffers of a block during one fused RK4 step.
 // FP64 form: y_n and the stage inputs Y2 (YA), Y3 (YB) and Y4 (ACC); stage 3
 // overwrites Y2 with W = Y2 + 2 Y3 - y_n, from which and Y4 the last stage
@@ -26,6 +29,7 @@ enum EdgeSrc : int {
   EDGE_INTERP = SBP_BC_INTERPOLATE,  // neighbour's opposite edge through a 2:1 interpolation operator
   EDGE_MULTIGRID = SBP_BC_MULTIGRID,  // pieces of several neighbours' opposite edges
   EDGE_EXPLICIT = 100,          // caller-provided 4 x n array (operator-level rhs)
+  EDGE_NONE = 101,              // no SAT on this edge (RHS_no_SAT)
 };
 
 // Stage-kernel mode: RHS only, or one of the four fused RK4 stages.
@@ -54,7 +58,10 @@ struct BlockDev {
   double* buf[4];  // BufRole
   double* rhs_in;  // MODE_RHS input state (4 planes)
   double* rhs_out;  // MODE_RHS output k (4 planes)
-  double* rk[6];    // embedded-RK work sets (4 planes each): k0..k3, stage input, second solution
+  // Runge-Kutta work sets (4 planes each): advance_adaptive uses k0..k3, its
+  // stage input (set 4) and second solution (set 5); a tableau of s stages
+  // (sbp_system_integrate) uses k0..k{s-1} and its stage input (set s)
+  double* rk[kMaxWorkSets];
   double idx_xi, idx_eta;  // 1/dξ and 1/dη (algos.rs:125-130)
   double hi_xi, hi_eta;    // SAT factors (n-1)/H0 (lib.rs:1034-1038,1095-1099)
   double dt_scale;         // this block steps with dt·dt_scale; its time is t·dt_scale

@@ -39,25 +39,53 @@ hipError_t launch_metric_pairs(const double* metrics, int64_t ps, int64_t ld, in
 // Vortex initial state on (ny, nx) coordinates into 4 planes.
 hipError_t launch_vortex(const VortexDev& vp, double gamma, double t, const double* xc, const double* yc, double* out,
                          int64_t ps, int64_t ld, int ny, int nx, unsigned* err, hipStream_t stream);
-// Per-workgroup compensated partial sums of Σ_c Σ_ij w_ij (a-b)^2 with
-// w = hx_i hy_j; b = vortex(t) if vp != nullptr, b = 0 if b == nullptr.
-// partials: 2 doubles (sum, compensation) per workgroup; returns #workgroups.
-int reduce_h2_workgroups(int ny, int nx);
-hipError_t launch_h2_reduce(int op_eta, int op_xi, const double* a, const double* b, const VortexDev* vp, double gamma,
-                            double t, const double* xc, const double* yc, int64_t ps, int64_t ld, int ny, int nx,
-                            double* partials, hipStream_t stream);
-// Per-workgroup max of |û| and |v̂| (system.rs:561-578); returns #workgroups.
-int reduce_max_workgroups(int ny, int nx);
-hipError_t launch_maxspeed(const double* q, const double* metrics, int64_t ps, int64_t ld, int ny, int nx, double* partials,
+// Compensated H-norm sums (Field::h2_err, euler/src/lib.rs:377-432):
+// Σ_c Σ_ij w_ij (a_c - b_c)² with w = hx_i hy_j, b = the vortex at time t on
+// (xc, yc) when use_vortex, b = 0 when b is null. A grid of
+// reduce_workgroups(ny) <= kReduceMaxWG workgroups strides over the rows;
+// scratch holds kReduceScratch doubles: per-workgroup (sum, compensation)
+// pairs, then the result pair at scratch[2·kReduceMaxWG].
+constexpr int kReduceMaxWG = 1024;
+constexpr int kReduceScratch = 2 * kReduceMaxWG + 2;
+int reduce_workgroups(int ny);
+struct H2Args {
+  const double* a;
+  int64_t psa, lda;
+  const double* b;
+  int64_t psb, ldb;
+  VortexDev vp;
+  int use_vortex;
+  double gamma, t;
+  const double* xc;
+  const double* yc;
+  int64_t ldc;
+  int ny, nx;
+};
+hipError_t launch_h2_sum(int op_eta, int op_xi, const H2Args& args, double* scratch, hipStream_t stream);
+// max |û| and max |v̂| (system.rs:561-578) into scratch[2·kReduceMaxWG],
+// scratch[2·kReduceMaxWG + 1] (kReduceScratch doubles of scratch).
+hipError_t launch_maxspeed(const double* q, const double* metrics, int64_t ps, int64_t ld, int ny, int nx, double* scratch,
                            hipStream_t stream);
+// Characteristic SAT of one edge added to k (SAT_north/south/east/west).
+struct SatEdgeArgs {
+  int dir;
+  double* k;
+  const double* y;
+  const double* met;
+  const double* g;
+  int64_t ps, ld;
+  int ny, nx;
+  double hi, gamma;
+};
+hipError_t launch_sat_edge(const SatEdgeArgs& args, hipStream_t stream);
 // time += dt on the device.
 hipError_t launch_advance_time(double* time, double dt, hipStream_t stream);
 // out = prev, then out = out + s_j·k_j for j < nk in order, unfused (the
 // reference's clone_from + scaled_add, integrate/src/lib.rs:187-211) over the
 // 4 planes of one block; out may be prev.
 struct LinComb {
-  const double* k[4];
-  double s[4];
+  const double* k[kMaxWorkSets];
+  double s[kMaxWorkSets];
   int nk;
 };
 hipError_t launch_lincomb(double* out, const double* prev, const LinComb& lc, int64_t ps, int64_t ld, int ny, int nx,

@@ -175,9 +175,11 @@ hipError_t launch_vortex(const VortexDev& vp, double gamma, double t, const doub
 }
 
 // ---------------------------------------------------------------------------
-// Compensated reductions. Each thread keeps a Neumaier sum; the workgroup
-// combines lanes with error-free TwoSum steps through LDS and writes one
-// (sum, compensation) pair. The host adds the pairs in long double.
+// Compensated reductions. Each thread keeps a Neumaier sum over the points it
+// visits (a grid of at most kReduceMaxWG workgroups strides over the rows);
+// a wavefront combines its 64 lanes with cross-lane shuffles and error-free
+// TwoSum merges, the four wavefronts of a workgroup meet in LDS once, and a
+// one-workgroup pass merges the per-workgroup partials into the result.
 struct KSum {
   double s, c;
 };
@@ -196,6 +198,36 @@ __device__ __forceinline__ void ksum_merge(KSum& a, const KSum& b) {
   a.c += b.c;
 }
 
+// Lane 0 of each wavefront ends with the merge of its 64 lanes.
+__device__ __forceinline__ KSum wave_ksum(KSum a) {
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) {
+    const KSum b = {__shfl_down(a.s, off, 64), __shfl_down(a.c, off, 64)};
+    ksum_merge(a, b);
+  }
+  return a;
+}
+
+// Thread 0 of a 256-thread workgroup ends with the merge of all its threads.
+__device__ __forceinline__ KSum block_ksum(KSum a) {
+  __shared__ KSum wsum[4];
+  a = wave_ksum(a);
+  if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = a;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    a = wsum[0];
+#pragma unroll
+    for (int w = 1; w < 4; ++w) ksum_merge(a, wsum[w]);
+  }
+  return a;
+}
+
+__device__ __forceinline__ double wave_max(double v) {
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) v = fmax(v, __shfl_down(v, off, 64));
+  return v;
+}
+
 template <class OX, class OY>
 __device__ __forceinline__ double hweight(int i, int n, int j, int m) {
   // h2_err weights (lib.rs:389-421): [H, 1.., rev(H)] scaled by 1/(n-1) (1/(n-2) for h2)
@@ -209,59 +241,57 @@ __device__ __forceinline__ double hweight(int i, int n, int j, int m) {
   return (hx * fx) * (hy * fy);
 }
 
-constexpr int kReduceRows = 8;
-
-int reduce_h2_workgroups(int ny, int nx) { return ((nx + 255) / 256) * ((ny + kReduceRows - 1) / kReduceRows); }
+int reduce_workgroups(int ny) { return ny < kReduceMaxWG ? ny : kReduceMaxWG; }
 
 template <class OX, class OY>
-__global__ __launch_bounds__(256) void h2_reduce_kernel(const double* __restrict__ a, const double* __restrict__ b,
-                                                        const VortexDev* vp, double gamma, double t,
-                                                        const double* __restrict__ xc, const double* __restrict__ yc,
-                                                        int64_t ps, int64_t ld, int ny, int nx, double* partials) {
-  __shared__ KSum red[256];
-  const int nbx = (nx + 255) / 256;
-  const int x = (blockIdx.x % nbx) * 256 + threadIdx.x;
-  const int yb = (blockIdx.x / nbx) * kReduceRows;
+__global__ __launch_bounds__(256) void h2_partial_kernel(const H2Args R, double* __restrict__ partials) {
   KSum acc = {0.0, 0.0};
-  if (x < nx) {
-    for (int y = yb; y < min(yb + kReduceRows, ny); ++y) {
-      const int64_t o = (int64_t)y * ld + x;
+  for (int y = blockIdx.x; y < R.ny; y += gridDim.x) {
+    for (int x = threadIdx.x; x < R.nx; x += 256) {
+      const int64_t oa = (int64_t)y * R.lda + x;
       double ref[4] = {0.0, 0.0, 0.0, 0.0};
-      if (vp) {
+      if (R.use_vortex) {
+        const int64_t oc = (int64_t)y * R.ldc + x;
         unsigned e = 0;
-        vortex_point(*vp, gamma, t, xc[o], yc[o], ref, e);
-      } else if (b) {
+        vortex_point(R.vp, R.gamma, R.t, R.xc[oc], R.yc[oc], ref, e);
+      } else if (R.b) {
+        const int64_t ob = (int64_t)y * R.ldb + x;
 #pragma unroll
-        for (int c = 0; c < 4; ++c) ref[c] = b[c * ps + o];
+        for (int c = 0; c < 4; ++c) ref[c] = R.b[c * R.psb + ob];
       }
-      const double w = hweight<OX, OY>(x, nx, y, ny);
+      const double w = hweight<OX, OY>(x, R.nx, y, R.ny);
 #pragma unroll
       for (int c = 0; c < 4; ++c) {
-        const double d = a[c * ps + o] - ref[c];
+        const double d = R.a[c * R.psa + oa] - ref[c];
         ksum_add(acc, d * d * w);
       }
     }
   }
-  red[threadIdx.x] = acc;
-  __syncthreads();
-  for (int off = 128; off > 0; off >>= 1) {
-    if ((int)threadIdx.x < off) ksum_merge(red[threadIdx.x], red[threadIdx.x + off]);
-    __syncthreads();
-  }
+  acc = block_ksum(acc);
   if (threadIdx.x == 0) {
-    partials[2 * blockIdx.x] = red[0].s;
-    partials[2 * blockIdx.x + 1] = red[0].c;
+    partials[2 * blockIdx.x] = acc.s;
+    partials[2 * blockIdx.x + 1] = acc.c;
+  }
+}
+
+// One workgroup: the n per-workgroup (sum, compensation) pairs into
+// scratch[2·kReduceMaxWG], scratch[2·kReduceMaxWG + 1].
+__global__ __launch_bounds__(256) void h2_finish_kernel(double* __restrict__ scratch, int n) {
+  KSum acc = {0.0, 0.0};
+  for (int i = threadIdx.x; i < n; i += 256) ksum_merge(acc, KSum{scratch[2 * i], scratch[2 * i + 1]});
+  acc = block_ksum(acc);
+  if (threadIdx.x == 0) {
+    scratch[2 * kReduceMaxWG] = acc.s;
+    scratch[2 * kReduceMaxWG + 1] = acc.c;
   }
 }
 
 template <class OX>
-static hipError_t h2_reduce_x(int op_eta, const double* a, const double* b, const VortexDev* vp, double gamma, double t,
-                              const double* xc, const double* yc, int64_t ps, int64_t ld, int ny, int nx, double* partials,
-                              hipStream_t stream) {
-  const int nwg = reduce_h2_workgroups(ny, nx);
-#define SBP_H2_CASE(ID, T)                                                                                      \
-  case ID:                                                                                                       \
-    h2_reduce_kernel<OX, ops::T><<<nwg, 256, 0, stream>>>(a, b, vp, gamma, t, xc, yc, ps, ld, ny, nx, partials); \
+static hipError_t h2_partial_x(int op_eta, const H2Args& R, double* scratch, hipStream_t stream) {
+  const int nwg = reduce_workgroups(R.ny);
+#define SBP_H2_CASE(ID, T)                                                      \
+  case ID:                                                                       \
+    h2_partial_kernel<OX, ops::T><<<nwg, 256, 0, stream>>>(R, scratch);          \
     break;
   switch (op_eta) {
     SBP_H2_CASE(UPWIND4, Upwind4)
@@ -276,32 +306,32 @@ static hipError_t h2_reduce_x(int op_eta, const double* a, const double* b, cons
   return hipGetLastError();
 }
 
-hipError_t launch_h2_reduce(int op_eta, int op_xi, const double* a, const double* b, const VortexDev* vp, double gamma,
-                            double t, const double* xc, const double* yc, int64_t ps, int64_t ld, int ny, int nx,
-                            double* partials, hipStream_t stream) {
+hipError_t launch_h2_sum(int op_eta, int op_xi, const H2Args& R, double* scratch, hipStream_t stream) {
+  if (R.ny <= 0 || R.nx <= 0) return hipErrorInvalidValue;
+  hipError_t e;
   switch (op_xi) {
-    case UPWIND4: return h2_reduce_x<ops::Upwind4>(op_eta, a, b, vp, gamma, t, xc, yc, ps, ld, ny, nx, partials, stream);
-    case UPWIND4H2: return h2_reduce_x<ops::Upwind4h2>(op_eta, a, b, vp, gamma, t, xc, yc, ps, ld, ny, nx, partials, stream);
-    case UPWIND9: return h2_reduce_x<ops::Upwind9>(op_eta, a, b, vp, gamma, t, xc, yc, ps, ld, ny, nx, partials, stream);
-    case UPWIND9H2: return h2_reduce_x<ops::Upwind9h2>(op_eta, a, b, vp, gamma, t, xc, yc, ps, ld, ny, nx, partials, stream);
-    case SBP4: return h2_reduce_x<ops::SBP4>(op_eta, a, b, vp, gamma, t, xc, yc, ps, ld, ny, nx, partials, stream);
-    case SBP8: return h2_reduce_x<ops::SBP8>(op_eta, a, b, vp, gamma, t, xc, yc, ps, ld, ny, nx, partials, stream);
+    case UPWIND4: e = h2_partial_x<ops::Upwind4>(op_eta, R, scratch, stream); break;
+    case UPWIND4H2: e = h2_partial_x<ops::Upwind4h2>(op_eta, R, scratch, stream); break;
+    case UPWIND9: e = h2_partial_x<ops::Upwind9>(op_eta, R, scratch, stream); break;
+    case UPWIND9H2: e = h2_partial_x<ops::Upwind9h2>(op_eta, R, scratch, stream); break;
+    case SBP4: e = h2_partial_x<ops::SBP4>(op_eta, R, scratch, stream); break;
+    case SBP8: e = h2_partial_x<ops::SBP8>(op_eta, R, scratch, stream); break;
     default: return hipErrorInvalidValue;
   }
+  if (e != hipSuccess) return e;
+  h2_finish_kernel<<<1, 256, 0, stream>>>(scratch, reduce_workgroups(R.ny));
+  return hipGetLastError();
 }
 
-// max |û|, |v̂| (system.rs:561-578); wavefront reduction with DPP-free shuffles.
-int reduce_max_workgroups(int ny, int nx) { return ((nx + 255) / 256) * ((ny + kReduceRows - 1) / kReduceRows); }
-
+// max |û|, |v̂| (system.rs:561-578): per-thread maxima over the strided
+// points, a wavefront maximum by cross-lane shuffles, one LDS step across
+// the four wavefronts, then one workgroup over the partials.
 __global__ __launch_bounds__(256) void maxspeed_kernel(const double* __restrict__ q, const double* __restrict__ met, int64_t ps,
-                                                       int64_t ld, int ny, int nx, double* partials) {
-  __shared__ double red[2][256];
-  const int nbx = (nx + 255) / 256;
-  const int x = (blockIdx.x % nbx) * 256 + threadIdx.x;
-  const int yb = (blockIdx.x / nbx) * kReduceRows;
+                                                       int64_t ld, int ny, int nx, double* __restrict__ partials) {
+  __shared__ double wmax[2][4];
   double mu = 0.0, mv = 0.0;
-  if (x < nx) {
-    for (int y = yb; y < min(yb + kReduceRows, ny); ++y) {
+  for (int y = blockIdx.x; y < ny; y += gridDim.x) {
+    for (int x = threadIdx.x; x < nx; x += 256) {
       const int64_t o = (int64_t)y * ld + x;
       const double u = q[ps + o] / q[o], v = q[2 * ps + o] / q[o];
       const double uh = met[ps + o] * u + met[2 * ps + o] * v;
@@ -310,25 +340,82 @@ __global__ __launch_bounds__(256) void maxspeed_kernel(const double* __restrict_
       mv = fmax(mv, fabs(vh));
     }
   }
-  red[0][threadIdx.x] = mu;
-  red[1][threadIdx.x] = mv;
-  __syncthreads();
-  for (int off = 128; off > 0; off >>= 1) {
-    if ((int)threadIdx.x < off) {
-      red[0][threadIdx.x] = fmax(red[0][threadIdx.x], red[0][threadIdx.x + off]);
-      red[1][threadIdx.x] = fmax(red[1][threadIdx.x], red[1][threadIdx.x + off]);
-    }
-    __syncthreads();
+  mu = wave_max(mu);
+  mv = wave_max(mv);
+  if ((threadIdx.x & 63) == 0) {
+    wmax[0][threadIdx.x >> 6] = mu;
+    wmax[1][threadIdx.x >> 6] = mv;
   }
+  __syncthreads();
   if (threadIdx.x == 0) {
-    partials[2 * blockIdx.x] = red[0][0];
-    partials[2 * blockIdx.x + 1] = red[1][0];
+    partials[2 * blockIdx.x] = fmax(fmax(wmax[0][0], wmax[0][1]), fmax(wmax[0][2], wmax[0][3]));
+    partials[2 * blockIdx.x + 1] = fmax(fmax(wmax[1][0], wmax[1][1]), fmax(wmax[1][2], wmax[1][3]));
   }
 }
 
-hipError_t launch_maxspeed(const double* q, const double* metrics, int64_t ps, int64_t ld, int ny, int nx, double* partials,
+__global__ __launch_bounds__(256) void max_finish_kernel(double* __restrict__ scratch, int n) {
+  __shared__ double wmax[2][4];
+  double mu = 0.0, mv = 0.0;
+  for (int i = threadIdx.x; i < n; i += 256) {
+    mu = fmax(mu, scratch[2 * i]);
+    mv = fmax(mv, scratch[2 * i + 1]);
+  }
+  mu = wave_max(mu);
+  mv = wave_max(mv);
+  if ((threadIdx.x & 63) == 0) {
+    wmax[0][threadIdx.x >> 6] = mu;
+    wmax[1][threadIdx.x >> 6] = mv;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    scratch[2 * kReduceMaxWG] = fmax(fmax(wmax[0][0], wmax[0][1]), fmax(wmax[0][2], wmax[0][3]));
+    scratch[2 * kReduceMaxWG + 1] = fmax(fmax(wmax[1][0], wmax[1][1]), fmax(wmax[1][2], wmax[1][3]));
+  }
+}
+
+hipError_t launch_maxspeed(const double* q, const double* metrics, int64_t ps, int64_t ld, int ny, int nx, double* scratch,
                            hipStream_t stream) {
-  maxspeed_kernel<<<reduce_max_workgroups(ny, nx), 256, 0, stream>>>(q, metrics, ps, ld, ny, nx, partials);
+  if (ny <= 0 || nx <= 0) return hipErrorInvalidValue;
+  const int nwg = reduce_workgroups(ny);
+  maxspeed_kernel<<<nwg, 256, 0, stream>>>(q, metrics, ps, ld, ny, nx, scratch);
+  max_finish_kernel<<<1, 256, 0, stream>>>(scratch, nwg);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Characteristic SAT of one block edge added to k (SAT_north / SAT_south /
+// SAT_east / SAT_west, euler/src/lib.rs:1025-1144, SAT_characteristic
+// :1150-1252), one thread per edge point: k += hi·τ·T((L + σ|L|)∘(TI(q - g))).
+// k, y: 4 planes (plane stride ps, row pitch ld); metrics: 5 planes;
+// g: 4 x n component-major boundary data.
+__global__ __launch_bounds__(256) void sat_edge_kernel(const SatEdgeArgs A) {
+  const bool ns = A.dir == SBP_NORTH || A.dir == SBP_SOUTH;
+  const int n = ns ? A.nx : A.ny;
+  const int t = blockIdx.x * 256 + threadIdx.x;
+  if (t >= n) return;
+  const int x = ns ? t : (A.dir == SBP_EAST ? A.nx - 1 : 0);
+  const int y = ns ? (A.dir == SBP_NORTH ? A.ny - 1 : 0) : t;
+  const int64_t o = (int64_t)y * A.ld + x;
+  double q[4], g[4], k[4];
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    q[c] = A.y[c * A.ps + o];
+    g[c] = A.g[c * n + t];
+    k[c] = A.k[c * A.ps + o];
+  }
+  // (σ, τ): north (-1, +1), south (+1, -1), east (-1, +1), west (+1, -1)
+  const double sign = (A.dir == SBP_NORTH || A.dir == SBP_EAST) ? -1.0 : 1.0;
+  const double tau = -sign;
+  const int m0 = ns ? 3 : 1;  // J·ηx, J·ηy for north/south, J·ξx, J·ξy for east/west
+  sat_point(k, q, g, A.hi, sign, tau, A.met[o], A.met[m0 * A.ps + o], A.met[(m0 + 1) * A.ps + o], A.gamma);
+#pragma unroll
+  for (int c = 0; c < 4; ++c) A.k[c * A.ps + o] = k[c];
+}
+
+hipError_t launch_sat_edge(const SatEdgeArgs& a, hipStream_t stream) {
+  const int n = (a.dir == SBP_NORTH || a.dir == SBP_SOUTH) ? a.nx : a.ny;
+  if (n <= 0) return hipErrorInvalidValue;
+  sat_edge_kernel<<<(n + 255) / 256, 256, 0, stream>>>(a);
   return hipGetLastError();
 }
 

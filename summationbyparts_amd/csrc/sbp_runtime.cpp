@@ -177,7 +177,8 @@ struct HostBlock {
   double* d_y = nullptr;
   double* d_buf[4] = {nullptr, nullptr, nullptr, nullptr};
   double* d_k = nullptr;
-  double* d_rk = nullptr;  // embedded-RK work sets (6 x 4 planes), allocated on first use
+  double* d_rk = nullptr;  // Runge-Kutta work sets (rk_sets x 4 planes), allocated on first use
+  int rk_sets = 0;
   double dt_scale = 1.0;  // the block steps with dt·dt_scale (independent solves in one batch)
   int bc_kind[4] = {SBP_BC_THIS, SBP_BC_THIS, SBP_BC_THIS, SBP_BC_THIS};
   int bc_nb[4] = {0, 0, 0, 0};
@@ -275,7 +276,7 @@ BlockDev make_dev(const sbp_system_s* s, int i, int shape) {
   for (int r = 0; r < 4; ++r) b.buf[r] = h.d_buf[r];
   b.rhs_in = h.d_buf[BUF_YN];
   b.rhs_out = h.d_k;
-  for (int r = 0; r < 6; ++r) b.rk[r] = h.d_rk ? h.d_rk + (int64_t)r * 4 * h.ps : nullptr;
+  for (int r = 0; r < kMaxWorkSets; ++r) b.rk[r] = r < h.rk_sets ? h.d_rk + (int64_t)r * 4 * h.ps : nullptr;
   OpInfo ox, oe;
   op_info(h.op_xi, ox);
   op_info(h.op_eta, oe);
@@ -634,37 +635,64 @@ int advance_impl(sbp_system_s* s, double dt, int64_t nsteps) {
   return SBP_OK;
 }
 
-int ensure_partials(sbp_system_s* s, int n) {
-  if (s->partials_cap >= n) return SBP_OK;
-  if (s->d_partials) hipFree(s->d_partials);
-  s->d_partials = nullptr;
-  SBP_TRY_HIP(hipMalloc(&s->d_partials, sizeof(double) * 2 * n));
-  s->partials_cap = n;
+int ensure_partials(sbp_system_s* s) {
+  if (s->d_partials) return SBP_OK;
+  SBP_TRY_HIP(hipMalloc(&s->d_partials, sizeof(double) * kReduceScratch));
+  return SBP_OK;
+}
+
+// At least `sets` Runge-Kutta work sets per block (zeroed once); descriptors
+// are rebuilt when a block's sets grow.
+int ensure_work_sets(sbp_system_s* s, int sets) {
+  hipStream_t st = s->ctx->stream;
+  for (auto& h : s->blocks) {
+    if (h.rk_sets >= sets) continue;
+    SBP_TRY_HIP(hipStreamSynchronize(st));
+    if (h.d_rk) hipFree(h.d_rk);
+    h.d_rk = nullptr;
+    h.rk_sets = 0;
+    SBP_TRY_HIP(hipMalloc(&h.d_rk, sizeof(double) * sets * 4 * h.ps));
+    SBP_TRY_HIP(hipMemsetAsync(h.d_rk, 0, sizeof(double) * sets * 4 * h.ps, st));
+    h.rk_sets = sets;
+    s->dirty = true;
+  }
+  return SBP_OK;
+}
+
+// Reads the result pair of a reduction in `scratch` (waits for the stream).
+int reduce_result(const double* scratch, hipStream_t st, double out[2]) {
+  SBP_TRY_HIP(hipMemcpyAsync(out, scratch + 2 * kReduceMaxWG, 2 * sizeof(double), hipMemcpyDeviceToHost, st));
+  SBP_TRY_HIP(hipStreamSynchronize(st));
   return SBP_OK;
 }
 
 int h2_sum(sbp_system_s* s, int bi, const double* b, const VortexDev* vp_host, double t, long double& sum) {
   SBP_TRY(normalize(s));
   HostBlock& h = s->blocks[bi];
-  const int nwg = reduce_h2_workgroups(h.ny, h.nx);
-  SBP_TRY(ensure_partials(s, nwg + 64));
-  VortexDev* d_vp = nullptr;
+  SBP_TRY(ensure_partials(s));
   hipStream_t st = s->ctx->stream;
   double gamma;
   gamma_get(gamma);
-  if (vp_host) {
-    // the parameters live right after the partials
-    d_vp = reinterpret_cast<VortexDev*>(s->d_partials + 2 * nwg);
-    SBP_TRY_HIP(hipMemcpyAsync(d_vp, vp_host, sizeof(VortexDev), hipMemcpyHostToDevice, st));
-  }
-  SBP_TRY_HIP(launch_h2_reduce(h.op_eta, h.op_xi, h.d_buf[BUF_YN], b, d_vp, gamma, t, h.d_x, h.d_y, h.ps, h.ld, h.ny, h.nx,
-                               s->d_partials, st));
-  std::vector<double> part(2 * nwg);
-  SBP_TRY_HIP(hipMemcpyAsync(part.data(), s->d_partials, sizeof(double) * 2 * nwg, hipMemcpyDeviceToHost, st));
-  SBP_TRY_HIP(hipStreamSynchronize(st));
-  long double acc = 0.0L;
-  for (int i = 0; i < nwg; ++i) acc += (long double)part[2 * i] + (long double)part[2 * i + 1];
-  sum = acc;
+  H2Args R{};
+  R.a = h.d_buf[BUF_YN];
+  R.psa = h.ps;
+  R.lda = h.ld;
+  R.b = b;
+  R.psb = h.ps;
+  R.ldb = h.ld;
+  if (vp_host) R.vp = *vp_host;
+  R.use_vortex = vp_host != nullptr;
+  R.gamma = gamma;
+  R.t = t;
+  R.xc = h.d_x;
+  R.yc = h.d_y;
+  R.ldc = h.ld;
+  R.ny = h.ny;
+  R.nx = h.nx;
+  SBP_TRY_HIP(launch_h2_sum(h.op_eta, h.op_xi, R, s->d_partials, st));
+  double r[2];
+  SBP_TRY(reduce_result(s->d_partials, st, r));
+  sum = (long double)r[0] + (long double)r[1];
   return SBP_OK;
 }
 
@@ -893,21 +921,33 @@ int sbp_metrics(sbp_context ctx, int op_eta, int op_xi, int ny, int nx, const do
   return status_from_errword(w);
 }
 
-int sbp_rhs(sbp_context ctx, int kind, int op_eta, int op_xi, int ny, int nx, double* k, const double* y, const double* metrics,
-            const double* const* boundaries) {
-  if (!ctx || !k || !y || !metrics || !boundaries) return SBP_ERR_INVALID;
-  if (kind != SBP_RHS_TRAD && kind != SBP_RHS_UPWIND) return SBP_ERR_INVALID;
+// The context's sticky error word (device errors of operator-level calls),
+// read and cleared.
+static int ctx_check_err(sbp_context ctx) {
+  unsigned w = 0;
+  SBP_TRY_HIP(hipMemcpyAsync(&w, ctx->d_err, sizeof(unsigned), hipMemcpyDeviceToHost, ctx->stream));
+  SBP_TRY_HIP(hipStreamSynchronize(ctx->stream));
+  if (w) SBP_TRY_HIP(hipMemsetAsync(ctx->d_err, 0, sizeof(unsigned), ctx->stream));
+  return status_from_errword(w);
+}
+
+// One block's right-hand side on caller buffers (contiguous (4, ny, nx)
+// fields, (5, ny, nx) metrics) with the upwind terms when `up`, the SATs of
+// the caller's boundary data when `boundaries` is given and none otherwise
+// (RHS_no_SAT).
+static int rhs_block(sbp_context ctx, bool up, int op_eta, int op_xi, int ny, int nx, double* k, const double* y,
+                     const double* metrics, const double* const* boundaries) {
   SBP_TRY(check_shape(op_eta, op_xi, ny, nx));
   if (5 * (int64_t)nx * ny * 8 >= ((int64_t)1 << 32)) return SBP_ERR_INVALID;
   OpInfo oe, ox;
   op_info(op_eta, oe);
   op_info(op_xi, ox);
-  const bool up = kind == SBP_RHS_UPWIND;
   if (up && !(oe.upwind && ox.upwind)) return SBP_ERR_NOT_UPWIND;
   double gamma;
   if (!gamma_get(gamma)) return SBP_ERR_GAMMA;
-  for (int d = 0; d < 4; ++d)
-    if (!boundaries[d]) return SBP_ERR_INVALID;
+  if (boundaries)
+    for (int d = 0; d < 4; ++d)
+      if (!boundaries[d]) return SBP_ERR_INVALID;
   SBP_TRY_HIP(hipSetDevice(ctx->device));
   const int shape = 0;
   const TileShape& ts = kTileShapes[shape];
@@ -930,8 +970,8 @@ int sbp_rhs(sbp_context ctx, int kind, int op_eta, int op_xi, int ny, int nx, do
   b.hi_xi = (ox.h2 ? (double)(nx - 2) : (double)(nx - 1)) / ox.h[0];
   b.hi_eta = (oe.h2 ? (double)(ny - 2) : (double)(ny - 1)) / oe.h[0];
   for (int d = 0; d < 4; ++d) {
-    b.edge_src[d] = EDGE_EXPLICIT;
-    b.edge_ptr[d] = boundaries[d];
+    b.edge_src[d] = boundaries ? EDGE_EXPLICIT : EDGE_NONE;
+    b.edge_ptr[d] = boundaries ? boundaries[d] : nullptr;
     b.edge_cs[d] = (d == SBP_NORTH || d == SBP_SOUTH) ? nx : ny;
   }
   std::vector<TileDev> tiles;
@@ -963,11 +1003,120 @@ int sbp_rhs(sbp_context ctx, int kind, int op_eta, int op_xi, int ny, int nx, do
   a.error = ctx->d_err;
   a.gamma = gamma;
   SBP_TRY_HIP(launch_stage(op_eta, op_xi, up, true, shape, MODE_RHS, a, (int)tiles.size(), st));
-  unsigned w = 0;
-  SBP_TRY_HIP(hipMemcpyAsync(&w, ctx->d_err, sizeof(unsigned), hipMemcpyDeviceToHost, st));
-  SBP_TRY_HIP(hipStreamSynchronize(st));
-  if (w) SBP_TRY_HIP(hipMemsetAsync(ctx->d_err, 0, sizeof(unsigned), st));
-  return status_from_errword(w);
+  return ctx_check_err(ctx);
+}
+
+int sbp_rhs(sbp_context ctx, int kind, int op_eta, int op_xi, int ny, int nx, double* k, const double* y, const double* metrics,
+            const double* const* boundaries) {
+  if (!ctx || !k || !y || !metrics || !boundaries) return SBP_ERR_INVALID;
+  if (kind != SBP_RHS_TRAD && kind != SBP_RHS_UPWIND) return SBP_ERR_INVALID;
+  return rhs_block(ctx, kind == SBP_RHS_UPWIND, op_eta, op_xi, ny, nx, k, y, metrics, boundaries);
+}
+
+int sbp_rhs_no_sat(sbp_context ctx, int op_eta, int op_xi, int ny, int nx, double* k, const double* y, const double* metrics) {
+  if (!ctx || !k || !y || !metrics) return SBP_ERR_INVALID;
+  OpInfo oe, ox;
+  if (!op_info(op_eta, oe) || !op_info(op_xi, ox)) return SBP_ERR_INVALID;
+  // op.upwind().is_some() for the operator pair (sbp/src/operators.rs:137-150)
+  return rhs_block(ctx, oe.upwind && ox.upwind, op_eta, op_xi, ny, nx, k, y, metrics, nullptr);
+}
+
+// SAT of the edges in `dirs` (in that order) added to k on caller buffers.
+static int sat_edges(sbp_context ctx, const int* dirs, int ndirs, int op_eta, int op_xi, int ny, int nx, double* k,
+                     const double* y, const double* metrics, const double* const* g) {
+  if (!ctx || !k || !y || !metrics || !g) return SBP_ERR_INVALID;
+  SBP_TRY(check_shape(op_eta, op_xi, ny, nx));
+  double gamma;
+  if (!gamma_get(gamma)) return SBP_ERR_GAMMA;
+  OpInfo oe, ox;
+  op_info(op_eta, oe);
+  op_info(op_xi, ox);
+  SBP_TRY_HIP(hipSetDevice(ctx->device));
+  for (int i = 0; i < ndirs; ++i) {
+    if (!g[i]) return SBP_ERR_INVALID;
+    SatEdgeArgs a{};
+    a.dir = dirs[i];
+    a.k = k;
+    a.y = y;
+    a.met = metrics;
+    a.g = g[i];
+    a.ps = (int64_t)nx * ny;
+    a.ld = nx;
+    a.ny = ny;
+    a.nx = nx;
+    const bool ns = a.dir == SBP_NORTH || a.dir == SBP_SOUTH;
+    // hi = (n - 1)/H[0], (n - 2)/H[0] on h2 grids (lib.rs:1034-1038, :1095-1099)
+    a.hi = ns ? (oe.h2 ? (double)(ny - 2) : (double)(ny - 1)) / oe.h[0]
+              : (ox.h2 ? (double)(nx - 2) : (double)(nx - 1)) / ox.h[0];
+    a.gamma = gamma;
+    SBP_TRY_HIP(launch_sat_edge(a, ctx->stream));
+  }
+  SBP_TRY_HIP(hipStreamSynchronize(ctx->stream));
+  return SBP_OK;
+}
+
+int sbp_sat_characteristics(sbp_context ctx, int op_eta, int op_xi, int ny, int nx, double* k, const double* y,
+                            const double* metrics, const double* const* boundaries) {
+  static const int order[4] = {SBP_NORTH, SBP_SOUTH, SBP_EAST, SBP_WEST};
+  if (!boundaries) return SBP_ERR_INVALID;
+  return sat_edges(ctx, order, 4, op_eta, op_xi, ny, nx, k, y, metrics, boundaries);
+}
+
+int sbp_sat_edge(sbp_context ctx, int dir, int op_eta, int op_xi, int ny, int nx, double* k, const double* y,
+                 const double* metrics, const double* boundary) {
+  if (dir < 0 || dir > 3) return SBP_ERR_INVALID;
+  const double* g[1] = {boundary};
+  return sat_edges(ctx, &dir, 1, op_eta, op_xi, ny, nx, k, y, metrics, g);
+}
+
+// Field::h2_err of two fields on caller buffers through `scratch`
+// (kReduceScratch doubles of device memory).
+static int h2_err_with(sbp_context ctx, int op_eta, int op_xi, int ny, int nx, const double* a, const double* b,
+                       double* scratch, double* out) {
+  H2Args R{};
+  R.a = a;
+  R.psa = (int64_t)nx * ny;
+  R.lda = nx;
+  R.b = b;
+  R.psb = R.psa;
+  R.ldb = nx;
+  R.ny = ny;
+  R.nx = nx;
+  SBP_TRY_HIP(launch_h2_sum(op_eta, op_xi, R, scratch, ctx->stream));
+  double r[2];
+  SBP_TRY(reduce_result(scratch, ctx->stream, r));
+  *out = std::sqrt(r[0] + r[1]);
+  return SBP_OK;
+}
+
+static int h2_err_args(sbp_context ctx, int op_eta, int op_xi, int ny, int nx, const double* a, const double* b,
+                       double* out) {
+  if (!ctx || !a || !b || !out) return SBP_ERR_INVALID;
+  OpInfo oe, ox;
+  if (!op_info(op_eta, oe) || !op_info(op_xi, ox)) return SBP_ERR_INVALID;
+  // the weights [H, 1.., rev(H)] need n >= 2M (lib.rs:389-395)
+  if (nx < 2 * ox.M || ny < 2 * oe.M) return SBP_ERR_SHAPE;
+  if ((int64_t)nx * ny > (int64_t)INT32_MAX) return SBP_ERR_INVALID;
+  return SBP_OK;
+}
+
+int sbp_h2_err(sbp_context ctx, int op_eta, int op_xi, int ny, int nx, const double* a, const double* b, double* out) {
+  SBP_TRY(h2_err_args(ctx, op_eta, op_xi, ny, nx, a, b, out));
+  SBP_TRY_HIP(hipSetDevice(ctx->device));
+  double* scratch = nullptr;
+  SBP_TRY_HIP(hipMallocAsync((void**)&scratch, sizeof(double) * kReduceScratch, ctx->stream));
+  const int rc = h2_err_with(ctx, op_eta, op_xi, ny, nx, a, b, scratch, out);
+  hipFreeAsync(scratch, ctx->stream);
+  return rc;
+}
+
+int sbp_h2linspace(double start, double end, int n, double* out) {
+  // the reference's arithmetic, two roundings per point (Rust never contracts)
+#pragma clang fp contract(off)
+  if (!out || n < 2) return SBP_ERR_INVALID;
+  const double h = (end - start) / (double)(n - 2);
+  for (int i = 0; i < n; ++i) out[i] = i == 0 ? start : i == n - 1 ? end : start + h * ((double)i - 0.5);
+  return SBP_OK;
 }
 
 int sbp_system_create(sbp_context ctx, int nblocks, const int* ny, const int* nx, const int* op_eta, const int* op_xi,
@@ -1296,18 +1445,12 @@ int sbp_system_max_dt(sbp_system s, double* out) {
   double max_dt = INFINITY;
   SBP_TRY(normalize(s));
   hipStream_t st = s->ctx->stream;
+  SBP_TRY(ensure_partials(s));
   for (auto& h : s->blocks) {
-    const int nwg = reduce_max_workgroups(h.ny, h.nx);
-    SBP_TRY(ensure_partials(s, nwg));
     SBP_TRY_HIP(launch_maxspeed(h.d_buf[BUF_YN], h.d_metrics, h.ps, h.ld, h.ny, h.nx, s->d_partials, st));
-    std::vector<double> part(2 * nwg);
-    SBP_TRY_HIP(hipMemcpyAsync(part.data(), s->d_partials, sizeof(double) * 2 * nwg, hipMemcpyDeviceToHost, st));
-    SBP_TRY_HIP(hipStreamSynchronize(st));
-    double mu = 0.0, mv = 0.0;
-    for (int i = 0; i < nwg; ++i) {
-      mu = std::fmax(mu, part[2 * i]);
-      mv = std::fmax(mv, part[2 * i + 1]);
-    }
+    double r[2];
+    SBP_TRY(reduce_result(s->d_partials, st, r));
+    const double mu = r[0], mv = r[1];
     const double dx = 1.0 / (double)h.nx, dy = 1.0 / (double)h.ny;
     const double c_dt = std::fmax(mu / dx, mv / dy);
     max_dt = std::fmin(max_dt, c_max / c_dt);
@@ -1444,14 +1587,7 @@ int sbp_system_advance_adaptive(sbp_system s, double dt, double* guess_dt, doubl
   if (!gamma_get(gamma)) return SBP_ERR_GAMMA;
   SBP_TRY_HIP(hipSetDevice(s->ctx->device));
   hipStream_t st = s->ctx->stream;
-  bool alloc = false;
-  for (auto& h : s->blocks)
-    if (!h.d_rk) {
-      SBP_TRY_HIP(hipMalloc(&h.d_rk, sizeof(double) * 6 * 4 * h.ps));
-      SBP_TRY_HIP(hipMemsetAsync(h.d_rk, 0, sizeof(double) * 6 * 4 * h.ps, st));
-      alloc = true;
-    }
-  if (alloc) s->dirty = true;
+  SBP_TRY(ensure_work_sets(s, 6));
   SBP_TRY(normalize(s));
   SBP_TRY(rebuild(s));
   // BogackiShampine (integrate/src/lib.rs:138-151): A, B, C (C carries its
@@ -1519,6 +1655,63 @@ int sbp_system_advance_adaptive(sbp_system s, double dt, double* guess_dt, doubl
   }
   *guess_dt = g;
   SBP_TRY_HIP(hipStreamSynchronize(st));
+  return check_err(s);
+}
+
+int sbp_system_integrate(sbp_system s, int nstages, const double* A, const double* B, const double* C, double dt,
+                         int64_t nsteps) {
+  if (!s || !A || !B || (nstages > 1 && !C) || nstages < 1 || nstages + 1 > kMaxWorkSets || nsteps < 0 ||
+      !std::isfinite(dt))
+    return SBP_ERR_INVALID;
+  if (s->precision != SBP_FP64) return SBP_ERR_INVALID;
+  double gamma;
+  if (!gamma_get(gamma)) return SBP_ERR_GAMMA;
+  SBP_TRY_HIP(hipSetDevice(s->ctx->device));
+  hipStream_t st = s->ctx->stream;
+  SBP_TRY(ensure_work_sets(s, nstages + 1));
+  SBP_TRY(normalize(s));
+  SBP_TRY(rebuild(s));
+  const int SET_STAGE = nstages;  // the stage input after k0..k{s-1}
+  auto set_of = [](const HostBlock& h, int r) { return h.d_rk + (int64_t)r * 4 * h.ps; };
+  // out = prev + Σ (w_j·dt_b)·k_j over the nonzero weights, in order (the
+  // reference's clone_from and scaled_adds, integrate/src/lib.rs:187-211)
+  auto combine = [&](const double* w, int nw, double* const* out) -> int {
+    for (size_t bi = 0; bi < s->blocks.size(); ++bi) {
+      const HostBlock& h = s->blocks[bi];
+      const double g = dt * h.dt_scale;
+      LinComb lc{};
+      for (int j = 0; j < nw; ++j) {
+        if (w[j] == 0.0) continue;
+        lc.k[lc.nk] = set_of(h, j);
+        lc.s[lc.nk] = w[j] * g;
+        ++lc.nk;
+      }
+      SBP_TRY_HIP(launch_lincomb(out[bi], h.d_buf[BUF_YN], lc, h.ps, h.ld, h.ny, h.nx, st));
+    }
+    return SBP_OK;
+  };
+  std::vector<double*> stage_out(s->blocks.size()), final_out(s->blocks.size());
+  for (size_t bi = 0; bi < s->blocks.size(); ++bi) {
+    stage_out[bi] = set_of(s->blocks[bi], SET_STAGE);
+    final_out[bi] = s->blocks[bi].d_buf[BUF_YN];
+  }
+  // integrate (utils/integrate/src/lib.rs:176-222): stage 0 at t from y_n,
+  // stage i > 0 at t + dt·C[i-1] from y_n + Σ_{j<i} (A[i][j]·dt)·k_j, then
+  // y_{n+1} = y_n + Σ (B[j]·dt)·k_j and t += dt (the device time, read by the
+  // far-field edge data)
+  for (int64_t step = 0; step < nsteps; ++step) {
+    for (int i = 0; i < nstages; ++i) {
+      if (i > 0) SBP_TRY(combine(A + (size_t)i * nstages, i, stage_out.data()));
+      StageArgs a = stage_args(s, 0.0, 0, 0);
+      a.time_offset = i == 0 ? 0.0 : dt * C[i - 1];
+      a.rk_a = a.rk_b = 0.0;
+      a.rhs_in_sel = i == 0 ? 0 : SET_STAGE + 1;
+      a.rhs_out_sel = i + 1;
+      for (const Group& grp : s->groups) SBP_TRY_HIP(launch_group(s, grp, grp.shape, MODE_RHS, a, st));
+    }
+    SBP_TRY(combine(B, nstages, final_out.data()));
+    SBP_TRY_HIP(launch_advance_time(s->d_time, dt, st));
+  }
   return check_err(s);
 }
 

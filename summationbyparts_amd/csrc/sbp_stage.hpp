@@ -585,19 +585,19 @@ __global__ __launch_bounds__((TX / RX) * (TY / RY)) void stage_kernel(const Stag
             meta1 = ldg(met + 4 * ps, o);
           }
           const double dj = ldg(met, o);
-          if (gy == ny - 1) {
+          if (gy == ny - 1 && B.edge_src[SBP_NORTH] != EDGE_NONE) {
             edge_data(A, B, SBP_NORTH, gx, mode, g);
             sat_point(sat, q4, g, B.hi_eta, -1.0, 1.0, dj, meta0, meta1, gamma);
           }
-          if (gy == 0) {
+          if (gy == 0 && B.edge_src[SBP_SOUTH] != EDGE_NONE) {
             edge_data(A, B, SBP_SOUTH, gx, mode, g);
             sat_point(sat, q4, g, B.hi_eta, 1.0, -1.0, dj, meta0, meta1, gamma);
           }
-          if (gx == nx - 1) {
+          if (gx == nx - 1 && B.edge_src[SBP_EAST] != EDGE_NONE) {
             edge_data(A, B, SBP_EAST, gy, mode, g);
             sat_point(sat, q4, g, B.hi_xi, -1.0, 1.0, dj, mxi0, mxi1, gamma);
           }
-          if (gx == 0) {
+          if (gx == 0 && B.edge_src[SBP_WEST] != EDGE_NONE) {
             edge_data(A, B, SBP_WEST, gy, mode, g);
             sat_point(sat, q4, g, B.hi_xi, 1.0, -1.0, dj, mxi0, mxi1, gamma);
           }

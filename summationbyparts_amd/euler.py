@@ -217,6 +217,17 @@ class BlockSystem:
                    "sbp_system_advance_adaptive")
         return g.value
 
+    def integrate(self, tableau, dt, nsteps=1):
+        """nsteps of integrate::<tableau> (utils/integrate/src/lib.rs:176-222)
+        for any explicit tableau (summationbyparts_amd.integrate.Rk4, Rk4_38,
+        EulerMethod, MidpointMethod, Rk6, Fehlberg, BogackiShampine, or an
+        object with A, B, C in the reference's form)."""
+        from .integrate import tableau_arrays
+        ns, A, B, C = tableau_arrays(tableau)
+        dp = ctypes.POINTER(ctypes.c_double)
+        _lib.check(_lib.lib().sbp_system_integrate(self._h, ns, A.ctypes.data_as(dp), B.ctypes.data_as(dp),
+                                                   C.ctypes.data_as(dp), dt, nsteps), "sbp_system_integrate")
+
     def advance_timed(self, dt, nsteps):
         """advance_steps(dt, nsteps); returns the ms between HIP events
         recorded on the library's stream around the steps."""
@@ -310,6 +321,71 @@ def RHS_trad(op, k, y, metrics, boundaries):
     rhs(_lib.RHS_TRAD, op, k, y, metrics, boundaries)
 
 
+def _check_fields(*ts):
+    import torch
+
+    for t in ts:
+        if not (t.is_cuda and t.dtype == torch.float64 and t.is_contiguous()):
+            raise ValueError("need contiguous float64 CUDA tensors")
+
+
+def RHS_no_SAT(op, k, y, metrics):
+    """euler::RHS_no_SAT (lib.rs:518-568): the right-hand side without
+    boundary penalties; upwind dissipation iff op.upwind() is not None."""
+    _check_fields(k, y, metrics)
+    oe, ox = _ids(op)
+    _lib.wait_for_torch()
+    _lib.check(_lib.lib().sbp_rhs_no_sat(_lib.context(), oe, ox, y.shape[1], y.shape[2], ctypes.c_void_p(k.data_ptr()),
+                                         ctypes.c_void_p(y.data_ptr()), ctypes.c_void_p(metrics.data_ptr())),
+               "sbp_rhs_no_sat")
+
+
+def SAT_characteristics(op, k, y, metrics, boundaries):
+    """euler::SAT_characteristics (lib.rs:1002-1013): k += the SATs of the
+    north, south, east and west edges; boundaries = (north, south, east,
+    west) boundary data, each (4, n) contiguous."""
+    _check_fields(k, y, metrics, *boundaries)
+    oe, ox = _ids(op)
+    bp = (ctypes.c_void_p * 4)(*[b.data_ptr() for b in boundaries])
+    _lib.wait_for_torch()
+    _lib.check(_lib.lib().sbp_sat_characteristics(_lib.context(), oe, ox, y.shape[1], y.shape[2],
+                                                  ctypes.c_void_p(k.data_ptr()), ctypes.c_void_p(y.data_ptr()),
+                                                  ctypes.c_void_p(metrics.data_ptr()), bp), "sbp_sat_characteristics")
+
+
+def _sat_edge(direction):
+    def sat(op, k, y, metrics, boundary):
+        _check_fields(k, y, metrics, boundary)
+        oe, ox = _ids(op)
+        _lib.wait_for_torch()
+        _lib.check(_lib.lib().sbp_sat_edge(_lib.context(), direction, oe, ox, y.shape[1], y.shape[2],
+                                           ctypes.c_void_p(k.data_ptr()), ctypes.c_void_p(y.data_ptr()),
+                                           ctypes.c_void_p(metrics.data_ptr()), ctypes.c_void_p(boundary.data_ptr())),
+                   "sbp_sat_edge")
+    return sat
+
+
+SAT_north = _sat_edge(_lib.NORTH)
+SAT_south = _sat_edge(_lib.SOUTH)
+SAT_east = _sat_edge(_lib.EAST)
+SAT_west = _sat_edge(_lib.WEST)
+# euler::SAT_FUNCTIONS (lib.rs:1015-1023), keyed by direction
+SAT_FUNCTIONS = {"north": SAT_north, "south": SAT_south, "east": SAT_east, "west": SAT_west}
+
+
+def h2_err(a, b, op):
+    """Field::h2_err (lib.rs:377-432) of two (4, ny, nx) device fields."""
+    _check_fields(a, b)
+    if tuple(a.shape) != tuple(b.shape):
+        raise ValueError("h2_err: fields differ in shape")
+    oe, ox = _ids(op)
+    out = ctypes.c_double()
+    _lib.wait_for_torch()
+    _lib.check(_lib.lib().sbp_h2_err(_lib.context(), oe, ox, a.shape[1], a.shape[2], ctypes.c_void_p(a.data_ptr()),
+                                     ctypes.c_void_p(b.data_ptr()), ctypes.byref(out)), "sbp_h2_err")
+    return out.value
+
+
 def grid_metrics(op, x, y):
     """Grid::metrics (sbp/src/grid.rs:64-66) on device tensors: (5, ny, nx)."""
     import torch
@@ -324,4 +400,5 @@ def grid_metrics(op, x, y):
 
 
 __all__ = ["set_gamma", "gamma", "Vortice", "VortexParameters", "System", "BlockSystem", "RHS_upwind", "RHS_trad",
-           "grid_metrics", "OperatorPair"]
+           "RHS_no_SAT", "SAT_characteristics", "SAT_north", "SAT_south", "SAT_east", "SAT_west", "SAT_FUNCTIONS",
+           "h2_err", "grid_metrics", "OperatorPair"]

@@ -142,3 +142,23 @@ def test_interpolation_rejects_short_edges(iop):
         # lengths the operator does not couple are rejected either way
         assert L.sbp_interpolate(None, iop, 1, None, fine + 1, None, coarse) == _lib.ERR_SHAPE
     assert rejected > 0
+
+
+def test_h2linspace_host():
+    """sbp_h2linspace (a host function) against the reference's test
+    (sbp/src/utils.rs:142-155) and its formula written out (two roundings
+    per point, as Rust computes start + h·(i - 0.5))."""
+    from summationbyparts_amd import utils
+    x = utils.h2linspace(-1.0, 1.0, 50)
+    assert abs(x[0] + 1) < 1e-6 and abs(x[49] - 1) < 1e-6
+    hend, h = x[1] - x[0], x[2] - x[1]
+    assert abs(x[49] - x[48] - hend) < 1e-6 and abs(2 * hend - h) < 1e-6
+    assert np.all(np.abs(np.diff(x[1:49]) - h) < 1e-6)
+    for start, end, n in ((-1.0, 1.0, 50), (0.0, 5.0, 50), (-5.0, 0.0, 98), (0.3, 2.7, 2), (1.0, 1.5, 7)):
+        got = utils.h2linspace(start, end, n)
+        hh = (end - start) / (n - 2) if n > 2 else 0.0
+        expect = [start] + [start + hh * (i - 0.5) for i in range(1, n - 1)] + [end]
+        assert got.tolist() == expect, (start, end, n)
+    from summationbyparts_amd import _lib
+    with pytest.raises(_lib.SbpError):
+        utils.h2linspace(0.0, 1.0, 1)

@@ -524,3 +524,141 @@ def test_upwind_dissipation_weight_is_spectral_radius(ops):
     uh = jxx * q[1] / q[0] + jxy * q[2] / q[0]
     vh = jyx * q[1] / q[0] + jyy * q[2] / q[0]
     assert np.min(uh) < 0 < np.max(uh) and np.min(vh) < 0 < np.max(vh)
+
+
+# --- boundary entry points: RHS_no_SAT, SAT functions, explicit tableaus ------
+
+def test_rhs_no_sat_and_sat_functions_compose_the_rhs():
+    """RHS_no_SAT (euler/src/lib.rs:518-568) equals the full right-hand side
+    away from the block edges, and the four SAT functions applied to it in
+    the order of SAT_characteristics (north, south, east, west,
+    lib.rs:1002-1013) with the boundary data of boundary_extracts rebuild the
+    full right-hand side bit for bit (far-field vortex data on a curvilinear
+    upwind grid, so every edge has a penalty)."""
+    vp = O.VortexParams.make(list(VORTEX), MACH)
+    ny, nx = 31, 36
+    X, Y = curvilinear(nx, ny, amp=0.05)
+    S = O.System([(X, Y)], [(O.UPWIND4, O.UPWIND9)])
+    for d in range(4):
+        S.set_bc(0, d, O.BC_VORTEX, 0, vp)
+    q = _smooth_state(X, Y)
+    S.set_state(0, q)
+    t = 0.3
+    full = S.rhs(t)[0]
+    S.set_sat(False)
+    nosat = S.rhs(t)[0]
+    S.set_sat(True)
+    inner = (slice(None), slice(1, ny - 1), slice(1, nx - 1))
+    assert np.array_equal(full[inner], nosat[inner])
+    assert np.max(np.abs(full - nosat)) > 1e-3 * np.max(np.abs(full))
+    k = nosat
+    met = S.metrics(0)
+    for d in (O.NORTH, O.SOUTH, O.EAST, O.WEST):
+        k = O.sat_edge(O.UPWIND4, O.UPWIND9, d, k, q, met, S.edge(0, d, t))
+    assert np.array_equal(k, full)
+
+
+@pytest.mark.parametrize("edge", ["north", "south", "east", "west"])
+def test_sat_function_supersonic_closed_form(edge):
+    """One SAT function on arrays (SAT_north / south / east / west,
+    lib.rs:1025-1144) against the characteristic sum over exact flux-Jacobian
+    eigenvectors, as the system-level test above: uniform supersonic state,
+    far-field data from the vortex, inflow edge penalised, nothing else
+    touched."""
+    nx, ny = 29, 26
+    x = np.linspace(-5.0, 5.0, nx)
+    y = np.linspace(-2.0, 3.0, ny)
+    X, Y = np.meshgrid(x, y)
+    speed = 3.0
+    vel = {"north": (0.3, -speed), "south": (0.3, speed), "east": (-speed, 0.2), "west": (speed, 0.2)}[edge]
+    rho, p = 1.3, 1.0
+    qv = np.array([rho, rho * vel[0], rho * vel[1], p / (GAMMA - 1) + 0.5 * rho * (vel[0] ** 2 + vel[1] ** 2)])
+    q = np.broadcast_to(qv[:, None, None], (4, ny, nx)).copy()
+    d = {"north": O.NORTH, "south": O.SOUTH, "east": O.EAST, "west": O.WEST}[edge]
+    met = O.metrics(O.SBP8, O.UPWIND4, X, Y)
+    g_full = O.vortex(O.VortexParams.make(list(VORTEX), MACH), 0.0, X, Y)
+    if edge in ("north", "south"):
+        row = ny - 1 if edge == "north" else 0
+        g = g_full[:, row, :]
+        sl = (slice(None), row, slice(None))
+        n_a, h0, kx, ky = ny, O.op_h(O.SBP8)[0], 0.0, 1.0 / (y[1] - y[0]) / (ny - 1)
+        sigma, tau = (-1.0, 1.0) if edge == "north" else (1.0, -1.0)
+    else:
+        col = nx - 1 if edge == "east" else 0
+        g = g_full[:, :, col]
+        sl = (slice(None), slice(None), col)
+        n_a, h0, kx, ky = nx, O.op_h(O.UPWIND4)[0], 1.0 / (x[1] - x[0]) / (nx - 1), 0.0
+        sigma, tau = (-1.0, 1.0) if edge == "east" else (1.0, -1.0)
+    k = O.sat_edge(O.SBP8, O.UPWIND4, d, np.zeros_like(q), q, met, g)
+    hi = (n_a - 1) / h0
+    expect = np.stack([hi * tau * _characteristic_penalty(qv, qv - g[:, i], kx, ky, sigma) for i in range(g.shape[1])],
+                      axis=1)
+    assert np.max(np.abs(k[sl] - expect)) < 1e-11 * np.max(np.abs(expect))
+    rest = np.ones(k.shape, bool)
+    rest[sl] = False
+    assert not np.any(k[rest])
+
+
+# integrate/src/lib.rs:30-35, :38-43, :45-56, :138-151
+RK4_TAB = ([[0, 0, 0, 0], [0.5, 0, 0, 0], [0, 0.5, 0, 0], [0, 0, 1.0, 0]], [1 / 6, 1 / 3, 1 / 3, 1 / 6], [0.5, 0.5, 1.0])
+RK38_TAB = ([[0, 0, 0, 0], [1 / 3, 0, 0, 0], [-1 / 3, 1.0, 0, 0], [1.0, -1.0, 1.0, 0]], [1 / 8, 3 / 8, 3 / 8, 1 / 8],
+            [1 / 3, 2 / 3, 1.0])
+MID_TAB = ([[0, 0], [0.5, 0]], [0.0, 1.0], [0.5])
+EULER_TAB = ([[0.0]], [1.0], [])
+BS_TAB = ([[0, 0, 0, 0], [0.5, 0, 0, 0], [0, 0.75, 0, 0], [2 / 9, 1 / 3, 4 / 9, 0]], [2 / 9, 1 / 3, 4 / 9, 0.0],
+          [0.0, 0.5, 0.75])
+
+
+def _farfield_system(n=22):
+    vp = O.VortexParams.make(list(VORTEX), MACH)
+    X, Y = cartesian(n, n, -3.0, 3.0)
+    S = O.System([(X, Y)], [(O.UPWIND4, O.UPWIND9)])
+    for d in range(4):
+        S.set_bc(0, d, O.BC_VORTEX, 0, vp)
+    S.vortex(0.0, vp)
+    return S
+
+
+def test_integrate_tableau_rk4_and_bogacki_shampine_match_fused_steps():
+    """The generic tableau step with Rk4's arrays is the oracle's RK4 step
+    bit for bit (pinned as classical RK4 above), and with BogackiShampine's
+    arrays (C starting with its 0, integrate/src/lib.rs:150) the embedded
+    step's solution bit for bit."""
+    a, b = _farfield_system(), _farfield_system()
+    a.time = b.time = 0.2
+    a.advance(0.01, 3)
+    b.integrate(*RK4_TAB, 0.01, 3)
+    assert np.array_equal(a.get_state(0), b.get_state(0)) and a.time == b.time
+    a, b = _farfield_system(), _farfield_system()
+    a.embedded_step(0.0, 0.01)
+    b.integrate(*BS_TAB, 0.01, 1)
+    assert np.array_equal(a.get_state(0), b.get_state(0))
+
+
+@pytest.mark.parametrize("name", ["rk4_38", "midpoint", "euler"])
+def test_integrate_tableau_composed_from_rhs(name):
+    """Rk4_38, MidpointMethod and EulerMethod (integrate/src/lib.rs:37-56):
+    one step equals the method composed by hand from the oracle's RHS at the
+    reference's stage times t + dt·C[i-1] (far-field vortex data, so a wrong
+    stage time fails), with the reference's order of scaled_adds."""
+    A, B, C = {"rk4_38": RK38_TAB, "midpoint": MID_TAB, "euler": EULER_TAB}[name]
+    S = _farfield_system()
+    t0, dt = 0.25, 0.01
+    q0 = S.get_state(0)
+    ks = []
+    for i in range(len(B)):
+        u = q0.copy()
+        for j in range(i):
+            if A[i][j] != 0.0:
+                u = u + (A[i][j] * dt) * ks[j]
+        S.set_state(0, u)
+        ks.append(S.rhs(t0 + (dt * C[i - 1] if i else 0.0))[0])
+    expect = q0.copy()
+    for j in range(len(B)):
+        if B[j] != 0.0:
+            expect = expect + (B[j] * dt) * ks[j]
+    S.set_state(0, q0)
+    S.time = t0
+    S.integrate(A, B, C, dt, 1)
+    assert np.array_equal(S.get_state(0), expect)
+    assert S.time == t0 + dt

@@ -32,7 +32,8 @@ typedef enum {
   SBP_ERR_HIP = 8,         /* HIP runtime failure */
   SBP_ERR_NOT_UPWIND = 9,  /* "This is not an upwind operator" (euler/src/lib.rs:597) */
   SBP_ERR_OOM = 10,        /* device allocation failed */
-  SBP_ERR_GAMMA_SET = 11   /* GAMMA already set to another value (OnceCell::set, euler/src/lib.rs:15) */
+  SBP_ERR_GAMMA_SET = 11,  /* GAMMA already set to another value (OnceCell::set, euler/src/lib.rs:15) */
+  SBP_ERR_SIZE = 12        /* a size beyond the entry point's stated maximum */
 } sbp_status;
 
 /* sbp/src/operators.rs:152-165 */
@@ -175,8 +176,19 @@ int sbp_sat_edge(sbp_context ctx, int dir, int op_eta, int op_xi, int ny, int nx
  * device fields: sqrt(Σ_c Σ_ij hx_i hy_j (a - b)²) with the operators' H
  * weights, summed compensated in FP64 on the device; *out is a host double.
  * Allocates (and frees) a small device buffer of partial sums per call.
- * SBP_ERR_SHAPE when n < 2M along an axis; at most 2^31 - 1 points. */
+ * SBP_ERR_SHAPE when n < 2M along an axis; SBP_ERR_SIZE above 2^31 - 1
+ * points (both checked before the handle). */
 int sbp_h2_err(sbp_context ctx, int op_eta, int op_xi, int ny, int nx, const double* a, const double* b, double* out);
+/* sbp_h2_err without any allocation: no counterpart in the reference; for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc). The same kernels
+ * run on a library-owned static device array of 16 KiB per GPU (1024
+ * per-workgroup partial sums and the result), which serves every size up to
+ * the same maximum of 2^31 - 1 points (SBP_ERR_SIZE beyond it). Calls that
+ * overlap on one GPU (other threads or contexts) share that array and race;
+ * sequential calls need no clearing between them. */
+int sbp_h2_err_noalloc(sbp_context ctx, int op_eta, int op_xi, int ny, int nx, const double* a, const double* b,
+                       double* out);
 
 /* sbp::utils::h2linspace (sbp/src/utils.rs:133-140): n >= 2 host values,
  * start, start + h/2, start + 3h/2, ..., end with h = (end - start)/(n - 2). */

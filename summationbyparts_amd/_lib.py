@@ -11,7 +11,7 @@ LIB_PATH = os.path.join(_HERE, "libsbpgpu.so")
 
 # sbp_status
 OK, ERR_INVALID, ERR_SHAPE, ERR_DETJ, ERR_RHO, ERR_PRESSURE, ERR_GAMMA, ERR_EDGE, ERR_HIP, ERR_NOT_UPWIND, ERR_OOM, \
-    ERR_GAMMA_SET = range(12)
+    ERR_GAMMA_SET, ERR_SIZE = range(13)
 # sbp_operator
 UPWIND4, UPWIND4H2, UPWIND9, UPWIND9H2, SBP4, SBP8 = range(6)
 DIFF, DISS = 0, 1
@@ -116,6 +116,7 @@ def _declare(L):
         "sbp_sat_edge": ([c_void_p, c_int, c_int, c_int, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p], c_int),
         "sbp_h2_err": ([c_void_p, c_int, c_int, c_int, c_int, c_void_p, c_void_p, dp], c_int),
         "sbp_h2linspace": ([c_double, c_double, c_int, dp], c_int),
+        "sbp_h2_err_noalloc": ([c_void_p, c_int, c_int, c_int, c_int, c_void_p, c_void_p, dp], c_int),
         "sbp_system_integrate": ([c_void_p, c_int, dp, dp, dp, c_double, c_int64], c_int),
         "sbp_tile_shape_count": ([], c_int),
         "sbp_tile_shape_get": ([c_int, ip, ip, ip], c_int),
@@ -142,7 +143,7 @@ EXPORTED = [
     "sbp_tile_shape_count",
     "sbp_tile_shape_get",
     "sbp_rhs_no_sat", "sbp_sat_characteristics", "sbp_sat_edge", "sbp_h2_err", "sbp_h2linspace",
-    "sbp_system_integrate",
+    "sbp_system_integrate", "sbp_h2_err_noalloc",
 ]
 
 

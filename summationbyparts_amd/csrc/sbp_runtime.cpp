@@ -150,8 +150,37 @@ int64_t pitch_of(int nx) { return ((int64_t)nx + 15) / 16 * 16; }
 
 }  // namespace
 
+// Device scratch of sbp_h2_err_noalloc: the per-workgroup partial sums and
+// the result pair of one H-norm reduction (kReduceScratch doubles, 16 KiB per
+// device, held until the process ends). In host code `scratch` is only a
+// stand-in for the symbol: its device address comes from
+// hipGetSymbolAddress, once per device (noalloc_scratch_of), and only that
+// address is passed to kernels and copies.
+static __device__ __align__(256) double scratch[kReduceScratch];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[kMaxDevices] = {};
+
+// The device address of `scratch` on `device` (current), looked up once.
+hipError_t noalloc_scratch_of(int device, double** out) {
+  if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[device]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    g_scratch_ptr[device] = static_cast<double*>(p);
+  }
+  *out = g_scratch_ptr[device];
+  return hipSuccess;
+}
+}  // namespace
+
 struct sbp_context_s {
   int device = 0;
+  double* noalloc_scratch = nullptr;  // this device's `scratch` (sbp_h2_err_noalloc)
   hipStream_t stream = nullptr;
   bool own_stream = false;
   // second stream of the context (default priority): the edge tiles of a
@@ -717,6 +746,7 @@ const char* sbp_status_string(int status) {
     case SBP_ERR_NOT_UPWIND: return "This is not an upwind operator";
     case SBP_ERR_OOM: return "device out of memory";
     case SBP_ERR_GAMMA_SET: return "GAMMA already set";
+    case SBP_ERR_SIZE: return "size beyond the entry point's maximum";
     default: return "unknown status";
   }
 }
@@ -789,7 +819,8 @@ int sbp_context_create(int device, sbp_context* out) {
       (e = hipMalloc(&c->d_tmpblock, sizeof(BlockDev))) != hipSuccess ||
       (e = hipStreamCreateWithFlags(&c->aux, hipStreamNonBlocking)) != hipSuccess ||
       (e = hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming)) != hipSuccess ||
-      (e = hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming)) != hipSuccess) {
+      (e = hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming)) != hipSuccess ||
+      (e = noalloc_scratch_of(device, &c->noalloc_scratch)) != hipSuccess) {
     if (c->aux) hipStreamDestroy(c->aux);
     if (c->ev_fork) hipEventDestroy(c->ev_fork);
     hipStreamDestroy(c->stream);
@@ -1089,14 +1120,15 @@ static int h2_err_with(sbp_context ctx, int op_eta, int op_xi, int ny, int nx, c
   return SBP_OK;
 }
 
+// Sizes first (host-only checks, also without a context), then handles.
 static int h2_err_args(sbp_context ctx, int op_eta, int op_xi, int ny, int nx, const double* a, const double* b,
                        double* out) {
-  if (!ctx || !a || !b || !out) return SBP_ERR_INVALID;
   OpInfo oe, ox;
   if (!op_info(op_eta, oe) || !op_info(op_xi, ox)) return SBP_ERR_INVALID;
+  if (nx <= 0 || ny <= 0 || (int64_t)nx * ny > (int64_t)INT32_MAX) return SBP_ERR_SIZE;
   // the weights [H, 1.., rev(H)] need n >= 2M (lib.rs:389-395)
   if (nx < 2 * ox.M || ny < 2 * oe.M) return SBP_ERR_SHAPE;
-  if ((int64_t)nx * ny > (int64_t)INT32_MAX) return SBP_ERR_INVALID;
+  if (!ctx || !a || !b || !out) return SBP_ERR_INVALID;
   return SBP_OK;
 }
 
@@ -1108,6 +1140,16 @@ int sbp_h2_err(sbp_context ctx, int op_eta, int op_xi, int ny, int nx, const dou
   const int rc = h2_err_with(ctx, op_eta, op_xi, ny, nx, a, b, scratch, out);
   hipFreeAsync(scratch, ctx->stream);
   return rc;
+}
+
+int sbp_h2_err_noalloc(sbp_context ctx, int op_eta, int op_xi, int ny, int nx, const double* a, const double* b,
+                       double* out) {
+  SBP_TRY(h2_err_args(ctx, op_eta, op_xi, ny, nx, a, b, out));
+  if (!ctx->noalloc_scratch) return SBP_ERR_INVALID;
+  SBP_TRY_HIP(hipSetDevice(ctx->device));
+  // every partial the finish pass reads is written by this call's partial
+  // pass first (stream order), so nothing in the scratch needs clearing
+  return h2_err_with(ctx, op_eta, op_xi, ny, nx, a, b, ctx->noalloc_scratch, out);
 }
 
 int sbp_h2linspace(double start, double end, int n, double* out) {

@@ -162,3 +162,16 @@ def test_h2linspace_host():
     from summationbyparts_amd import _lib
     with pytest.raises(_lib.SbpError):
         utils.h2linspace(0.0, 1.0, 1)
+
+
+def test_h2_err_size_checks_before_handles():
+    """sbp_h2_err and sbp_h2_err_noalloc check their sizes on the host before
+    the handle: beyond 2^31 - 1 points SBP_ERR_SIZE, below the closures
+    SBP_ERR_SHAPE, and only then a missing context (SBP_ERR_INVALID)."""
+    from summationbyparts_amd import _lib
+    L = _lib.lib()
+    for fn in (L.sbp_h2_err, L.sbp_h2_err_noalloc):
+        assert fn(None, _lib.UPWIND9, _lib.UPWIND9, 65536, 65536, None, None, None) == _lib.ERR_SIZE
+        assert fn(None, _lib.UPWIND9, _lib.UPWIND9, 10, 40, None, None, None) == _lib.ERR_SHAPE
+        assert fn(None, _lib.UPWIND9, _lib.UPWIND9, 40, 40, None, None, None) == _lib.ERR_INVALID
+        assert fn(None, 17, _lib.UPWIND9, 40, 40, None, None, None) == _lib.ERR_INVALID

@@ -33,7 +33,9 @@ typedef enum {
   SBP_ERR_NOT_UPWIND = 9,  /* "This is not an upwind operator" (euler/src/lib.rs:597) */
   SBP_ERR_OOM = 10,        /* device allocation failed */
   SBP_ERR_GAMMA_SET = 11,  /* GAMMA already set to another value (OnceCell::set, euler/src/lib.rs:15) */
-  SBP_ERR_SIZE = 12        /* a size beyond the entry point's stated maximum */
+  SBP_ERR_SIZE = 12,       /* a size beyond the entry point's stated maximum */
+  SBP_ERR_EXPR = 13        /* an expression does not compile or fails to evaluate (evalexpr errors,
+                              multigrid/src/eval/evalexpr.rs: .unwrap() on build and eval) */
 } sbp_status;
 
 /* sbp/src/operators.rs:152-165 */
@@ -59,7 +61,8 @@ typedef enum {
   SBP_BC_GRID = 1,
   SBP_BC_VORTEX = 2,
   SBP_BC_INTERPOLATE = 3,
-  SBP_BC_MULTIGRID = 4
+  SBP_BC_MULTIGRID = 4,
+  SBP_BC_EVAL = 5   /* BoundaryCharacteristic::Eval (euler/src/lib.rs:741): set with sbp_system_set_bc_eval */
 } sbp_bc_kind;
 
 /* 2:1 interface interpolation operators (sbp/src/operators/interpolation.rs:1-13) */

@@ -5,6 +5,9 @@
 #include "../../include/sbpgpu.h"
 
 namespace sbp {
+namespace expr {
+struct Program;
+}
 
 // Work sets of the Runge-Kutta paths that keep their stage derivatives.
 constexpr int kMaxWorkSets = 10;
@@ -28,6 +31,7 @@ enum EdgeSrc : int {
   EDGE_VORTEX = SBP_BC_VORTEX,  // analytic vortex on this edge's coordinates at stage time
   EDGE_INTERP = SBP_BC_INTERPOLATE,  // neighbour's opposite edge through a 2:1 interpolation operator
   EDGE_MULTIGRID = SBP_BC_MULTIGRID,  // pieces of several neighbours' opposite edges
+  EDGE_EVAL = SBP_BC_EVAL,      // expressions of (t, x, y) on this edge's coordinates at stage time
   EDGE_EXPLICIT = 100,          // caller-provided 4 x n array (operator-level rhs)
   EDGE_NONE = 101,              // no SAT on this edge (RHS_no_SAT)
 };
@@ -70,15 +74,20 @@ struct BlockDev {
   const double* edge_ptr[4];  // EDGE_EXPLICIT data
   int64_t edge_cs[4];         // component stride of EDGE_EXPLICIT data
   VortexDev vortex[4];        // EDGE_VORTEX parameters
-  double* vedge[4];           // EDGE_VORTEX data at the current stage time: 4 x edge length, component-major
+  const expr::Program* eprog[4];  // EDGE_EVAL program (device copy)
+  double* edata[4];               // EDGE_EVAL data at the current stage time (4 x edge length)
+  // characteristic SAT of each edge at the current stage (4 x edge length,
+  // component-major), written by the edge-SAT kernel before the edge tiles
+  // add it to k
+  double* sat[4];
   int edge_iop[4];            // EDGE_INTERP: interpolation operator (InterpId)
   int edge_f2c[4];            // EDGE_INTERP: 1 = fine to coarse
   const int* edge_pieces[4];  // EDGE_MULTIGRID: (block, start, end) triples
   int edge_npieces[4];
 };
 
-// One far-field (EDGE_VORTEX) edge whose boundary data a stage precomputes.
-struct VortexSeg {
+// One block edge whose SAT the edge-SAT kernel evaluates at every stage.
+struct SatSeg {
   int32_t block, dir;
 };
 

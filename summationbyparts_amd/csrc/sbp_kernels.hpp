@@ -31,8 +31,11 @@ int stage_threads(int shape);
 // Pointwise metrics from the four derivative planes (grid.rs:110-131).
 hipError_t launch_metrics_finish(const double* dx_dxi, const double* dx_deta, const double* dy_dxi, const double* dy_deta,
                                  double* metrics, int64_t ps, int64_t ld, int ny, int nx, unsigned* err, hipStream_t stream);
-// Far-field boundary data of the listed vortex edges at the stage time of `a`.
-hipError_t launch_vortex_edges(const StageArgs& a, const VortexSeg* segs, int nsegs, int maxlen, hipStream_t stream);
+// Characteristic SATs of the listed edges at stage `mode` into BlockDev::sat
+// (maxlen: the longest listed edge).
+hipError_t launch_edge_sat(const StageArgs& a, const SatSeg* segs, int nsegs, int maxlen, int mode, hipStream_t stream);
+// Boundary data of the listed Eval edges at the stage time of `a` into BlockDev::edata.
+hipError_t launch_eval_edges(const StageArgs& a, const SatSeg* segs, int nsegs, int maxlen, hipStream_t stream);
 // Metric pairs (BlockDev::mpair, 4·ps doubles) from the 5 metric planes.
 hipError_t launch_metric_pairs(const double* metrics, int64_t ps, int64_t ld, int ny, int nx, double* pairs,
                                hipStream_t stream);

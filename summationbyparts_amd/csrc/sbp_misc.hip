@@ -119,34 +119,77 @@ hipError_t launch_metric_pairs(const double* metrics, int64_t ps, int64_t ld, in
   return hipGetLastError();
 }
 
-// Far-field (Vortex) boundary data of every vortex edge of a system at the
-// stage's time (t + c_s·dt, times the block's step scale), evaluated once per
-// stage before the edge tiles read it (BlockDev::vedge; boundary_extracts'
-// Vortex case, euler/src/lib.rs:983-998): one y-block of the grid per edge.
-__global__ __launch_bounds__(256) void vortex_edges_kernel(const StageArgs A, const VortexSeg* __restrict__ segs) {
-  const VortexSeg sg = segs[blockIdx.y];
+// Characteristic SATs of the listed block edges at one stage (SAT_north /
+// south / east / west on boundary_extracts' data, euler/src/lib.rs:1002-1144
+// and :869-915), one thread per edge point: B.sat[dir] = hi·τ·T((L + σ|L|)∘
+// (TI(q - g))) with q the stage input at the edge point and g its boundary
+// data (a neighbour's edge, interpolated or pieced together, or far-field
+// data evaluated here at the stage time). The stage kernel's edge tiles add
+// them to k; keeping the 4x4 characteristic algebra in this small kernel
+// keeps it out of the stage kernel's registers. Grid: (segment, points / 256).
+__global__ __launch_bounds__(256) void edge_sat_kernel(const StageArgs A, const SatSeg* __restrict__ segs, const int mode) {
+  const SatSeg sg = segs[blockIdx.x];
   const BlockDev& B = A.blocks[sg.block];
   const int dir = sg.dir;
   const bool ns = dir == SBP_NORTH || dir == SBP_SOUTH;
   const int len = ns ? B.nx : B.ny;
-  const int t = blockIdx.x * 256 + threadIdx.x;
+  const int t = blockIdx.y * 256 + threadIdx.x;
   if (t >= len) return;
   const int x = ns ? t : (dir == SBP_EAST ? B.nx - 1 : 0);
   const int y = ns ? (dir == SBP_NORTH ? B.ny - 1 : 0) : t;
-  const int64_t o = (int64_t)y * B.ld + x;
-  const double time = ((A.time ? *A.time : 0.0) + A.time_offset) * B.dt_scale;
-  double g[4] = {0.0, 0.0, 0.0, 0.0};
-  unsigned e = 0;
-  vortex_point(B.vortex[dir], A.gamma, time, B.xc[o], B.yc[o], g, e);
-  if (e) record_error(A.error, e);
-  double* out = B.vedge[dir];
+  const uint32_t o = (uint32_t)(y * B.ld + x);
+  const uint32_t ps = (uint32_t)B.ps;
+  unsigned err = 0;
+  double q[4], g[4];
+  load_stage_state(A, B, mode, o, q);
+  edge_data(A, B, dir, t, o, mode, g, err);
+  // (σ, τ): north (-1, +1), south (+1, -1), east (-1, +1), west (+1, -1);
+  // north/south use J·∇η, east/west J·∇ξ
+  const double sign = (dir == SBP_NORTH || dir == SBP_EAST) ? -1.0 : 1.0;
+  const int m0 = ns ? 3 : 1;
+  double sat[4] = {0.0, 0.0, 0.0, 0.0};
+  sat_point(sat, q, g, ns ? B.hi_eta : B.hi_xi, sign, -sign, ldg(B.metrics, o), ldg(B.metrics + m0 * ps, o),
+            ldg(B.metrics + (m0 + 1) * ps, o), A.gamma);
+  double* out = B.sat[dir];
 #pragma unroll
-  for (int c = 0; c < 4; ++c) out[c * len + t] = g[c];
+  for (int c = 0; c < 4; ++c) out[c * len + t] = sat[c];
+  if (err) record_error(A.error, err);
 }
 
-hipError_t launch_vortex_edges(const StageArgs& a, const VortexSeg* segs, int nsegs, int maxlen, hipStream_t stream) {
-  dim3 grid((maxlen + 255) / 256, nsegs);
-  vortex_edges_kernel<<<grid, 256, 0, stream>>>(a, segs);
+// Boundary data of the listed Eval edges at the stage time of `a` into
+// BlockDev::edata, before the edge-SAT kernel reads it: the expression
+// interpreter's stack lives in this kernel's registers and scratch, not in
+// the SAT kernel's. Grid: (segment, points / 256).
+__global__ __launch_bounds__(256) void eval_edges_kernel(const StageArgs A, const SatSeg* __restrict__ segs) {
+  const SatSeg sg = segs[blockIdx.x];
+  const BlockDev& B = A.blocks[sg.block];
+  const int dir = sg.dir;
+  const bool ns = dir == SBP_NORTH || dir == SBP_SOUTH;
+  const int len = ns ? B.nx : B.ny;
+  const int t = blockIdx.y * 256 + threadIdx.x;
+  if (t >= len) return;
+  const int x = ns ? t : (dir == SBP_EAST ? B.nx - 1 : 0);
+  const int y = ns ? (dir == SBP_NORTH ? B.ny - 1 : 0) : t;
+  unsigned err = 0;
+  double g[4];
+  eval_edge_point(A, B, dir, (uint32_t)(y * B.ld + x), g, err);
+  double* out = B.edata[dir];
+#pragma unroll
+  for (int c = 0; c < 4; ++c) out[c * len + t] = g[c];
+  if (err) record_error(A.error, err);
+}
+
+hipError_t launch_eval_edges(const StageArgs& a, const SatSeg* segs, int nsegs, int maxlen, hipStream_t stream) {
+  if (nsegs <= 0) return hipSuccess;
+  dim3 grid(nsegs, (maxlen + 255) / 256);
+  eval_edges_kernel<<<grid, 256, 0, stream>>>(a, segs);
+  return hipGetLastError();
+}
+
+hipError_t launch_edge_sat(const StageArgs& a, const SatSeg* segs, int nsegs, int maxlen, int mode, hipStream_t stream) {
+  if (nsegs <= 0) return hipSuccess;
+  dim3 grid(nsegs, (maxlen + 255) / 256);
+  edge_sat_kernel<<<grid, 256, 0, stream>>>(a, segs, mode);
   return hipGetLastError();
 }
 

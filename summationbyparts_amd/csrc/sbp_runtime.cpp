@@ -214,6 +214,7 @@ struct HostBlock {
   int bc_iop[4] = {0, 0, 0, 0};   // SBP_BC_INTERPOLATE operator
   std::vector<int> bc_pieces[4];  // SBP_BC_MULTIGRID (block, start, end) triples
   sbp_vortex_params bc_vortex[4];
+  std::shared_ptr<const std::vector<unsigned char>> bc_prog[4];  // SBP_BC_EVAL program blobs
 };
 
 struct Group {
@@ -223,6 +224,8 @@ struct Group {
   int shape = -1;
   int tile_offset = 0, ntiles = 0;    // interior tiles
   int etile_offset = 0, entiles = 0;  // tiles on block edges (they add the edge SATs)
+  int sat_off = 0, nsat = 0, sat_maxlen = 0;  // the group's block edges for the edge-SAT kernel
+  int eval_off = 0, neval = 0, eval_maxlen = 0;  // the group's Eval edges (same list, after the SAT edges)
 };
 
 }  // namespace
@@ -234,11 +237,12 @@ struct sbp_system_s {
   BlockDev* d_blocks = nullptr;
   TileDev* d_tiles = nullptr;
   int* d_pieces = nullptr;  // MultiGrid piece lists of all edges
-  // far-field (Vortex) edges: boundary data buffers and the edge list the
-  // per-stage vortex_edges_kernel walks
-  double* d_vedge = nullptr;
-  VortexSeg* d_vsegs = nullptr;
-  int n_vsegs = 0, max_vlen = 0;
+  // per-edge SAT buffers (BlockDev::sat) and the edge list of the edge-SAT
+  // kernel, grouped by operator group (Group::sat_off / nsat)
+  double* d_sat = nullptr;
+  SatSeg* d_satsegs = nullptr;
+  // device copies of the Eval edges' expression programs (BlockDev::eprog)
+  unsigned char* d_progs = nullptr;
   double* d_time = nullptr;      // system time
   double* d_time_rhs = nullptr;  // explicit time for sbp_system_rhs
   unsigned* d_err = nullptr;
@@ -402,19 +406,20 @@ int normalize(sbp_system_s* s) {
   return SBP_OK;
 }
 
-// One stage of one group: its edge tiles (with the edge SATs) on the
-// context's second stream and its interior tiles on `st`, concurrently; `st`
-// continues once both are done (also under graph capture: a fork and a join).
+// One stage of one group: the SATs of its block edges and then its edge
+// tiles on the context's second stream, its interior tiles on `st`,
+// concurrently; `st` continues once both are done (also under graph
+// capture: a fork and a join).
 hipError_t launch_group(const sbp_system_s* s, const Group& g, int shape, int mode, const StageArgs& a, hipStream_t st) {
   const sbp_context_s* c = s->ctx;
   hipError_t e;
-  // far-field boundary data of this stage's time, read by the edge tiles
-  if (s->n_vsegs && (e = launch_vortex_edges(a, s->d_vsegs, s->n_vsegs, s->max_vlen, st)) != hipSuccess) return e;
   if ((e = hipEventRecord(c->ev_fork, st)) != hipSuccess) return e;
   if ((e = hipStreamWaitEvent(c->aux, c->ev_fork, 0)) != hipSuccess) return e;
+  hipError_t ee = launch_eval_edges(a, s->d_satsegs + g.eval_off, g.neval, g.eval_maxlen, c->aux);
+  if (ee == hipSuccess) ee = launch_edge_sat(a, s->d_satsegs + g.sat_off, g.nsat, g.sat_maxlen, mode, c->aux);
   StageArgs ae = a;
   ae.tiles = s->d_tiles + g.etile_offset;
-  const hipError_t ee = launch_stage(g.op_eta, g.op_xi, g.upwind, true, shape, mode, ae, g.entiles, c->aux);
+  if (ee == hipSuccess) ee = launch_stage(g.op_eta, g.op_xi, g.upwind, true, shape, mode, ae, g.entiles, c->aux);
   StageArgs ai = a;
   ai.tiles = s->d_tiles + g.tile_offset;
   const hipError_t ei = ee == hipSuccess ? launch_stage(g.op_eta, g.op_xi, g.upwind, false, shape, mode, ai, g.ntiles, st) : ee;
@@ -475,33 +480,74 @@ int rebuild(sbp_system_s* s) {
     if (s->d_blocks) hipFree(s->d_blocks);
     if (s->d_tiles) hipFree(s->d_tiles);
     if (s->d_pieces) hipFree(s->d_pieces);
-    if (s->d_vedge) hipFree(s->d_vedge);
-    if (s->d_vsegs) hipFree(s->d_vsegs);
+    if (s->d_sat) hipFree(s->d_sat);
+    if (s->d_satsegs) hipFree(s->d_satsegs);
+    if (s->d_progs) hipFree(s->d_progs);
     s->d_blocks = nullptr;
     s->d_tiles = nullptr;
     s->d_pieces = nullptr;
-    s->d_vedge = nullptr;
-    s->d_vsegs = nullptr;
-    s->n_vsegs = s->max_vlen = 0;
-    // far-field edges: 4 x edge length doubles each in one device array
-    std::vector<VortexSeg> vsegs;
-    std::vector<size_t> voff;
-    size_t vtotal = 0;
+    s->d_sat = nullptr;
+    s->d_satsegs = nullptr;
+    s->d_progs = nullptr;
+    // SAT buffers: 4 x edge length doubles per block edge in one device
+    // array; the edge list in group order
+    std::vector<SatSeg> segs;
+    std::vector<size_t> soff;
+    size_t stotal = 0;
+    for (auto& g : s->groups) {
+      g.sat_off = (int)segs.size();
+      g.sat_maxlen = 0;
+      for (int bi : g.blocks)
+        for (int d = 0; d < 4; ++d) {
+          const int len = (d == SBP_NORTH || d == SBP_SOUTH) ? s->blocks[bi].nx : s->blocks[bi].ny;
+          segs.push_back({(int32_t)bi, d});
+          soff.push_back(stotal);
+          stotal += 4 * (size_t)len;
+          g.sat_maxlen = std::max(g.sat_maxlen, len);
+        }
+      g.nsat = (int)segs.size() - g.sat_off;
+    }
+    // Eval edges of each group, after all SAT edges, with their data buffers
+    // after the SAT buffers
+    for (auto& g : s->groups) {
+      g.eval_off = (int)segs.size();
+      g.eval_maxlen = 0;
+      for (int bi : g.blocks)
+        for (int d = 0; d < 4; ++d)
+          if (s->blocks[bi].bc_kind[d] == SBP_BC_EVAL) {
+            const int len = (d == SBP_NORTH || d == SBP_SOUTH) ? s->blocks[bi].nx : s->blocks[bi].ny;
+            segs.push_back({(int32_t)bi, d});
+            soff.push_back(stotal);
+            stotal += 4 * (size_t)len;
+            g.eval_maxlen = std::max(g.eval_maxlen, len);
+          }
+      g.neval = (int)segs.size() - g.eval_off;
+    }
+    SBP_TRY_HIP(hipMalloc(&s->d_sat, sizeof(double) * std::max<size_t>(stotal, 1)));
+    SBP_TRY_HIP(hipMalloc(&s->d_satsegs, sizeof(SatSeg) * std::max<size_t>(segs.size(), 1)));
+    SBP_TRY_HIP(hipMemsetAsync(s->d_sat, 0, sizeof(double) * std::max<size_t>(stotal, 1), st));
+    SBP_TRY_HIP(hipMemcpyAsync(s->d_satsegs, segs.data(), sizeof(SatSeg) * segs.size(), hipMemcpyHostToDevice, st));
+    for (auto& g : s->groups) {
+      for (int k = g.sat_off; k < g.sat_off + g.nsat; ++k) devs[segs[k].block].sat[segs[k].dir] = s->d_sat + soff[k];
+      for (int k = g.eval_off; k < g.eval_off + g.neval; ++k)
+        devs[segs[k].block].edata[segs[k].dir] = s->d_sat + soff[k];
+    }
+    // Eval edges: their programs, 16-byte aligned, in one device array
+    std::vector<unsigned char> progs;
+    std::vector<std::pair<size_t, size_t>> pref;  // (block*4 + dir, offset)
     for (size_t bi = 0; bi < s->blocks.size(); ++bi)
       for (int d = 0; d < 4; ++d)
-        if (s->blocks[bi].bc_kind[d] == SBP_BC_VORTEX) {
-          const int len = (d == SBP_NORTH || d == SBP_SOUTH) ? s->blocks[bi].nx : s->blocks[bi].ny;
-          vsegs.push_back({(int32_t)bi, d});
-          voff.push_back(vtotal);
-          vtotal += 4 * (size_t)len;
-          s->max_vlen = std::max(s->max_vlen, len);
+        if (s->blocks[bi].bc_kind[d] == SBP_BC_EVAL) {
+          const auto& blob = *s->blocks[bi].bc_prog[d];
+          pref.push_back({4 * bi + d, progs.size()});
+          progs.insert(progs.end(), blob.begin(), blob.end());
+          progs.resize((progs.size() + 15) / 16 * 16, 0);
         }
-    if (!vsegs.empty()) {
-      SBP_TRY_HIP(hipMalloc(&s->d_vedge, sizeof(double) * vtotal));
-      SBP_TRY_HIP(hipMalloc(&s->d_vsegs, sizeof(VortexSeg) * vsegs.size()));
-      SBP_TRY_HIP(hipMemcpyAsync(s->d_vsegs, vsegs.data(), sizeof(VortexSeg) * vsegs.size(), hipMemcpyHostToDevice, st));
-      for (size_t k = 0; k < vsegs.size(); ++k) devs[vsegs[k].block].vedge[vsegs[k].dir] = s->d_vedge + voff[k];
-      s->n_vsegs = (int)vsegs.size();
+    if (!progs.empty()) {
+      SBP_TRY_HIP(hipMalloc(&s->d_progs, progs.size()));
+      SBP_TRY_HIP(hipMemcpyAsync(s->d_progs, progs.data(), progs.size(), hipMemcpyHostToDevice, st));
+      for (auto& pr : pref)
+        devs[pr.first / 4].eprog[pr.first % 4] = reinterpret_cast<const expr::Program*>(s->d_progs + pr.second);
     }
     // MultiGrid piece lists: one device array, each edge pointing into it
     std::vector<int> pieces;
@@ -1005,6 +1051,9 @@ static int rhs_block(sbp_context ctx, bool up, int op_eta, int op_xi, int ny, in
     b.edge_ptr[d] = boundaries ? boundaries[d] : nullptr;
     b.edge_cs[d] = (d == SBP_NORTH || d == SBP_SOUTH) ? nx : ny;
   }
+  // the SATs of the four edges land in a per-call buffer after the metric
+  // pairs (4 x (2 nx + 2 ny) doubles)
+  const int64_t nsat = boundaries ? 4 * (2 * (int64_t)nx + 2 * (int64_t)ny) : 0;
   std::vector<TileDev> tiles;
   for (int ty = 0; ty < b.nty; ++ty)
     for (int tx = 0; tx < b.ntx; ++tx) tiles.push_back({0, (int16_t)tx, (int16_t)ty});
@@ -1012,13 +1061,23 @@ static int rhs_block(sbp_context ctx, bool up, int op_eta, int op_xi, int ny, in
   SBP_TRY_HIP(hipStreamSynchronize(st));
   // the stage kernel reads the metric pairs; build them from the caller's planes
   double* pairs = nullptr;
-  SBP_TRY_HIP(hipMalloc((void**)&pairs, sizeof(double) * 4 * b.ps));
+  SBP_TRY_HIP(hipMalloc((void**)&pairs, sizeof(double) * (4 * b.ps + nsat) + sizeof(SatSeg) * 4));
   struct FreeOnExit {
     double* p;
     ~FreeOnExit() { hipFree(p); }
   } free_pairs{pairs};
   SBP_TRY_HIP(launch_metric_pairs(metrics, b.ps, b.ld, ny, nx, pairs, st));
   b.mpair = pairs;
+  SatSeg* d_segs = reinterpret_cast<SatSeg*>(pairs + 4 * b.ps + nsat);
+  if (boundaries) {
+    double* sat = pairs + 4 * b.ps;
+    for (int d = 0; d < 4; ++d) {
+      b.sat[d] = sat;
+      sat += 4 * (int64_t)((d == SBP_NORTH || d == SBP_SOUTH) ? nx : ny);
+    }
+    const SatSeg segs[4] = {{0, SBP_NORTH}, {0, SBP_SOUTH}, {0, SBP_EAST}, {0, SBP_WEST}};
+    SBP_TRY_HIP(hipMemcpy(d_segs, segs, sizeof(segs), hipMemcpyHostToDevice));
+  }
   if (ctx->tmptiles_cap < (int)tiles.size()) {
     if (ctx->d_tmptiles) hipFree(ctx->d_tmptiles);
     ctx->d_tmptiles = nullptr;
@@ -1033,6 +1092,7 @@ static int rhs_block(sbp_context ctx, bool up, int op_eta, int op_xi, int ny, in
   a.time = nullptr;
   a.error = ctx->d_err;
   a.gamma = gamma;
+  if (boundaries) SBP_TRY_HIP(launch_edge_sat(a, d_segs, 4, std::max(nx, ny), MODE_RHS, st));
   SBP_TRY_HIP(launch_stage(op_eta, op_xi, up, true, shape, MODE_RHS, a, (int)tiles.size(), st));
   return ctx_check_err(ctx);
 }
@@ -1252,8 +1312,9 @@ int sbp_system_destroy(sbp_system s) {
   if (s->d_blocks) hipFree(s->d_blocks);
   if (s->d_tiles) hipFree(s->d_tiles);
   if (s->d_pieces) hipFree(s->d_pieces);
-  if (s->d_vedge) hipFree(s->d_vedge);
-  if (s->d_vsegs) hipFree(s->d_vsegs);
+  if (s->d_sat) hipFree(s->d_sat);
+  if (s->d_satsegs) hipFree(s->d_satsegs);
+  if (s->d_progs) hipFree(s->d_progs);
   if (s->d_time) hipFree(s->d_time);
   if (s->d_err) hipFree(s->d_err);
   if (s->d_partials) hipFree(s->d_partials);
@@ -1278,6 +1339,7 @@ int sbp_system_set_bc(sbp_system s, int block, int dir, int kind, int neighbour,
   h.bc_kind[dir] = kind;
   h.bc_nb[dir] = neighbour;
   h.bc_pieces[dir].clear();
+  h.bc_prog[dir].reset();
   s->dirty = true;
   return SBP_OK;
 }
@@ -1298,6 +1360,7 @@ int sbp_system_set_bc_interpolate(sbp_system s, int block, int dir, int neighbou
   h.bc_nb[dir] = neighbour;
   h.bc_iop[dir] = interp_op;
   h.bc_pieces[dir].clear();
+  h.bc_prog[dir].reset();
   s->dirty = true;
   return SBP_OK;
 }
@@ -1319,6 +1382,7 @@ int sbp_system_set_bc_multigrid(sbp_system s, int block, int dir, int npieces, c
   }
   if (covered != (ns ? h.nx : h.ny)) return SBP_ERR_EDGE;
   h.bc_kind[dir] = SBP_BC_MULTIGRID;
+  h.bc_prog[dir].reset();
   h.bc_pieces[dir].assign(pieces, pieces + 3 * (size_t)npieces);
   s->dirty = true;
   return SBP_OK;

@@ -17,15 +17,17 @@
 //   η-phase: the same over the tile columns widened by the η halo, with
 //     (J·ηx, J·ηy), and η-stencils.
 //   finish: k = (idx_ξ·Sξ + idx_η·Sη)/J, RK4 combination, store.
-// Mixed form (MIX): the face terms are FP32 deviations from a per-tile
-// reference state (face_terms_mix), LDS and stencil sums are FP32, y_n and
+// Mixed form (MIX): the face terms are FP64, rounded to FP32 as deviations
+// from those of a per-tile reference state (face_terms_mix64), LDS and
+// stencil sums are FP32, y_n and
 // the RK4 sum stay FP64; each stage input is y_n + Δ with the FP32 increment
 // Δ = a·dt·k stored by the previous stage, and the last stage sums y_n and
 // the increments in FP64.
-// The characteristic SATs of the block-edge points are added to k by the
-// EDGE instantiation, which the host launches for the tiles on block edges
-// only; the interior tiles' instantiation carries none of the 4x4
-// characteristic algebra, so its register count stays low.
+// The characteristic SATs of the block-edge points, evaluated for the stage by
+// the edge-SAT kernel, are added to k by the EDGE instantiation, which the
+// host launches for the tiles on block edges only (it also carries the
+// closure rows); neither instantiation carries the 4x4 characteristic
+// algebra.
 #pragma once
 #include <type_traits>
 
@@ -120,8 +122,11 @@ __device__ __forceinline__ void col_window(const T* p, int stride, T out[RS][RX]
 // 96 VGPRs, but its re-reads cost more than the occupancy gains: 6.2 and 5.8
 // against 7.2 Gpoint-steps/s at 4096².)
 // Threads own RX adjacent ξ-columns and RY consecutive η-rows of the tile.
+// The edge instantiation asks for 4 waves per SIMD (without the hint the
+// upwind9 edge tiles land 2 VGPRs above that budget, at 130).
 template <class OX, class OY, bool UP, int TX, int TY, int RX, int RY, int VAR, bool MIX = false, bool EDGE = false>
-__global__ __launch_bounds__((TX / RX) * (TY / RY)) void stage_kernel(const StageArgs A, const int mode) {
+__global__ __launch_bounds__((TX / RX) * (TY / RY)) __attribute__((amdgpu_waves_per_eu(EDGE ? 4 : 1, 8))) void stage_kernel(
+    const StageArgs A, const int mode) {
   static_assert(!MIX || VAR == 2, "the mixed form keeps flux and α·q together in LDS");
   using CT = std::conditional_t<MIX, float, double>;  // face terms, LDS, stencil sums
   using G = StageGeom<OX, OY, TX, TY, RX, RY, VAR, sizeof(CT)>;
@@ -223,11 +228,11 @@ __global__ __launch_bounds__((TX / RX) * (TY / RY)) void stage_kernel(const Stag
     const int rx0 = max(0, x0 - HX), rx1 = min(nx, x0 + TX + HX);
     const int xw = rx1 - rx0;
     const bool interior = x0 >= OX::M && x0 + TX <= nx - OX::M;
-    MixRef R;
+    MixRef64 R;
     if constexpr (MIX) {
       double qc[4];
       load_q(oc, qc);
-      R = mix_ref(qc, ldg(met + ps, oc), ldg(met + 2 * ps, oc), gamma);
+      R = mix_ref64(qc, ldg(met + ps, oc), ldg(met + 2 * ps, oc), gamma);
     }
     // region point (tile row r, column c from rx0) of this thread's
     // iteration it: own output points first (OWNQ), then the halo columns
@@ -295,7 +300,7 @@ __global__ __launch_bounds__((TX / RX) * (TY / RY)) void stage_kernel(const Stag
           CT f[4], aq[4];
           double al = 0.0;
           if constexpr (MIX) {
-            face_terms_mix<UP>(v[j], v[j][4], v[j][5], R, (float)gamma, f, aq, err);
+            face_terms_mix64<UP>(v[j], v[j][4], v[j][5], R, gamma, f, aq, err);
           } else {
             face_terms<UP>(v[j][0], v[j][1], v[j][2], v[j][3], v[j][4], v[j][5], gamma, f, aq, al, err);
           }
@@ -411,11 +416,11 @@ __global__ __launch_bounds__((TX / RX) * (TY / RY)) void stage_kernel(const Stag
     const int yh = ry1 - ry0;
     const int xend = min(x0 + TX, nx);
     const bool interior = y0 >= OY::M && y0 + TY <= ny - OY::M;
-    MixRef R;
+    MixRef64 R;
     if constexpr (MIX) {
       double qc[4];
       load_q(oc, qc);
-      R = mix_ref(qc, ldg(met + 3 * ps, oc), ldg(met + 4 * ps, oc), gamma);
+      R = mix_ref64(qc, ldg(met + 3 * ps, oc), ldg(met + 4 * ps, oc), gamma);
     }
     // region point (row r from ry0, column c from x0) of this thread's
     // iteration it: own output points first (OWNQ), then the halo rows
@@ -478,7 +483,7 @@ __global__ __launch_bounds__((TX / RX) * (TY / RY)) void stage_kernel(const Stag
           CT f[4], aq[4];
           double al = 0.0;
           if constexpr (MIX) {
-            face_terms_mix<UP>(v[j], v[j][4], v[j][5], R, (float)gamma, f, aq, err);
+            face_terms_mix64<UP>(v[j], v[j][4], v[j][5], R, gamma, f, aq, err);
           } else {
             face_terms<UP>(v[j][0], v[j][1], v[j][2], v[j][3], v[j][4], v[j][5], gamma, f, aq, al, err);
           }
@@ -551,62 +556,6 @@ __global__ __launch_bounds__((TX / RX) * (TY / RY)) void stage_kernel(const Stag
     }
   }
 
-  // ------------------------------------------------------------ edge SATs
-  // Tiles on a block edge (EDGE) compute the characteristic SATs of their
-  // edge points (SAT_characteristics, euler/src/lib.rs:1002-1013: north,
-  // south, east, west in this order; corners get two) between the stencil
-  // phases and the finish loads, one point at a time, and park them in LDS
-  // slots of their own thread; the finish adds them to k. Out of the
-  // stencil phases and before the finish operands are live, the 4x4
-  // characteristic algebra does not raise the kernel's register peak.
-  __shared__ double satl[EDGE ? 4 * RY * RX * NT : 1];
-  if constexpr (EDGE) {
-#pragma unroll
-    for (int s = 0; s < RY; ++s)
-#pragma unroll
-      for (int i = 0; i < RX; ++i) {
-        const int gx = xb + i, gy = yb + s;
-        double sat[4] = {0.0, 0.0, 0.0, 0.0};
-        if (xin[i] && yin[s] && (gx == 0 || gx == nx - 1 || gy == 0 || gy == ny - 1)) {
-          const uint32_t o = (uint32_t)(gy * ld + gx);
-          double q4[4], g[4], mxi0, mxi1, meta0, meta1;
-          if constexpr (OWNQ) {
-#pragma unroll
-            for (int q = 0; q < 4; ++q) q4[q] = ownq[s][q];
-            mxi0 = xsave[(2 * s) * NT + tid];
-            mxi1 = xsave[(2 * s + 1) * NT + tid];
-            meta0 = ownm[s][0];
-            meta1 = ownm[s][1];
-          } else {
-            load_stage_state(A, B, mode, o, q4);
-            mxi0 = ldg(met + ps, o);
-            mxi1 = ldg(met + 2 * ps, o);
-            meta0 = ldg(met + 3 * ps, o);
-            meta1 = ldg(met + 4 * ps, o);
-          }
-          const double dj = ldg(met, o);
-          if (gy == ny - 1 && B.edge_src[SBP_NORTH] != EDGE_NONE) {
-            edge_data(A, B, SBP_NORTH, gx, mode, g);
-            sat_point(sat, q4, g, B.hi_eta, -1.0, 1.0, dj, meta0, meta1, gamma);
-          }
-          if (gy == 0 && B.edge_src[SBP_SOUTH] != EDGE_NONE) {
-            edge_data(A, B, SBP_SOUTH, gx, mode, g);
-            sat_point(sat, q4, g, B.hi_eta, 1.0, -1.0, dj, meta0, meta1, gamma);
-          }
-          if (gx == nx - 1 && B.edge_src[SBP_EAST] != EDGE_NONE) {
-            edge_data(A, B, SBP_EAST, gy, mode, g);
-            sat_point(sat, q4, g, B.hi_xi, -1.0, 1.0, dj, mxi0, mxi1, gamma);
-          }
-          if (gx == 0 && B.edge_src[SBP_WEST] != EDGE_NONE) {
-            edge_data(A, B, SBP_WEST, gy, mode, g);
-            sat_point(sat, q4, g, B.hi_xi, 1.0, -1.0, dj, mxi0, mxi1, gamma);
-          }
-        }
-#pragma unroll
-        for (int q = 0; q < 4; ++q) satl[((s * RX + i) * 4 + q) * NT + tid] = sat[q];
-      }
-  }
-
   // ------------------------------------------------------------ finish
   load_finish();
   // (b_s·dt)·k and (a_s·dt)·k with this block's step (integrate, lib.rs:188-211)
@@ -623,8 +572,24 @@ __global__ __launch_bounds__((TX / RX) * (TY / RY)) void stage_kernel(const Stag
 #pragma unroll
     for (int q = 0; q < 4; ++q) k[q] = (double)acc[s][i][q] * ij;
     if constexpr (EDGE) {
+      // the characteristic SATs of the block-edge points (the edge-SAT kernel
+      // evaluated them for this stage), summed in the order of
+      // SAT_characteristics (north, south, east, west; corners get two)
+      const int gx = xb + i, gy = yb + s;
+      if (gx == 0 || gx == nx - 1 || gy == 0 || gy == ny - 1) {
+        double sat[4] = {0.0, 0.0, 0.0, 0.0};
+        auto add = [&](int d, int t, int len) {
+          if (B.edge_src[d] == EDGE_NONE) return;
 #pragma unroll
-      for (int q = 0; q < 4; ++q) k[q] += satl[((s * RX + i) * 4 + q) * NT + tid];
+          for (int q = 0; q < 4; ++q) sat[q] += ldg(B.sat[d], (uint32_t)(q * len + t));
+        };
+        if (gy == ny - 1) add(SBP_NORTH, gx, nx);
+        if (gy == 0) add(SBP_SOUTH, gx, nx);
+        if (gx == nx - 1) add(SBP_EAST, gy, ny);
+        if (gx == 0) add(SBP_WEST, gy, ny);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) k[q] += sat[q];
+      }
     }
     if (mode == MODE_RHS) {
 #pragma unroll

@@ -91,6 +91,12 @@ typedef struct {
 
 typedef struct sbp_context_s* sbp_context;
 typedef struct sbp_system_s* sbp_system;
+typedef struct sbp_expr_s* sbp_expr;
+
+/* The two expression forms of the multigrid configuration
+ * (multigrid/src/input.rs:165-190): conservative variables (rho, rhou, rhov,
+ * e) or primitive ones (rho, u, v, p). */
+typedef enum { SBP_EXPR_CONSERVATION = 0, SBP_EXPR_PRESSURE = 1 } sbp_expr_form;
 
 /* Human-readable text of a status code. */
 const char* sbp_status_string(int status);
@@ -297,6 +303,40 @@ int sbp_system_integrate(sbp_system sys, int nstages, const double* A, const dou
                          int64_t nsteps);
 /* Sticky device error word of the last advance (0 = none); cleared by reading. */
 int sbp_system_check(sbp_system sys);
+
+/* ---- Expressions (initial and boundary conditions) ----------------------
+ * The evalexpr expressions of a multigrid configuration
+ * (multigrid/src/eval/evalexpr.rs; the evalexpr crate 7.2.0): `globals`
+ * (statements evaluated once, may be NULL or empty) and four expressions of
+ * t, x, y: rho, rhou, rhov, e (conservation form; rhou and rhov may read rho,
+ * e may read rho, rhou, rhov) or rho, u, v, p (pressure form; u and v may
+ * read rho, p may read rho, u, v; e = p/(GAMMA-1) + rho(u²+v²)/2, as
+ * euler/src/eval.rs's EvaluatorPressureWrapper). GAMMA, PI and E are
+ * predefined; integers stay integers (5/2 = 2) and `^` is a float power, as
+ * in evalexpr. Compiled into a program that runs per point on the device.
+ * SBP_ERR_EXPR with a message in errbuf when an expression does not
+ * compile (a syntax error, an unknown name or function, a variable read
+ * before its assignment); SBP_ERR_GAMMA when GAMMA is not set. */
+int sbp_expr_create(int form, const char* globals, const char* const* exprs4, sbp_expr* out, char* errbuf, int errlen);
+int sbp_expr_destroy(sbp_expr expr);
+/* The four conservative variables at n host points (x[i], y[i]) at time t
+ * into out (4 x n, component-major), on the host; SBP_ERR_EXPR when an
+ * evaluation fails (the reference panics). */
+int sbp_expr_eval_host(sbp_expr expr, double t, int64_t n, const double* x, const double* y, double* out);
+/* BoundaryCharacteristic::Eval (euler/src/lib.rs:741, :853-866): the edge's
+ * boundary data are the expressions evaluated at every stage time on the
+ * edge's coordinates (multigrid/src/parsing.rs:179-184 picks it for edges
+ * without an explicit condition when the configuration's boundary
+ * conditions are expressions). The system keeps its own copy of the
+ * program. */
+int sbp_system_set_bc_eval(sbp_system sys, int block, int dir, sbp_expr expr);
+/* Initial conditions from expressions on one block at time t, on the device
+ * (BaseSystem::create, multigrid/src/system.rs:143-152). */
+int sbp_system_eval_block(sbp_system sys, int block, double t, sbp_expr expr);
+/* System::error for expression initial conditions (multigrid/src/system.rs:
+ * 618-635): the sum over blocks of Field::h2_err against the expressions at
+ * the system time. */
+int sbp_system_error_eval(sbp_system sys, sbp_expr expr, double* out);
 
 /* ---- Measurement ---------------------------------------------------------
  * Times nsteps RK4 steps (the state is advanced) with HIP events on the

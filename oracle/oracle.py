@@ -42,6 +42,11 @@ class OracleError(RuntimeError):
     pass
 
 
+# Eval boundary data callback: (t, n, x, y, out 4 x n, user)
+EVAL_FN = ctypes.CFUNCTYPE(None, ctypes.c_double, ctypes.c_int, ctypes.POINTER(ctypes.c_double),
+                           ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double), ctypes.c_void_p)
+
+
 def _load():
     if not os.path.exists(_LIB_PATH):
         raise OracleError(f"oracle library missing: {_LIB_PATH} (run make -C oracle)")
@@ -84,6 +89,7 @@ def _load():
     lib.orc_system_edge.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_double, dp]
     lib.orc_system_embedded_step.argtypes = [ctypes.c_void_p, dp, ctypes.c_double, dp, ctypes.c_int]
     lib.orc_system_advance_adaptive.argtypes = [ctypes.c_void_p, ctypes.c_double, dp, ctypes.c_double, ctypes.c_int]
+    lib.orc_system_set_bc_eval.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, EVAL_FN, ctypes.c_void_p]
     lib.orc_system_set_sat.argtypes = [ctypes.c_void_p, ctypes.c_int]
     lib.orc_sat_edge.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, dp, dp, dp, dp]
     lib.orc_system_integrate.argtypes = [ctypes.c_void_p, ctypes.c_int, dp, dp, dp, ctypes.c_double, ctypes.c_long,
@@ -274,6 +280,19 @@ class System:
             res.append(out[off:off + n].reshape((4,) + s))
             off += n
         return res
+
+    def set_bc_eval(self, block, direction, evaluator):
+        """BoundaryCharacteristic::Eval (euler/src/lib.rs:741): `evaluator`
+        maps (t, x, y) arrays of edge points to (rho, rhou, rhov, e) arrays
+        (e.g. oracle.evalexpr.Evaluator.field)."""
+        def cb(t, n, x, y, out, _user):
+            X = np.ctypeslib.as_array(x, shape=(n,)).copy()
+            Y = np.ctypeslib.as_array(y, shape=(n,)).copy()
+            q = np.asarray(evaluator(t, X, Y), dtype=np.float64).reshape(4, n)
+            np.ctypeslib.as_array(out, shape=(4 * n,))[:] = q.ravel()
+        fn = EVAL_FN(cb)
+        self._callbacks = getattr(self, "_callbacks", []) + [fn]  # kept alive with the system
+        _check(lib().orc_system_set_bc_eval(self.h, block, direction, fn, None), "set_bc_eval")
 
     def set_sat(self, on):
         """False: the right-hand side is RHS_no_SAT (euler/src/lib.rs:518-568)."""

@@ -23,7 +23,7 @@
 //   rhs            euler/src/lib.rs:483-515 (trad), :571-616 (upwind)
 //   sat            euler/src/lib.rs:1002-1252
 //   boundaries     euler/src/lib.rs:763-802, :804-915 (This / Grid / Vortex /
-//                  Interpolate / MultiGrid)
+//                  Interpolate / MultiGrid / Eval, the last through a callback)
 //   interpolate    sbp/src/operators/interpolation.rs:15-50 (2:1 interfaces;
 //                  tables in sbp_interp_tables.h)
 //   integrate      utils/integrate/src/lib.rs:30-35, :176-222 (Rk4; any
@@ -52,7 +52,11 @@ enum { OP_UPWIND4 = 0, OP_UPWIND4H2 = 1, OP_UPWIND9 = 2, OP_UPWIND9H2 = 3, OP_SB
 enum { KIND_DIFF = 0, KIND_DISS = 1 };
 enum { ST_OK = 0, ST_INVALID = 1, ST_SHAPE = 2, ST_DETJ = 3, ST_RHO = 4, ST_PRESSURE = 5, ST_GAMMA = 6, ST_EDGE = 7,
        ST_NOT_UPWIND = 9 };
-enum { BC_THIS = 0, BC_GRID = 1, BC_VORTEX = 2, BC_INTERPOLATE = 3, BC_MULTIGRID = 4 };
+enum { BC_THIS = 0, BC_GRID = 1, BC_VORTEX = 2, BC_INTERPOLATE = 3, BC_MULTIGRID = 4, BC_EVAL = 5 };
+
+// Eval boundary data (lib.rs:853-866): the evaluator is a callback that fills
+// rho, rhou, rhov, e (4 x n, component-major) at time t on n edge points.
+typedef void (*orc_eval_fn)(double t, int n, const double* x, const double* y, double* out, void* user);
 enum { IOP_4 = 0, IOP_8 = 1, IOP_9 = 2, IOP_9H2 = 3, IOP_COUNT = 4 };
 
 // One direction (fine to coarse or coarse to fine) of an interpolation
@@ -370,6 +374,8 @@ struct Block {
   int bc_iop[4] = {0, 0, 0, 0};           // Interpolate: operator
   std::vector<int> bc_pieces[4];          // MultiGrid: (block, start, end) triples
   orc_vortex_params bc_vortex[4];
+  orc_eval_fn bc_eval[4] = {nullptr, nullptr, nullptr, nullptr};  // Eval: evaluator and its user data
+  void* bc_eval_user[4] = {nullptr, nullptr, nullptr, nullptr};
   size_t npts() const { return (size_t)ny * nx; }
 };
 
@@ -509,6 +515,26 @@ int extract_edge(System& S, size_t bi, const std::vector<std::vector<double>>& f
     int have = (dir == DIR_NORTH || dir == DIR_SOUTH) ? s.nx : s.ny;
     if (need != have) return ST_EDGE;
     out = sel(src);
+  } else if (kind == BC_EVAL) {
+    // Eval (lib.rs:853-866): the evaluator on this block's own edge
+    // coordinates at the stage time
+    const int n = (dir == DIR_NORTH || dir == DIR_SOUTH) ? b.nx : b.ny;
+    std::vector<double> ex(n), ey(n);
+    for (int i = 0; i < n; ++i) {
+      size_t q;
+      switch (dir) {
+        case DIR_NORTH: q = (size_t)(b.ny - 1) * b.nx + i; break;
+        case DIR_SOUTH: q = (size_t)i; break;
+        case DIR_EAST: q = (size_t)i * b.nx + (b.nx - 1); break;
+        default: q = (size_t)i * b.nx; break;
+      }
+      ex[i] = b.x[q];
+      ey[i] = b.y[q];
+    }
+    if (!b.bc_eval[dir]) return ST_INVALID;
+    double* eb = w.edge[dir].data();
+    b.bc_eval[dir](time, n, ex.data(), ey.data(), eb, b.bc_eval_user[dir]);
+    out = {eb, (size_t)n, 1};
   } else if (kind == BC_VORTEX) {
     // vortexify on this block's own edge coordinates (lib.rs:983-998, grid.rs:68-91)
     int n = (dir == DIR_NORTH || dir == DIR_SOUTH) ? b.nx : b.ny;
@@ -983,6 +1009,17 @@ int orc_system_set_bc(void* s, int block, int dir, int kind, int neighbour, cons
   b.bc_kind[dir] = kind;
   b.bc_nb[dir] = neighbour;
   if (vp) b.bc_vortex[dir] = *vp;
+  return ST_OK;
+}
+
+// BoundaryCharacteristic::Eval(evaluator) (lib.rs:741) through a callback
+int orc_system_set_bc_eval(void* s, int block, int dir, orc_eval_fn fn, void* user) {
+  System* S = (System*)s;
+  if (block < 0 || block >= (int)S->blocks.size() || dir < 0 || dir > 3 || !fn) return ST_INVALID;
+  Block& b = S->blocks[block];
+  b.bc_kind[dir] = BC_EVAL;
+  b.bc_eval[dir] = fn;
+  b.bc_eval_user[dir] = user;
   return ST_OK;
 }
 

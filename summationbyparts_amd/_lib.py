@@ -11,12 +11,13 @@ LIB_PATH = os.path.join(_HERE, "libsbpgpu.so")
 
 # sbp_status
 OK, ERR_INVALID, ERR_SHAPE, ERR_DETJ, ERR_RHO, ERR_PRESSURE, ERR_GAMMA, ERR_EDGE, ERR_HIP, ERR_NOT_UPWIND, ERR_OOM, \
-    ERR_GAMMA_SET, ERR_SIZE = range(13)
+    ERR_GAMMA_SET, ERR_SIZE, ERR_EXPR = range(14)
 # sbp_operator
 UPWIND4, UPWIND4H2, UPWIND9, UPWIND9H2, SBP4, SBP8 = range(6)
 DIFF, DISS = 0, 1
 NORTH, SOUTH, EAST, WEST = 0, 1, 2, 3
-BC_THIS, BC_GRID, BC_VORTEX, BC_INTERPOLATE, BC_MULTIGRID = 0, 1, 2, 3, 4
+BC_THIS, BC_GRID, BC_VORTEX, BC_INTERPOLATE, BC_MULTIGRID, BC_EVAL = 0, 1, 2, 3, 4, 5
+EXPR_CONSERVATION, EXPR_PRESSURE = 0, 1
 INTERP4, INTERP8, INTERP9, INTERP9H2 = 0, 1, 2, 3
 RHS_AUTO, RHS_TRAD, RHS_UPWIND = 0, 1, 2
 FP64, MIXED = 0, 1
@@ -117,6 +118,13 @@ def _declare(L):
         "sbp_h2_err": ([c_void_p, c_int, c_int, c_int, c_int, c_void_p, c_void_p, dp], c_int),
         "sbp_h2linspace": ([c_double, c_double, c_int, dp], c_int),
         "sbp_h2_err_noalloc": ([c_void_p, c_int, c_int, c_int, c_int, c_void_p, c_void_p, dp], c_int),
+        "sbp_expr_create": ([c_int, ctypes.c_char_p, ctypes.POINTER(ctypes.c_char_p), ctypes.POINTER(c_void_p),
+                             ctypes.c_char_p, c_int], c_int),
+        "sbp_expr_destroy": ([c_void_p], c_int),
+        "sbp_expr_eval_host": ([c_void_p, c_double, c_int64, dp, dp, dp], c_int),
+        "sbp_system_set_bc_eval": ([c_void_p, c_int, c_int, c_void_p], c_int),
+        "sbp_system_eval_block": ([c_void_p, c_int, c_double, c_void_p], c_int),
+        "sbp_system_error_eval": ([c_void_p, c_void_p, dp], c_int),
         "sbp_system_integrate": ([c_void_p, c_int, dp, dp, dp, c_double, c_int64], c_int),
         "sbp_tile_shape_count": ([], c_int),
         "sbp_tile_shape_get": ([c_int, ip, ip, ip], c_int),
@@ -143,7 +151,8 @@ EXPORTED = [
     "sbp_tile_shape_count",
     "sbp_tile_shape_get",
     "sbp_rhs_no_sat", "sbp_sat_characteristics", "sbp_sat_edge", "sbp_h2_err", "sbp_h2linspace",
-    "sbp_system_integrate", "sbp_h2_err_noalloc",
+    "sbp_system_integrate", "sbp_h2_err_noalloc", "sbp_expr_create", "sbp_expr_destroy", "sbp_expr_eval_host",
+    "sbp_system_set_bc_eval", "sbp_system_eval_block", "sbp_system_error_eval",
 ]
 
 

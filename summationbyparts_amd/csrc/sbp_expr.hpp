@@ -136,7 +136,8 @@ __host__ __device__ inline bool int_ok(double v) { return v >= -9007199254740992
 // Runs the expression at code[0..len) with the given inputs; false on an
 // evaluation error (a type error, an integer division by zero, a missing
 // value), which the reference reports by panicking.
-__host__ __device__ inline bool run(const Instr* code, int len, const Value* consts, const double* in, Value& out) {
+__host__ __device__ inline bool run(const Instr* code, int len, const Value* consts, const double* in, Value& out,
+                                    Value* slots_out = nullptr) {
   Value st[kMaxStack];
   Value slot[kMaxSlots];
   int sp = 0;
@@ -286,8 +287,8 @@ __host__ __device__ inline bool run(const Instr* code, int len, const Value* con
           case FN_IS_NORMAL: {
             if (!is_num(a[0])) return false;
             const double v = a[0].v;
-            bool b = fn == FN_IS_NAN ? isnan(v) : fn == FN_IS_FINITE ? isfinite(v) : fn == FN_IS_INFINITE ? isinf(v)
-                                                                                                          : isnormal(v);
+            bool b = fn == FN_IS_NAN ? __builtin_isnan(v) : fn == FN_IS_FINITE ? __builtin_isfinite(v) : fn == FN_IS_INFINITE ? __builtin_isinf(v)
+                                                                                                          : __builtin_isnormal(v);
             r = mk(b ? 1.0 : 0.0, TAG_BOOL);
             break;
           }
@@ -333,6 +334,8 @@ __host__ __device__ inline bool run(const Instr* code, int len, const Value* con
   }
   if (sp != 1) return false;
   out = st[0];
+  if (slots_out)
+    for (int s = 0; s < kMaxSlots; ++s) slots_out[s] = slot[s];
   return true;
 }
 

@@ -34,6 +34,9 @@ hipError_t launch_metrics_finish(const double* dx_dxi, const double* dx_deta, co
 // Characteristic SATs of the listed edges at stage `mode` into BlockDev::sat
 // (maxlen: the longest listed edge).
 hipError_t launch_edge_sat(const StageArgs& a, const SatSeg* segs, int nsegs, int maxlen, int mode, hipStream_t stream);
+// Expressions of `prog` (a device copy) on a block at time t into 4 planes.
+hipError_t launch_eval_field(const expr::Program* prog, double t, const double* xc, const double* yc, double* out, int64_t ps,
+                             int64_t ld, int ny, int nx, unsigned* err, hipStream_t stream);
 // Boundary data of the listed Eval edges at the stage time of `a` into BlockDev::edata.
 hipError_t launch_eval_edges(const StageArgs& a, const SatSeg* segs, int nsegs, int maxlen, hipStream_t stream);
 // Metric pairs (BlockDev::mpair, 4·ps doubles) from the 5 metric planes.

@@ -179,6 +179,33 @@ __global__ __launch_bounds__(256) void eval_edges_kernel(const StageArgs A, cons
   if (err) record_error(A.error, err);
 }
 
+// Expressions on a whole block at time t into 4 planes (initial conditions
+// from expressions, BaseSystem::create, multigrid/src/system.rs:143-152; the
+// reference field of System::error :618-635).
+__global__ __launch_bounds__(256) void eval_field_kernel(const expr::Program* __restrict__ P, double t,
+                                                         const double* __restrict__ xc, const double* __restrict__ yc,
+                                                         double* __restrict__ out, int64_t ps, int64_t ld, int ny, int nx,
+                                                         unsigned* err) {
+  const int x = blockIdx.x * 64 + (threadIdx.x & 63);
+  const int y = blockIdx.y * 4 + (threadIdx.x >> 6);
+  if (x >= nx || y >= ny) return;
+  const int64_t o = (int64_t)y * ld + x;
+  double q[4];
+  if (!expr::eval_point(P, t, xc[o], yc[o], q)) {
+    record_error(err, ERRBIT_EXPR);
+    q[0] = q[1] = q[2] = q[3] = 0.0;
+  }
+#pragma unroll
+  for (int c = 0; c < 4; ++c) out[c * ps + o] = q[c];
+}
+
+hipError_t launch_eval_field(const expr::Program* prog, double t, const double* xc, const double* yc, double* out, int64_t ps,
+                             int64_t ld, int ny, int nx, unsigned* err, hipStream_t stream) {
+  dim3 grid((nx + 63) / 64, (ny + 3) / 4);
+  eval_field_kernel<<<grid, 256, 0, stream>>>(prog, t, xc, yc, out, ps, ld, ny, nx, err);
+  return hipGetLastError();
+}
+
 hipError_t launch_eval_edges(const StageArgs& a, const SatSeg* segs, int nsegs, int maxlen, hipStream_t stream) {
   if (nsegs <= 0) return hipSuccess;
   dim3 grid(nsegs, (maxlen + 255) / 256);

@@ -20,6 +20,7 @@
 #include "sbp_operators.hpp"
 
 namespace sbp {
+std::shared_ptr<const std::vector<unsigned char>> expr_blob(sbp_expr e);
 hipError_t launch_stage_opxi_0(int, bool, bool, int, int, const StageArgs&, int, hipStream_t);
 hipError_t launch_stage_opxi_1(int, bool, bool, int, int, const StageArgs&, int, hipStream_t);
 hipError_t launch_stage_opxi_2(int, bool, bool, int, int, const StageArgs&, int, hipStream_t);
@@ -793,6 +794,7 @@ const char* sbp_status_string(int status) {
     case SBP_ERR_OOM: return "device out of memory";
     case SBP_ERR_GAMMA_SET: return "GAMMA already set";
     case SBP_ERR_SIZE: return "size beyond the entry point's maximum";
+    case SBP_ERR_EXPR: return "expression does not compile or evaluate";
     default: return "unknown status";
   }
 }
@@ -1385,6 +1387,70 @@ int sbp_system_set_bc_multigrid(sbp_system s, int block, int dir, int npieces, c
   h.bc_prog[dir].reset();
   h.bc_pieces[dir].assign(pieces, pieces + 3 * (size_t)npieces);
   s->dirty = true;
+  return SBP_OK;
+}
+
+int sbp_system_set_bc_eval(sbp_system s, int block, int dir, sbp_expr e) {
+  if (!s || !e || block < 0 || block >= (int)s->blocks.size() || dir < 0 || dir > 3) return SBP_ERR_INVALID;
+  HostBlock& h = s->blocks[block];
+  h.bc_kind[dir] = SBP_BC_EVAL;
+  h.bc_prog[dir] = expr_blob(e);
+  h.bc_pieces[dir].clear();
+  s->dirty = true;
+  return SBP_OK;
+}
+
+// A device copy of an expression program, freed with the returned guard.
+struct DeviceProgram {
+  void* p = nullptr;
+  ~DeviceProgram() {
+    if (p) hipFree(p);
+  }
+};
+
+static int upload_program(sbp_expr e, hipStream_t st, DeviceProgram& out) {
+  auto blob = expr_blob(e);
+  if (!blob) return SBP_ERR_INVALID;
+  SBP_TRY_HIP(hipMalloc(&out.p, blob->size()));
+  SBP_TRY_HIP(hipMemcpyAsync(out.p, blob->data(), blob->size(), hipMemcpyHostToDevice, st));
+  return SBP_OK;
+}
+
+int sbp_system_eval_block(sbp_system s, int block, double t, sbp_expr e) {
+  if (!s || !e || block < 0 || block >= (int)s->blocks.size()) return SBP_ERR_INVALID;
+  SBP_TRY_HIP(hipSetDevice(s->ctx->device));
+  SBP_TRY(normalize(s));
+  hipStream_t st = s->ctx->stream;
+  DeviceProgram prog;
+  SBP_TRY(upload_program(e, st, prog));
+  HostBlock& h = s->blocks[block];
+  SBP_TRY_HIP(launch_eval_field(static_cast<const expr::Program*>(prog.p), t, h.d_x, h.d_y, h.d_buf[BUF_YN], h.ps, h.ld,
+                                h.ny, h.nx, s->d_err, st));
+  return check_err(s);
+}
+
+int sbp_system_error_eval(sbp_system s, sbp_expr e, double* out) {
+  if (!s || !e || !out) return SBP_ERR_INVALID;
+  SBP_TRY_HIP(hipSetDevice(s->ctx->device));
+  SBP_TRY(normalize(s));
+  hipStream_t st = s->ctx->stream;
+  double t;
+  SBP_TRY(sbp_system_get_time(s, &t));
+  DeviceProgram prog;
+  SBP_TRY(upload_program(e, st, prog));
+  double err = 0.0;
+  for (int i = 0; i < (int)s->blocks.size(); ++i) {
+    HostBlock& h = s->blocks[i];
+    // the accumulator buffer is free between steps: it holds the reference field
+    double* ref = h.d_buf[BUF_ACC];
+    SBP_TRY_HIP(launch_eval_field(static_cast<const expr::Program*>(prog.p), t * h.dt_scale, h.d_x, h.d_y, ref, h.ps, h.ld,
+                                  h.ny, h.nx, s->d_err, st));
+    long double sum;
+    SBP_TRY(h2_sum(s, i, ref, nullptr, 0.0, sum));
+    err += std::sqrt((double)sum);
+  }
+  SBP_TRY(check_err(s));
+  *out = err;
   return SBP_OK;
 }
 

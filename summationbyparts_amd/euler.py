@@ -217,6 +217,22 @@ class BlockSystem:
                    "sbp_system_advance_adaptive")
         return g.value
 
+    def set_bc_eval(self, block, direction, expressions):
+        """BoundaryCharacteristic::Eval (lib.rs:741) on one edge."""
+        _lib.check(_lib.lib().sbp_system_set_bc_eval(self._h, block, direction, expressions.handle),
+                   "sbp_system_set_bc_eval")
+
+    def eval_block(self, block, t, expressions):
+        """The state of one block from expressions at time t (on the device)."""
+        _lib.check(_lib.lib().sbp_system_eval_block(self._h, block, t, expressions.handle), "sbp_system_eval_block")
+
+    def error_eval(self, expressions):
+        """System::error against expressions at the system time."""
+        out = ctypes.c_double()
+        _lib.check(_lib.lib().sbp_system_error_eval(self._h, expressions.handle, ctypes.byref(out)),
+                   "sbp_system_error_eval")
+        return out.value
+
     def integrate(self, tableau, dt, nsteps=1):
         """nsteps of integrate::<tableau> (utils/integrate/src/lib.rs:176-222)
         for any explicit tableau (summationbyparts_amd.integrate.Rk4, Rk4_38,

@@ -41,6 +41,14 @@ class Interpolate:
 
 
 @dataclass
+class Eval:
+    """BoundaryCharacteristic::Eval(evaluator) (euler/src/lib.rs:741): the
+    expressions (summationbyparts_amd.expressions.Expressions) evaluated on
+    the edge's coordinates at every stage time."""
+    expressions: object
+
+
+@dataclass
 class MultiGrid:
     """BoundaryCharacteristic::MultiGrid(pieces) (euler/src/lib.rs:744):
     (block, start, end) pieces of the neighbours' opposite edges."""
@@ -63,6 +71,8 @@ class System(BlockSystem):
             self.set_bc_interpolate(block, direction, bc.block, bc.operator.ID)
         elif isinstance(bc, MultiGrid):
             self.set_bc_multigrid(block, direction, bc.pieces)
+        elif isinstance(bc, Eval):
+            self.set_bc_eval(block, direction, bc.expressions)
         else:
             raise TypeError(f"unsupported boundary condition {bc!r}")
 
@@ -74,8 +84,11 @@ class System(BlockSystem):
         self.advance_steps(self.dt, nsteps)
 
     def error(self, params):
-        """System::error for vortex initial conditions (system.rs:618-635)."""
-        return self.error_vortex(params)
+        """System::error (system.rs:618-635) against the vortex or the
+        expressions of the initial conditions at the system time."""
+        if isinstance(params, VortexParameters):
+            return self.error_vortex(params)
+        return self.error_eval(params)
 
     def fields(self):
         return [self.get_state(i) for i in range(len(self.shapes))]

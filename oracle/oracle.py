@@ -91,6 +91,7 @@ def _load():
     lib.orc_system_advance_adaptive.argtypes = [ctypes.c_void_p, ctypes.c_double, dp, ctypes.c_double, ctypes.c_int]
     lib.orc_system_set_bc_eval.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, EVAL_FN, ctypes.c_void_p]
     lib.orc_system_set_sat.argtypes = [ctypes.c_void_p, ctypes.c_int]
+    lib.orc_set_inner_threads.argtypes = [ctypes.c_int]
     lib.orc_sat_edge.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, dp, dp, dp, dp]
     lib.orc_system_integrate.argtypes = [ctypes.c_void_p, ctypes.c_int, dp, dp, dp, ctypes.c_double, ctypes.c_long,
                                          ctypes.c_int]
@@ -171,6 +172,12 @@ def h2_err(op_eta, op_xi, a, b):
     out = ctypes.c_double()
     _check(lib().orc_h2_err(op_eta, op_xi, _ptr(a), _ptr(b), a.shape[1], a.shape[2], ctypes.byref(out)), "h2_err")
     return out.value
+
+
+def set_inner_threads(n):
+    """Threads inside each block (1, the default, is the reference's one
+    thread per block); results do not depend on it."""
+    _check(lib().orc_set_inner_threads(int(n)), "set_inner_threads")
 
 
 def sat_edge(op_eta, op_xi, direction, k, y, metrics, g):

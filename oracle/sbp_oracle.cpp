@@ -40,6 +40,8 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <algorithm>
+#include <mutex>
 #include <thread>
 #include <vector>
 
@@ -192,6 +194,28 @@ const BlockMat* get_block(int op, int kind) {
   return nullptr;
 }
 
+// Threads inside one block (orc_set_inner_threads; 1 = the reference's
+// one thread per block). Work is split into contiguous ranges of rows or
+// points, each computed with the same arithmetic as the sequential loop, so
+// the results do not depend on the thread count.
+int g_inner_threads = 1;
+
+template <class F>
+void pfor(size_t n, F&& f) {
+  const int T = g_inner_threads;
+  if (T <= 1 || n < 256) {
+    f((size_t)0, n);
+    return;
+  }
+  std::vector<std::thread> th;
+  const size_t chunk = (n + (size_t)T - 1) / (size_t)T;
+  for (int t = 0; t < T; ++t) {
+    const size_t lo = (size_t)t * chunk, hi = std::min(n, lo + chunk);
+    if (lo < hi) th.emplace_back(f, lo, hi);
+  }
+  for (auto& x : th) x.join();
+}
+
 inline double inv_spacing(int n, bool h2) {
   // algos.rs:125-130: dx = 1/(n-1) (1/(n-2) for h2), idx = 1/dx
   double dx = h2 ? 1.0 / (double)(n - 2) : 1.0 / (double)(n - 1);
@@ -205,7 +229,8 @@ void diff_rows(const BlockMat& B, bool h2, const double* prev, double* fut, int 
   const int M = B.M, N = B.N, D = B.D;
   const double idx = inv_spacing(nx, h2);
   const int skip = M - (D - 1) / 2;
-  for (int j = 0; j < ny; ++j) {
+  pfor((size_t)ny, [&](size_t j0, size_t j1) {
+  for (int j = (int)j0; j < (int)j1; ++j) {
     const double* p = prev + (size_t)j * nx;
     double* f = fut + (size_t)j * nx;
     for (int i = 0; i < M; ++i) {
@@ -228,6 +253,7 @@ void diff_rows(const BlockMat& B, bool h2, const double* prev, double* fut, int 
       f[nx - M + i] = t * idx;
     }
   }
+  });
 }
 
 // η-direction operator on a C-order (ny, nx) array. The reference applies
@@ -258,7 +284,8 @@ void diff_cols(const BlockMat& B, bool h2, const double* prev, double* fut, int 
     }
   };
   block_rows(B.start, 0, 0);
-  for (int r = M; r < ny - M; ++r) {
+  pfor((size_t)(ny - 2 * M), [&](size_t r0, size_t r1) {
+  for (int r = M + (int)r0; r < M + (int)r1; ++r) {
     double* f = fut + (size_t)r * nx;
     for (int i = 0; i < simd; ++i) {
       double s = 0.0;
@@ -271,6 +298,7 @@ void diff_cols(const BlockMat& B, bool h2, const double* prev, double* fut, int 
       f[i] = idx * s;
     }
   }
+  });
   block_rows(B.end, ny - N, ny - M);
 }
 
@@ -616,8 +644,9 @@ int block_rhs(System& S, size_t bi, const std::vector<std::vector<double>>& fiel
   double *eh = w.ehat.data(), *fh = w.fhat.data(), *p = w.p.data();
   const double *rho = y, *rhou = y + n, *rhov = y + 2 * n, *e = y + 3 * n;
   // fluxes, lib.rs:680-735
-  for (size_t q = 0; q < n; ++q) p[q] = pressure(gamma, rho[q], rhou[q], rhov[q], e[q]);
-  for (size_t q = 0; q < n; ++q) {
+  pfor(n, [&](size_t q0, size_t q1) {
+  for (size_t q = q0; q < q1; ++q) p[q] = pressure(gamma, rho[q], rhou[q], rhov[q], e[q]);
+  for (size_t q = q0; q < q1; ++q) {
     eh[q] = rhou[q];
     eh[n + q] = rhou[q] * rhou[q] / rho[q] + p[q];
     eh[2 * n + q] = rhou[q] * rhov[q] / rho[q];
@@ -628,26 +657,40 @@ int block_rhs(System& S, size_t bi, const std::vector<std::vector<double>>& fiel
     fh[3 * n + q] = rhov[q] * (e[q] + p[q]) / rho[q];
   }
   for (int c = 0; c < 4; ++c)
-    for (size_t q = 0; q < n; ++q) {
+    for (size_t q = q0; q < q1; ++q) {
       double ef = eh[c * n + q], ff = fh[c * n + q];
       eh[c * n + q] = b.jxx[q] * ef + b.jxy[q] * ff;
       fh[c * n + q] = b.jyx[q] * ef + b.jyy[q] * ff;
     }
+  });
   int st;
   for (int c = 0; c < 4; ++c) {
     if ((st = diff_2d(b.op_xi, KIND_DIFF, 0, eh + c * n, w.dE.data() + c * n, b.ny, b.nx))) return st;
     if ((st = diff_2d(b.op_eta, KIND_DIFF, 1, fh + c * n, w.dF.data() + c * n, b.ny, b.nx))) return st;
   }
   if (upwind) {
-    // upwind_dissipation, lib.rs:619-674 (tmp0/tmp1 reuse ehat/fhat)
-    for (size_t q = 0; q < n; ++q) {
+    // upwind_dissipation, lib.rs:619-674 (tmp0/tmp1 reuse ehat/fhat); the
+    // first failing point in point order decides the status, as the
+    // sequential loop's first assert does
+    std::vector<std::pair<size_t, int>> fails;
+    std::mutex fmu;
+    pfor(n, [&](size_t q0, size_t q1) {
+    for (size_t q = q0; q < q1; ++q) {
       double r = rho[q], ru = rhou[q], rv = rhov[q], ee = e[q];
-      if (!(r > 0.0)) return ST_RHO;
+      if (!(r > 0.0)) {
+        std::lock_guard<std::mutex> lk(fmu);
+        fails.push_back({q, ST_RHO});
+        return;
+      }
       double u = ru / r, v = rv / r;
       double uhat = b.jxx[q] * u + b.jxy[q] * v;
       double vhat = b.jyx[q] * u + b.jyy[q] * v;
       double pp = pressure(gamma, r, ru, rv, ee);
-      if (!(pp > 0.0)) return ST_PRESSURE;
+      if (!(pp > 0.0)) {
+        std::lock_guard<std::mutex> lk(fmu);
+        fails.push_back({q, ST_PRESSURE});
+        return;
+      }
       double c = std::sqrt(gamma * pp / r);
       double au = std::fabs(uhat) + c * std::sqrt(b.jxx[q] * b.jxx[q] + b.jxy[q] * b.jxy[q]);
       double av = std::fabs(vhat) + c * std::sqrt(b.jyx[q] * b.jyx[q] + b.jyy[q] * b.jyy[q]);
@@ -660,16 +703,22 @@ int block_rhs(System& S, size_t bi, const std::vector<std::vector<double>>& fiel
       eh[3 * n + q] = au * ee;
       fh[3 * n + q] = av * ee;
     }
+    });
+    if (!fails.empty()) return std::min_element(fails.begin(), fails.end())->second;
     for (int c = 0; c < 4; ++c) {
       if ((st = diff_2d(b.op_xi, KIND_DISS, 0, eh + c * n, w.adx.data() + c * n, b.ny, b.nx))) return st;
       if ((st = diff_2d(b.op_eta, KIND_DISS, 1, fh + c * n, w.ady.data() + c * n, b.ny, b.nx))) return st;
     }
-    for (int c = 0; c < 4; ++c)
-      for (size_t q = 0; q < n; ++q)
-        kout[c * n + q] = (-w.dE[c * n + q] - w.dF[c * n + q] + w.adx[c * n + q] + w.ady[c * n + q]) / b.detj[q];
+    pfor(n, [&](size_t q0, size_t q1) {
+      for (int c = 0; c < 4; ++c)
+        for (size_t q = q0; q < q1; ++q)
+          kout[c * n + q] = (-w.dE[c * n + q] - w.dF[c * n + q] + w.adx[c * n + q] + w.ady[c * n + q]) / b.detj[q];
+    });
   } else {
-    for (int c = 0; c < 4; ++c)
-      for (size_t q = 0; q < n; ++q) kout[c * n + q] = (-w.dE[c * n + q] - w.dF[c * n + q]) / b.detj[q];
+    pfor(n, [&](size_t q0, size_t q1) {
+      for (int c = 0; c < 4; ++c)
+        for (size_t q = q0; q < q1; ++q) kout[c * n + q] = (-w.dE[c * n + q] - w.dF[c * n + q]) / b.detj[q];
+    });
   }
   if (!S.sat) return ST_OK;  // RHS_no_SAT, lib.rs:518-568
   // SAT_characteristics, lib.rs:1002-1013 (north, south, east, west)
@@ -752,8 +801,9 @@ int system_step(System& S, double dt, int nthreads) {
     for (size_t bi = 0; bi < nb; ++bi) {
       double* f = S.fnext[bi].data();
       const double* kk = S.k[j][bi].data();
-      size_t m = S.fnext[bi].size();
-      for (size_t q = 0; q < m; ++q) f[q] = f[q] + scale * kk[q];
+      pfor(S.fnext[bi].size(), [&](size_t q0, size_t q1) {
+        for (size_t q = q0; q < q1; ++q) f[q] = f[q] + scale * kk[q];
+      });
     }
   };
   int st;
@@ -1142,6 +1192,14 @@ int orc_system_rhs(void* s, double t, double* kout) {
     std::memcpy(kout + off, kk.data(), kk.size() * 8);
     off += kk.size();
   }
+  return ST_OK;
+}
+
+// Threads used inside each block (1 = the reference's one thread per block;
+// the results do not depend on it)
+int orc_set_inner_threads(int n) {
+  if (n < 1) return ST_INVALID;
+  g_inner_threads = n;
   return ST_OK;
 }
 

@@ -662,3 +662,22 @@ def test_integrate_tableau_composed_from_rhs(name):
     S.integrate(A, B, C, dt, 1)
     assert np.array_equal(S.get_state(0), expect)
     assert S.time == t0 + dt
+
+
+def test_inner_threads_do_not_change_results():
+    """The oracle's threads inside a block (the CPU baseline's all-core
+    variant) split rows and points into ranges computed with the same
+    arithmetic: results bit for bit those of one thread."""
+    X, Y = curvilinear(300, 260, amp=0.07)
+    q = _smooth_state(X, Y)
+    out = []
+    for n in (1, 5):
+        O.set_inner_threads(n)
+        try:
+            S = O.System([(X, Y)], [(O.UPWIND9, O.UPWIND4)])
+            S.set_state(0, q)
+            S.advance(1e-3, 2)
+            out.append(S.get_state(0))
+        finally:
+            O.set_inner_threads(1)
+    assert np.array_equal(out[0], out[1])

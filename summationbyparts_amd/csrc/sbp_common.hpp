@@ -151,6 +151,43 @@ __device__ __forceinline__ T row_sum(const T* line, int stride, int i, int n, in
   return interior_sum<B>(line + (i - r0) * stride, stride);
 }
 
+// row_sum of the four components at once (component q at line + q·qs): each
+// closure coefficient is read from the tables once for all four, the sums
+// run in row_sum's order.
+template <class B, bool CL = true, typename T>
+__device__ __forceinline__ void row_sum4(const T* line, int qs, int stride, int i, int n, int r0, T out[4]) {
+  if constexpr (CL) {
+    if (i < B::M) {
+      T s[4] = {T(0), T(0), T(0), T(0)};
+#pragma unroll 2
+      for (int k = 0; k < B::N; ++k) {
+        const T c = T(B::start(i, k));
+#pragma unroll
+        for (int q = 0; q < 4; ++q) s[q] += c * line[q * qs + (k - r0) * stride];
+      }
+#pragma unroll
+      for (int q = 0; q < 4; ++q) out[q] = s[q];
+      return;
+    }
+    if (i >= n - B::M) {
+      const int r = i - (n - B::M);
+      const int b0 = n - B::N - r0;
+      T s[4] = {T(0), T(0), T(0), T(0)};
+#pragma unroll 2
+      for (int k = 0; k < B::N; ++k) {
+        const T c = T(B::end(r, k));
+#pragma unroll
+        for (int q = 0; q < 4; ++q) s[q] += c * line[q * qs + (b0 + k) * stride];
+      }
+#pragma unroll
+      for (int q = 0; q < 4; ++q) out[q] = s[q];
+      return;
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < 4; ++q) out[q] = interior_sum<B>(line + q * qs + (i - r0) * stride, stride);
+}
+
 // Tile placement along one axis of length n with K = ceil(n/T) tiles: tile t
 // computes [x0, x0+T) and writes [w0, w1). The last tile is right-aligned and
 // owns every end-closure row, so a halo of H points always covers the rows a

@@ -223,8 +223,10 @@ struct Group {
   bool upwind;
   std::vector<int> blocks;
   int shape = -1;
-  int tile_offset = 0, ntiles = 0;    // interior tiles
-  int etile_offset = 0, entiles = 0;  // tiles on block edges (they add the edge SATs)
+  int merged = -1;  // schedule: 0 split over two streams, 1 one dispatch (-1 not chosen yet)
+  // the tiles on block edges (they add the edge SATs), then the interior tiles
+  int etile_offset = 0, entiles = 0;
+  int tile_offset = 0, ntiles = 0;
   int sat_off = 0, nsat = 0, sat_maxlen = 0;  // the group's block edges for the edge-SAT kernel
   int eval_off = 0, neval = 0, eval_maxlen = 0;  // the group's Eval edges (same list, after the SAT edges)
 };
@@ -254,6 +256,7 @@ struct sbp_system_s {
   // mixed form: 1 while y_n lives in the accumulator buffer (odd steps)
   int parity = 0;
   int forced_shape = -1;
+  int forced_schedule = -1;
   bool dirty = true;  // device descriptors / tiles need rebuilding
   // captured steps: graph of `graph_steps` RK4 steps for step size graph_dt
   hipGraphExec_t graph = nullptr;
@@ -407,13 +410,23 @@ int normalize(sbp_system_s* s) {
   return SBP_OK;
 }
 
-// One stage of one group: the SATs of its block edges and then its edge
-// tiles on the context's second stream, its interior tiles on `st`,
-// concurrently; `st` continues once both are done (also under graph
-// capture: a fork and a join).
-hipError_t launch_group(const sbp_system_s* s, const Group& g, int shape, int mode, const StageArgs& a, hipStream_t st) {
+// One stage of one group. Split schedule: the SATs of its block edges and
+// then its edge tiles on the context's second stream, its interior tiles on
+// `st`, concurrently; `st` continues once both are done (also under graph
+// capture: a fork and a join). Merged schedule: the edge SATs, then one
+// dispatch of the edge instantiation over all tiles, the edge tiles first
+// (it takes the interior path on interior tiles), all on `st`.
+hipError_t launch_group(const sbp_system_s* s, const Group& g, int shape, bool merged, int mode, const StageArgs& a,
+                        hipStream_t st) {
   const sbp_context_s* c = s->ctx;
   hipError_t e;
+  if (merged) {
+    if ((e = launch_eval_edges(a, s->d_satsegs + g.eval_off, g.neval, g.eval_maxlen, st)) != hipSuccess) return e;
+    if ((e = launch_edge_sat(a, s->d_satsegs + g.sat_off, g.nsat, g.sat_maxlen, mode, st)) != hipSuccess) return e;
+    StageArgs am = a;
+    am.tiles = s->d_tiles + g.etile_offset;
+    return launch_stage(g.op_eta, g.op_xi, g.upwind, true, shape, mode, am, g.entiles + g.ntiles, st);
+  }
   if ((e = hipEventRecord(c->ev_fork, st)) != hipSuccess) return e;
   if ((e = hipStreamWaitEvent(c->aux, c->ev_fork, 0)) != hipSuccess) return e;
   hipError_t ee = launch_eval_edges(a, s->d_satsegs + g.eval_off, g.neval, g.eval_maxlen, c->aux);
@@ -428,6 +441,25 @@ hipError_t launch_group(const sbp_system_s* s, const Group& g, int shape, int mo
   if ((e = hipEventRecord(c->ev_join, c->aux)) != hipSuccess) return e;
   if ((e = hipStreamWaitEvent(st, c->ev_join, 0)) != hipSuccess) return e;
   return ei;
+}
+
+// Autotuner timing of one group at one shape and schedule: a warm-up
+// stage, then three rounds of the stage-1..3 kernels (modes S0, S1, S2 read
+// y_n and write only the accumulator and the stage inputs, so the state is
+// untouched), between two events.
+hipError_t time_group(const sbp_system_s* s, const Group& g, int shape, bool merged, hipEvent_t e0, hipEvent_t e1,
+                      float& ms) {
+  hipStream_t st = s->ctx->stream;
+  const StageArgs a = stage_args(s, 0.0, 0, 0);
+  hipError_t e = launch_group(s, g, shape, merged, MODE_S0, a, st);
+  if (e != hipSuccess) return e;
+  if ((e = hipEventRecord(e0, st)) != hipSuccess) return e;
+  for (int r = 0; r < 3; ++r)
+    for (int m = MODE_S0; m <= MODE_S2; ++m)
+      if ((e = launch_group(s, g, shape, merged, m, a, st)) != hipSuccess) return e;
+  if ((e = hipEventRecord(e1, st)) != hipSuccess) return e;
+  if ((e = hipEventSynchronize(e1)) != hipSuccess) return e;
+  return hipEventElapsedTime(&ms, e0, e1);
 }
 
 int rebuild(sbp_system_s* s) {
@@ -455,27 +487,32 @@ int rebuild(sbp_system_s* s) {
     int64_t pts = 0;
     for (int bi : g.blocks) pts += (int64_t)s->blocks[bi].nx * s->blocks[bi].ny;
     auto key = std::make_tuple(g.op_eta, g.op_xi, tune_kind(s, g), pts, (int)g.blocks.size());
-    if (s->forced_shape >= 0) {
-      g.shape = s->forced_shape;
-    } else {
-      auto it = s->ctx->tuned.find(key);
-      g.shape = it != s->ctx->tuned.end() ? it->second : -1;
-    }
+    // cached choices: shape | schedule << 8, for groups tuned without a forced shape or schedule
+    auto it = s->ctx->tuned.find(key);
+    const bool cached = it != s->ctx->tuned.end() && s->forced_shape < 0 && s->forced_schedule < 0;
+    g.shape = s->forced_shape >= 0 ? s->forced_shape : cached ? (it->second & 0xff) : -1;
+    g.merged = s->forced_schedule >= 0 ? s->forced_schedule : cached ? (it->second >> 8) : -1;
   }
-  bool need_tune = false;
-  for (auto& g : s->groups) need_tune = need_tune || g.shape < 0;
+  bool need_tune = false, need_sched = false;
+  std::vector<bool> tune_cache(s->groups.size());
+  for (size_t gi = 0; gi < s->groups.size(); ++gi) {
+    const Group& g = s->groups[gi];
+    need_tune = need_tune || g.shape < 0;
+    need_sched = need_sched || g.merged < 0;
+    tune_cache[gi] = g.shape < 0 && g.merged < 0;
+  }
   auto upload = [&](void) -> int {
     tiles.clear();
     for (auto& g : s->groups) {
       const int sh = g.shape < 0 ? 0 : g.shape;
       std::vector<TileDev> ti, te;
       build_tiles(s, g, sh, ti, te);
-      g.tile_offset = (int)tiles.size();
-      g.ntiles = (int)ti.size();
-      tiles.insert(tiles.end(), ti.begin(), ti.end());
       g.etile_offset = (int)tiles.size();
       g.entiles = (int)te.size();
       tiles.insert(tiles.end(), te.begin(), te.end());
+      g.tile_offset = (int)tiles.size();
+      g.ntiles = (int)ti.size();
+      tiles.insert(tiles.end(), ti.begin(), ti.end());
       for (int bi : g.blocks) devs[bi] = make_dev(s, bi, sh);
     }
     if (s->d_blocks) hipFree(s->d_blocks);
@@ -596,19 +633,10 @@ int rebuild(sbp_system_s* s) {
         for (size_t gi = 0; gi < s->groups.size(); ++gi) {
           Group& g = s->groups[gi];
           if (saved[gi] >= 0) continue;
-          const StageArgs a = stage_args(s, 0.0, 0, 0);
-          const hipError_t w = launch_group(s, g, sh, MODE_S0, a, st);  // warm-up
+          float ms = 0.f;
+          const hipError_t w = time_group(s, g, sh, g.merged == 1, e0, e1, ms);
           if (w == hipErrorNotSupported) continue;  // configuration not built for this operator pair
           SBP_TRY_HIP(w);
-          const int reps = 3;
-          SBP_TRY_HIP(hipEventRecord(e0, st));
-          for (int r = 0; r < reps; ++r)
-            for (int m = MODE_S0; m <= MODE_S2; ++m)
-              SBP_TRY_HIP(launch_group(s, g, sh, m, a, st));
-          SBP_TRY_HIP(hipEventRecord(e1, st));
-          SBP_TRY_HIP(hipEventSynchronize(e1));
-          float ms = 0.f;
-          SBP_TRY_HIP(hipEventElapsedTime(&ms, e0, e1));
           cfg_ms[gi][sh] = std::min(cfg_ms[gi][sh], ms);
         }
         for (size_t gi = 0; gi < s->groups.size(); ++gi) s->groups[gi].shape = saved[gi];
@@ -623,14 +651,38 @@ int rebuild(sbp_system_s* s) {
     hipEventDestroy(e1);
     // the timing launches may have raised error bits on garbage accumulators: clear
     SBP_TRY_HIP(hipMemsetAsync(s->d_err, 0, sizeof(unsigned), st));
-    for (size_t gi = 0; gi < s->groups.size(); ++gi) {
-      Group& g = s->groups[gi];
-      if (g.shape >= 0) continue;
-      g.shape = best[gi];
-      int64_t pts = 0;
-      for (int bi : g.blocks) pts += (int64_t)s->blocks[bi].nx * s->blocks[bi].ny;
-      s->ctx->tuned[std::make_tuple(g.op_eta, g.op_xi, tune_kind(s, g), pts, (int)g.blocks.size())] = g.shape;
+    for (size_t gi = 0; gi < s->groups.size(); ++gi)
+      if (s->groups[gi].shape < 0) s->groups[gi].shape = best[gi];
+  }
+  if (need_sched) {
+    // schedule at the chosen shape: split and merged, two rounds each, the
+    // smaller time of each
+    SBP_TRY(upload());
+    hipEvent_t e0, e1;
+    SBP_TRY_HIP(hipEventCreate(&e0));
+    SBP_TRY_HIP(hipEventCreate(&e1));
+    for (auto& g : s->groups) {
+      if (g.merged >= 0) continue;
+      float t[2] = {1e30f, 1e30f};
+      for (int round = 0; round < 2; ++round)
+        for (int m = 0; m < 2; ++m) {
+          float ms = 0.f;
+          SBP_TRY_HIP(time_group(s, g, g.shape, m == 1, e0, e1, ms));
+          t[m] = std::min(t[m], ms);
+        }
+      g.merged = t[1] < t[0] ? 1 : 0;
     }
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    SBP_TRY_HIP(hipMemsetAsync(s->d_err, 0, sizeof(unsigned), st));
+  }
+  for (size_t gi = 0; gi < s->groups.size(); ++gi) {
+    const Group& g = s->groups[gi];
+    if (!tune_cache[gi]) continue;
+    int64_t pts = 0;
+    for (int bi : g.blocks) pts += (int64_t)s->blocks[bi].nx * s->blocks[bi].ny;
+    s->ctx->tuned[std::make_tuple(g.op_eta, g.op_xi, tune_kind(s, g), pts, (int)g.blocks.size())] =
+        g.shape | (g.merged << 8);
   }
   SBP_TRY(upload());
   destroy_graph(s);
@@ -651,7 +703,7 @@ int check_err(sbp_system_s* s) {
 int enqueue_step(sbp_system_s* s, double dt, int swap, hipStream_t st) {
   for (int stage = 0; stage < 4; ++stage) {
     const StageArgs a = stage_args(s, dt, stage, swap);
-    for (const Group& g : s->groups) SBP_TRY_HIP(launch_group(s, g, g.shape, stage, a, st));
+    for (const Group& g : s->groups) SBP_TRY_HIP(launch_group(s, g, g.shape, g.merged == 1, stage, a, st));
   }
   SBP_TRY_HIP(launch_advance_time(s->d_time, dt, st));
   return SBP_OK;
@@ -1569,7 +1621,7 @@ int sbp_system_rhs(sbp_system s, double t, double* k_host) {
   StageArgs a = stage_args(s, 0.0, 0, 0);
   a.time = s->d_time_rhs;
   a.rk_b = a.rk_a = 0.0;
-  for (const Group& g : s->groups) SBP_TRY_HIP(launch_group(s, g, g.shape, MODE_RHS, a, st));
+  for (const Group& g : s->groups) SBP_TRY_HIP(launch_group(s, g, g.shape, g.merged == 1, MODE_RHS, a, st));
   size_t off = 0;
   for (auto& h : s->blocks) {
     for (int c = 0; c < 4; ++c)
@@ -1723,7 +1775,7 @@ int sbp_system_profile(sbp_system s, double dt, int64_t nsteps, double* ms_total
         SBP_TRY_HIP(hipEventCreate(&a0));
         SBP_TRY_HIP(hipEventCreate(&a1));
         SBP_TRY_HIP(hipEventRecord(a0, st));
-        SBP_TRY_HIP(launch_group(s, g, g.shape, stage, a, st));
+        SBP_TRY_HIP(launch_group(s, g, g.shape, g.merged == 1, stage, a, st));
         SBP_TRY_HIP(hipEventRecord(a1, st));
         ev.push_back(a0);
         ev.push_back(a1);
@@ -1805,7 +1857,7 @@ int sbp_system_advance_adaptive(sbp_system s, double dt, double* guess_dt, doubl
       a.rk_a = a.rk_b = 0.0;
       a.rhs_in_sel = i == 0 ? 0 : SET_STAGE + 1;
       a.rhs_out_sel = i + 1;
-      for (const Group& grp : s->groups) SBP_TRY_HIP(launch_group(s, grp, grp.shape, MODE_RHS, a, st));
+      for (const Group& grp : s->groups) SBP_TRY_HIP(launch_group(s, grp, grp.shape, grp.merged == 1, MODE_RHS, a, st));
     }
     SBP_TRY(combine(Bw, 4, g, -1));
     SBP_TRY(combine(Bs, 4, g, SET_SECOND));
@@ -1879,7 +1931,7 @@ int sbp_system_integrate(sbp_system s, int nstages, const double* A, const doubl
       a.rk_a = a.rk_b = 0.0;
       a.rhs_in_sel = i == 0 ? 0 : SET_STAGE + 1;
       a.rhs_out_sel = i + 1;
-      for (const Group& grp : s->groups) SBP_TRY_HIP(launch_group(s, grp, grp.shape, MODE_RHS, a, st));
+      for (const Group& grp : s->groups) SBP_TRY_HIP(launch_group(s, grp, grp.shape, grp.merged == 1, MODE_RHS, a, st));
     }
     SBP_TRY(combine(B, nstages, final_out.data()));
     SBP_TRY_HIP(launch_advance_time(s->d_time, dt, st));
@@ -1934,6 +1986,20 @@ int sbp_system_set_tile_shape(sbp_system s, int shape_index) {
   if (!s || shape_index < -1 || shape_index >= kNumTileShapes) return SBP_ERR_INVALID;
   s->forced_shape = shape_index;
   s->dirty = true;
+  return SBP_OK;
+}
+
+int sbp_system_set_schedule(sbp_system s, int schedule) {
+  if (!s || schedule < -1 || schedule > 1) return SBP_ERR_INVALID;
+  s->forced_schedule = schedule;
+  s->dirty = true;
+  return SBP_OK;
+}
+
+int sbp_system_schedule(sbp_system s, int* schedule) {
+  if (!s || !schedule) return SBP_ERR_INVALID;
+  SBP_TRY(rebuild(s));
+  *schedule = s->groups.empty() ? 0 : s->groups[0].merged;
   return SBP_OK;
 }
 

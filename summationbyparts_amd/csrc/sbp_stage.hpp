@@ -329,6 +329,19 @@ __global__ __launch_bounds__((TX / RX) * (TY / RY)) __attribute__((amdgpu_waves_
       for (int s = 0; s < RY; ++s) {
         if (!(yin[s] && any)) continue;
         const CT* line = lds + (size_t)(tr * RY + s) * WX;
+        if constexpr (EDGE) {
+          if (!interior) {
+            // closure rows: each coefficient read once for the four components
+#pragma unroll
+            for (int i = 0; i < RX; ++i) {
+              CT o[4] = {CT(0), CT(0), CT(0), CT(0)};
+              if (xin[i]) row_sum4<BB>(line + slot * NP, NP, 1, xb + i, nx, rx0, o);
+#pragma unroll
+              for (int q = 0; q < 4; ++q) acc[s][i][q] = first ? sg * o[q] : fma_t(sg, o[q], acc[s][i][q]);
+            }
+            continue;
+          }
+        }
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
           const CT* lq = line + (slot + q) * NP;
@@ -337,7 +350,7 @@ __global__ __launch_bounds__((TX / RX) * (TY / RY)) __attribute__((amdgpu_waves_
             row_window<BB, RX>(lq + (xb - rx0 - HX), o);
           } else {
 #pragma unroll
-            for (int i = 0; i < RX; ++i) o[i] = xin[i] ? row_sum<BB, EDGE>(lq, 1, xb + i, nx, rx0) : CT(0);
+            for (int i = 0; i < RX; ++i) o[i] = xin[i] ? row_sum<BB, false>(lq, 1, xb + i, nx, rx0) : CT(0);
           }
 #pragma unroll
           for (int i = 0; i < RX; ++i) acc[s][i][q] = first ? sg * o[i] : fma_t(sg, o[i], acc[s][i][q]);
@@ -508,6 +521,20 @@ __global__ __launch_bounds__((TX / RX) * (TY / RY)) __attribute__((amdgpu_waves_
     auto eta_pass = [&](auto blk, int slot, CT sg) {
       using BB = decltype(blk);
       if (!any) return;
+      if constexpr (EDGE) {
+        if (!interior) {
+#pragma unroll
+          for (int s = 0; s < RY; ++s)
+#pragma unroll
+            for (int i = 0; i < RX; ++i) {
+              CT o[4] = {CT(0), CT(0), CT(0), CT(0)};
+              if (yin[s] && xin[i]) row_sum4<BB>(lds + slot * NP + tc * RX + i, NP, TX, yb + s, ny, ry0, o);
+#pragma unroll
+              for (int q = 0; q < 4; ++q) acc[s][i][q] = fma_t(sg, o[q], acc[s][i][q]);
+            }
+          return;
+        }
+      }
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
         const CT* col = lds + (slot + q) * NP + tc * RX;
@@ -519,7 +546,7 @@ __global__ __launch_bounds__((TX / RX) * (TY / RY)) __attribute__((amdgpu_waves_
           for (int s = 0; s < RY; ++s)
 #pragma unroll
             for (int i = 0; i < RX; ++i)
-              o[s][i] = (yin[s] && xin[i]) ? row_sum<BB, EDGE>(col + i, TX, yb + s, ny, ry0) : CT(0);
+              o[s][i] = (yin[s] && xin[i]) ? row_sum<BB, false>(col + i, TX, yb + s, ny, ry0) : CT(0);
         }
 #pragma unroll
         for (int s = 0; s < RY; ++s)

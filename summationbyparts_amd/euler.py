@@ -130,6 +130,16 @@ class BlockSystem:
         _lib.check(_lib.lib().sbp_system_tile_index(self._h, ctypes.byref(i)), "sbp_system_tile_index")
         return i.value
 
+    def set_schedule(self, schedule):
+        """Stage schedule: 0 split over two streams, 1 one dispatch over all
+        tiles, -1 autotuned (results do not depend on it)."""
+        _lib.check(_lib.lib().sbp_system_set_schedule(self._h, schedule), "sbp_system_set_schedule")
+
+    def schedule(self):
+        i = ctypes.c_int()
+        _lib.check(_lib.lib().sbp_system_schedule(self._h, ctypes.byref(i)), "sbp_system_schedule")
+        return i.value
+
     # ---- state
     def set_state(self, block, q):
         q = np.ascontiguousarray(q, dtype=np.float64)

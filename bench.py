@@ -565,16 +565,31 @@ def mixed_leg(n, steps, warmup, fp64_value):
 def run_c5_pair(steps, warmup, no_cpu):
     """C5 in FP64 and in the mixed form (the same batch, steps and timing),
     and the mixed form's error against FP64 per solve after the timed steps."""
-    res, states = {}, {}
+    import torch
+    from summationbyparts_amd import euler, operators
+
+    res, states, verr = {}, {}, {}
     base = None
     for prec in ("fp64", "mixed"):
         w, batch = build_c5(prec)
         t = time_steps(w, steps, warmup)
         res[prec] = line("c5", w, prec, t, steps, warmup, traffic_key="c5")
         states[prec] = [w.sys.get_state(i) for i in range(len(batch))]
+        # System::error of every solve against its own analytic vortex
+        verr[prec] = [w.sys.error_vortex(euler.VortexParameters([euler.Vortice(-1.0, 0.0, b["rstar"], b["eps"])],
+                                                                 b["mach"]), block=i) for i, b in enumerate(batch)]
         if prec == "fp64" and not no_cpu:
             base = cpu_baseline(w)
         w.sys.close()
+    # the mixed-vs-FP64 difference in the H-norm of the error, against the
+    # discretisation error of the FP64 solve
+    op = operators.Upwind9()
+    ratio_disc, change = 0.0, 0.0
+    for i in range(len(states["fp64"])):
+        a = torch.from_numpy(states["mixed"][i]).cuda()
+        b = torch.from_numpy(states["fp64"][i]).cuda()
+        ratio_disc = max(ratio_disc, euler.h2_err(a, b, op) / verr["fp64"][i])
+        change = max(change, abs(verr["mixed"][i] - verr["fp64"][i]) / verr["fp64"][i])
     worst = {"rel_l2": 0.0, "rel_max": 0.0}
     comp_l2, comp_max = [0.0] * 4, [0.0] * 4
     for a, b in zip(states["mixed"], states["fp64"]):
@@ -588,6 +603,8 @@ def run_c5_pair(steps, warmup, no_cpu):
                                "worst_solve_rel_l2_per_component": comp_l2,
                                "worst_solve_rel_max_per_component": comp_max, "tolerance": MIXED_TOL,
                                "within_tolerance": worst["rel_l2"] <= MIXED_TOL,
+                               "worst_solve_diff_over_discretisation_error": ratio_disc,
+                               "worst_solve_rel_change_of_vortex_error": change,
                                "accuracy_run": f"{warmup + steps} RK4 steps of every solve, same initial states"}
     if base is not None:
         res["fp64"]["cpu_baseline"] = base

@@ -6,9 +6,13 @@ oracle directly.
 Tolerances (relative to the FP64 result):
 - right-hand side, relative to max|k|: MIX_RHS_TOL;
 - states after many steps: relative L2 of the whole state (all four
-  components) below 1e-8; per-component relative L2 and max errors are
-  printed (ρv, which vanishes outside the vortex, has the largest relative
-  error) and bounded by MIX_COMPONENT_TOL.
+  components) below MIX_STATE_TOL = 1e-8, and each component's error
+  ||a_c - b_c|| over the whole state's norm ||b|| below the same 1e-8 (the
+  definition bench.py's mixed leg flags). Each component's error over its
+  own norm is printed and bounded by the looser MIX_COMPONENT_TOL: ρv
+  vanishes outside the vortex, so its own norm is small and its
+  self-normalised error (1.5e-8 on C4) exceeds 1e-8 while its share of the
+  state's error does not.
 """
 import numpy as np
 import pytest
@@ -46,6 +50,13 @@ def rel_l2(a, b):
 def rel_l2_all(a, b):
     """||a - b||_2 / ||b||_2 over the whole state."""
     return float(np.linalg.norm((a - b).ravel()) / np.linalg.norm(b.ravel()))
+
+
+def l2_over_state(a, b):
+    """||a_c - b_c||_2 / ||b||_2 per component (each component's share of the
+    whole-state relative error), as a list."""
+    nb = np.linalg.norm(b.ravel())
+    return [float(np.linalg.norm(a[c] - b[c]) / nb) for c in range(a.shape[0])]
 
 
 def _system(X, Y, name, precision):
@@ -162,7 +173,10 @@ def test_c4_mixed_full(gpu):
     l2 = rel_l2(out["mixed"], out["fp64"])
     mx = [float(np.max(np.abs(out["mixed"][c] - out["fp64"][c])) / np.max(np.abs(out["fp64"][c]))) for c in range(4)]
     tot = rel_l2_all(out["mixed"], out["fp64"])
-    print("C4 mixed vs fp64: rel L2", tot, "per component", l2, "rel max", mx, "energy drift", drift)
+    share = l2_over_state(out["mixed"], out["fp64"])
+    print("C4 mixed vs fp64: rel L2", tot, "per component", l2, "over the state", share, "rel max", mx,
+          "energy drift", drift)
     assert tot < MIX_STATE_TOL
+    assert max(share) < MIX_STATE_TOL
     assert max(l2) < MIX_COMPONENT_TOL
     assert abs(drift["mixed"] - drift["fp64"]) < 1e-8

@@ -167,3 +167,17 @@ def test_full_run_and_error(gpu, name):
     e_ref = ref.error_vortex(O.VortexParams.make([(v.x0, v.y0, v.rstar, v.eps) for v in vp.vortices], vp.mach))
     print(f"{name}: {nsteps} steps of dt {dt:.6g}, error {e:.6e} (oracle {e_ref:.6e})")
     assert abs(e - e_ref) <= 1e-8 * e_ref
+
+
+@pytest.mark.parametrize("name", ["interpolation.json", "expressions_vortex.json", "tsection.json"])
+@pytest.mark.parametrize("schedule", [0, 1])
+def test_example_configuration_both_schedules(gpu, name, schedule):
+    """Interpolate, MultiGrid and Eval edges in each stage schedule (split:
+    edge SATs beside the interior tiles; merged: one dispatch over all
+    tiles), three steps per step against the oracle."""
+    from summationbyparts_amd import config
+    rt = config.load(_text(name))
+    sys_ = rt.system()
+    sys_.set_schedule(schedule)
+    assert sys_.schedule() == schedule
+    _check_steps(rt, sys_, _oracle_system(rt))

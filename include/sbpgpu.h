@@ -158,7 +158,9 @@ int sbp_interp_coefficients(int interp_op, int fine2coarse, double* block, doubl
  * multigrid/src/parsing.rs:164). k (out), y (in): (4, ny, nx) contiguous;
  * metrics: (5, ny, nx) contiguous; boundaries[dir]: 4 x n contiguous arrays
  * (n = nx for north/south, ny for east/west) holding the boundary data g of
- * BoundaryTerms (euler/src/lib.rs:760). kind: SBP_RHS_TRAD or SBP_RHS_UPWIND. */
+ * BoundaryTerms (euler/src/lib.rs:760). kind: SBP_RHS_TRAD or SBP_RHS_UPWIND.
+ * SBP_ERR_SIZE when 5 x nx x ny doubles reach 2^32 bytes (also for
+ * sbp_rhs_no_sat). */
 int sbp_rhs(sbp_context ctx, int kind, int op_eta, int op_xi, int ny, int nx, double* k, const double* y,
             const double* metrics, const double* const* boundaries);
 
@@ -215,7 +217,11 @@ int sbp_metrics(sbp_context ctx, int op_eta, int op_xi, int ny, int nx, const do
 /* Blocks with sizes (ny[i], nx[i]), operator pairs (op_eta[i], op_xi[i]) and
  * host coordinate arrays x[i], y[i] of shape (ny[i], nx[i]). Metrics are
  * computed on the device (BaseSystem::create, multigrid/src/system.rs:73-78).
- * Boundary conditions default to SBP_BC_THIS on every edge. */
+ * Boundary conditions default to SBP_BC_THIS on every edge. At most 32767
+ * blocks; SBP_ERR_SHAPE for a block below its operators' closures,
+ * SBP_ERR_SIZE for a block whose 5 planes (nx rounded up to a multiple of
+ * 16, times ny, doubles) reach 2^32 bytes (about 10 300 x 10 300 points),
+ * both checked before the context and the coordinate pointers. */
 int sbp_system_create(sbp_context ctx, int nblocks, const int* ny, const int* nx, const int* op_eta, const int* op_xi,
                       const double* const* x, const double* const* y, sbp_system* out);
 int sbp_system_destroy(sbp_system sys);

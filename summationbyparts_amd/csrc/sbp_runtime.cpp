@@ -149,6 +149,10 @@ VortexDev to_dev(const sbp_vortex_params& p) {
 
 int64_t pitch_of(int nx) { return ((int64_t)nx + 15) / 16 * 16; }
 
+// The stage kernels address 5 planes of a block with 32-bit byte offsets:
+// 5 planes of pitch x ny doubles stay below 2^32 bytes.
+bool planes_fit(int nx, int ny) { return 5 * pitch_of(nx) * (int64_t)ny * 8 < ((int64_t)1 << 32); }
+
 }  // namespace
 
 // Device scratch of sbp_h2_err_noalloc: the per-workgroup partial sums and
@@ -1069,7 +1073,8 @@ static int ctx_check_err(sbp_context ctx) {
 static int rhs_block(sbp_context ctx, bool up, int op_eta, int op_xi, int ny, int nx, double* k, const double* y,
                      const double* metrics, const double* const* boundaries) {
   SBP_TRY(check_shape(op_eta, op_xi, ny, nx));
-  if (5 * (int64_t)nx * ny * 8 >= ((int64_t)1 << 32)) return SBP_ERR_INVALID;
+  // contiguous planes (pitch nx) with the stage kernels' 32-bit offsets
+  if (5 * (int64_t)nx * ny * 8 >= ((int64_t)1 << 32)) return SBP_ERR_SIZE;
   OpInfo oe, ox;
   op_info(op_eta, oe);
   op_info(op_xi, ox);
@@ -1277,11 +1282,15 @@ int sbp_h2linspace(double start, double end, int n, double* out) {
 
 int sbp_system_create(sbp_context ctx, int nblocks, const int* ny, const int* nx, const int* op_eta, const int* op_xi,
                       const double* const* x, const double* const* y, sbp_system* out) {
-  if (!ctx || nblocks <= 0 || nblocks > 32767 || !ny || !nx || !op_eta || !op_xi || !x || !y || !out) return SBP_ERR_INVALID;
+  if (nblocks <= 0 || nblocks > 32767 || !ny || !nx || !op_eta || !op_xi) return SBP_ERR_INVALID;
+  // sizes first (host-side checks, before the handles)
   for (int i = 0; i < nblocks; ++i) {
     SBP_TRY(check_shape(op_eta[i], op_xi[i], ny[i], nx[i]));
-    if (!x[i] || !y[i] || nx[i] > 32767 * 16 || ny[i] > 32767 * 16) return SBP_ERR_INVALID;
+    if (nx[i] > 32767 * 16 || ny[i] > 32767 * 16 || !planes_fit(nx[i], ny[i])) return SBP_ERR_SIZE;
   }
+  if (!ctx || !x || !y || !out) return SBP_ERR_INVALID;
+  for (int i = 0; i < nblocks; ++i)
+    if (!x[i] || !y[i]) return SBP_ERR_INVALID;
   SBP_TRY_HIP(hipSetDevice(ctx->device));
   std::unique_ptr<sbp_system_s> s(new sbp_system_s());
   s->ctx = ctx;
@@ -1299,8 +1308,6 @@ int sbp_system_create(sbp_context ctx, int nblocks, const int* ny, const int* nx
     h.op_xi = op_xi[i];
     h.ld = pitch_of(h.nx);
     h.ps = h.ld * h.ny;
-    // the stage kernels address 5 planes with 32-bit byte offsets
-    if (5 * h.ps * 8 >= ((int64_t)1 << 32)) return fail(SBP_ERR_INVALID);
     std::memset(h.bc_vortex, 0, sizeof(h.bc_vortex));
   }
   int rc = SBP_OK;

@@ -175,3 +175,28 @@ def test_h2_err_size_checks_before_handles():
         assert fn(None, _lib.UPWIND9, _lib.UPWIND9, 10, 40, None, None, None) == _lib.ERR_SHAPE
         assert fn(None, _lib.UPWIND9, _lib.UPWIND9, 40, 40, None, None, None) == _lib.ERR_INVALID
         assert fn(None, 17, _lib.UPWIND9, 40, 40, None, None, None) == _lib.ERR_INVALID
+
+
+def test_system_create_size_checks_before_handles():
+    """sbp_system_create checks every block's size on the host before the
+    context: SBP_ERR_SHAPE below the operators' closures, SBP_ERR_SIZE when
+    the 5 planes of a block (pitch x ny doubles) reach 2^32 bytes (the stage
+    kernels' 32-bit offsets), then SBP_ERR_INVALID for the missing context."""
+    import ctypes
+    from summationbyparts_amd import _lib
+    L = _lib.lib()
+
+    def create(shapes, ops=(_lib.UPWIND9, _lib.UPWIND9)):
+        n = len(shapes)
+        ny = (ctypes.c_int * n)(*[s[0] for s in shapes])
+        nx = (ctypes.c_int * n)(*[s[1] for s in shapes])
+        oe = (ctypes.c_int * n)(*[ops[0]] * n)
+        ox = (ctypes.c_int * n)(*[ops[1]] * n)
+        return L.sbp_system_create(None, n, ny, nx, oe, ox, None, None, None)
+
+    # 10 368^2: pitch 10 368, 5 planes = 4.30e9 bytes > 2^32; 10 352^2 fits
+    assert create([(64, 64), (10368, 10368)]) == _lib.ERR_SIZE
+    assert create([(64, 64), (10352, 10352)]) == _lib.ERR_INVALID
+    assert create([(64, 64), (10, 40)]) == _lib.ERR_SHAPE
+    assert create([(64, 64)]) == _lib.ERR_INVALID
+    assert create([]) == _lib.ERR_INVALID

@@ -45,15 +45,15 @@ HBM_PEAK_GBS = 8000.0
 BYTES_PER_POINT_STEP = {"fp64": 672.0, "mixed": 576.0}
 # This kernel's own schedule moves fewer bytes. FP64 (J formed from the four
 # metric products, so 4 metric planes): stage 1 reads y_n, metrics, writes
-# Y2 (96 B); stage 2 reads Y2, metrics, y_n, writes Y3 (128 B); stage 3 reads
-# Y3, metrics, y_n, Y2, writes Y4 and W = Y2 + 2 Y3 - y_n (192 B); stage 4
-# reads Y4, metrics, W and writes y_{n+1} (128 B): 544 B.
+# Y2 (96 B); stage 2 reads Y2, metrics, y_n, writes Y3 and W = Y2 + 2 Y3 - y_n
+# (160 B); stage 3 reads Y3, metrics, y_n, writes Y4 (128 B); stage 4 reads
+# Y4, metrics, W and writes y_{n+1} (128 B): 512 B.
 # Mixed: the stage inputs are y_n + Δ_s with FP32 increments Δ_s, and the
 # last stage sums y_n and the increments in FP64: stage 1 reads y_n, metrics,
 # writes Δ_1 (80 B); stages 2-3 read y_n, Δ_s, metrics and write Δ_{s+1}
 # (96 B each); stage 4 reads y_n, Δ_3, metrics, Δ_1, Δ_2 and writes y_{n+1}
 # (144 B): 416 B.
-OWN_BYTES_PER_POINT_STEP = {"fp64": 544.0, "mixed": 416.0}
+OWN_BYTES_PER_POINT_STEP = {"fp64": 512.0, "mixed": 416.0}
 # advance_adaptive's embedded Bogacki-Shampine step, unfused: four RHS passes
 # (stage input, 4 metric planes in, k out: 96 B each), the stage inputs
 # (y_n + k0; y_n + k1; y_n + k0..k2: 96, 96, 160 B), fut (160 B), the

@@ -216,9 +216,12 @@ __device__ __forceinline__ void tile_place(int t, int K, int n, int T, int M, in
 //   û = jx u + jy v,  flux = [ρû, ρu û + jx p, ρv û + jy p, (e+p) û]
 //   α = |û| + c √(jx²+jy²),  aq = α q.
 //   c √(jx²+jy²) is taken as one square root, √(γ p/ρ · (jx²+jy²)).
+// The density and pressure checks of upwind_dissipation (lib.rs:636-646) run
+// when `check` is set: the stage kernel sets it at the points a tile owns in
+// its ξ phase, which cover every point of a block once.
 template <bool UP>
 __device__ __forceinline__ void face_terms(double rho, double ru, double rv, double e, double jx, double jy, double gamma,
-                                           double f[4], double aq[4], double& alpha, unsigned& err) {
+                                           double f[4], double aq[4], double& alpha, unsigned& err, bool check = true) {
   const double ir = frcp(rho);
   const double u = ru * ir, v = rv * ir;
   const double p = (gamma - 1.0) * (e - (ru * ru + rv * rv) * (0.5 * ir));
@@ -228,8 +231,10 @@ __device__ __forceinline__ void face_terms(double rho, double ru, double rv, dou
   f[2] = rv * uh + jy * p;
   f[3] = (e + p) * uh;
   if constexpr (UP) {
-    if (!(rho > 0.0)) err |= ERRBIT_RHO;
-    if (!(p > 0.0)) err |= ERRBIT_PRESSURE;
+    if (check) {
+      if (!(rho > 0.0)) err |= ERRBIT_RHO;
+      if (!(p > 0.0)) err |= ERRBIT_PRESSURE;
+    }
     const double c2g = gamma * p * ir * (jx * jx + jy * jy);
     const double al = fabs(uh) + fsqrt(fmax(c2g, 1e-300));
     alpha = al;
@@ -289,9 +294,9 @@ __device__ __forceinline__ MixRef64 mix_ref64(const double q[4], double jx, doub
 
 template <bool UP>
 __device__ __forceinline__ void face_terms_mix64(const double q[4], double jx, double jy, const MixRef64& R, double gamma,
-                                                 float f[4], float aq[4], unsigned& err) {
+                                                 float f[4], float aq[4], unsigned& err, bool check = true) {
   double f64[4], aq64[4], al;
-  face_terms<UP>(q[0], q[1], q[2], q[3], jx, jy, gamma, f64, aq64, al, err);
+  face_terms<UP>(q[0], q[1], q[2], q[3], jx, jy, gamma, f64, aq64, al, err, check);
 #pragma unroll
   for (int c = 0; c < 4; ++c) f[c] = (float)(f64[c] - (jx * R.ec[c] + jy * R.fc[c]));
   if constexpr (UP) {
@@ -398,9 +403,10 @@ __device__ __forceinline__ void vortex_point(const VortexDev& vp, double gamma, 
   }
 }
 
-// FP64 stage inputs: y_n, Y2 (BUF_YA), Y3 (BUF_YB), Y4 (BUF_ACC); the
-// right-hand-side mode reads rhs_in or the selected work set.
-// Y2, Y3, Y4 (and W over Y2) are internal to a step and stored point-major:
+// FP64 stage inputs: y_n, Y2 (BUF_YA), Y3 (BUF_YB), Y4 (BUF_YA again: Y2 is
+// dead once stage 2 has formed W into BUF_ACC); the right-hand-side mode
+// reads rhs_in or the selected work set.
+// Y2, Y3, Y4 and W are internal to a step and stored point-major:
 // the four components of point o at doubles 4o .. 4o + 3, two 16-byte loads
 // per point; y_n and every buffer a caller sees keep the plane layout.
 __device__ __forceinline__ bool stage_input_pm(const StageArgs& A, int mode) { return !A.mixed && mode >= MODE_S1; }
@@ -435,7 +441,7 @@ __device__ __forceinline__ const double* stage_input(const StageArgs& A, const B
     case MODE_S0: return B.buf[BUF_YN];
     case MODE_S1: return B.buf[BUF_YA];
     case MODE_S2: return B.buf[BUF_YB];
-    case MODE_S3: return B.buf[BUF_ACC];
+    case MODE_S3: return B.buf[BUF_YA];
     default: return A.rhs_in_sel ? B.rk[A.rhs_in_sel - 1] : B.rhs_in;
   }
 }

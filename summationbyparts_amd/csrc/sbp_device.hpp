@@ -13,10 +13,11 @@ struct Program;
 constexpr int kMaxWorkSets = 10;
 
 // Roles of the four state buffers of a block during one fused RK4 step.
-// FP64 form: y_n and the stage inputs Y2 (YA), Y3 (YB) and Y4 (ACC); stage 3
-// overwrites Y2 with W = Y2 + 2 Y3 - y_n, from which and Y4 the last stage
-// forms y_{n+1} (see the stage kernel's finish). Y2, Y3, Y4 and W are stored
-// point-major (the four components of a point adjacent), y_n in planes.
+// FP64 form: y_n and the stage inputs Y2 (YA), Y3 (YB) and Y4 (YA, written
+// by stage 3 over Y2); stage 2 also writes W = Y2 + 2 Y3 - y_n into ACC,
+// from which and Y4 the last stage forms y_{n+1} (see the stage kernel's
+// finish). Y2, Y3, Y4 and W are stored point-major (the four components of a
+// point adjacent), y_n in planes.
 // Mixed form: YA holds the FP32 stage increments Δ_1 = Y2 - y_n and
 // Δ_2 = Y3 - y_n (two sets of 4·ps floats, the second at float offset 4·ps;
 // the four components of a point are adjacent), YB holds Δ_3 = Y4 - y_n in

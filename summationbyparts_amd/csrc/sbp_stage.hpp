@@ -299,10 +299,12 @@ __global__ __launch_bounds__((TX / RX) * (TY / RY)) __attribute__((amdgpu_waves_
           const int l = r * WX + c;
           CT f[4], aq[4];
           double al = 0.0;
+          // ρ and p are checked once per point: at the points this tile owns
+          const bool chk = !OWNQ || it < RY;
           if constexpr (MIX) {
-            face_terms_mix64<UP>(v[j], v[j][4], v[j][5], R, gamma, f, aq, err);
+            face_terms_mix64<UP>(v[j], v[j][4], v[j][5], R, gamma, f, aq, err, chk);
           } else {
-            face_terms<UP>(v[j][0], v[j][1], v[j][2], v[j][3], v[j][4], v[j][5], gamma, f, aq, al, err);
+            face_terms<UP>(v[j][0], v[j][1], v[j][2], v[j][3], v[j][4], v[j][5], gamma, f, aq, al, err, chk);
           }
 #pragma unroll
           for (int q = 0; q < 4; ++q) lds[q * NP + l] = f[q];
@@ -389,9 +391,9 @@ __global__ __launch_bounds__((TX / RX) * (TY / RY)) __attribute__((amdgpu_waves_
 
   // Finish operands at the output points, all loaded before any store.
   // FP64 (see the finish), with q the stage input held from the η phase:
-  //   S0 q = y_n;  S1 fa = y_n, q = Y2;  S2 fa = y_n, fb = Y2, q = Y3;  S3 fa = W, q = Y4
+  //   S0 q = y_n;  S1 fa = y_n, q = Y2;  S2 fa = y_n, q = Y3;  S3 fa = W, q = Y4
   // mixed: S3 fd = Δ_1, Δ_2 (S0-S2 need no operand)
-  double fJ[RY][RX], fa[RY][RX][4], fb[RY][RX][4];
+  double fJ[RY][RX], fa[RY][RX][4];
   f4 fd[MIX ? RY : 1][MIX ? RX : 1][2];
   static_assert(OWNQ, "the stage combination takes the stage input from the η phase");
   auto load_finish = [&]() {
@@ -412,12 +414,11 @@ __global__ __launch_bounds__((TX / RX) * (TY / RY)) __attribute__((amdgpu_waves_
             for (int t = 0; t < 2; ++t) fd[MIX ? s : 0][MIX ? i : 0][t] = ldg_nt(delta_set(B, t), o);
           }
         } else if (mode == MODE_S3) {
-          load_pm_nt(B.buf[BUF_YA], o, fa[s][i]);  // W, point-major in the Y2 buffer
+          load_pm_nt(B.buf[BUF_ACC], o, fa[s][i]);  // W, point-major
         } else if (mode != MODE_S0) {
-          // S1, S2: fa = y_n (planes); S2 also fb = Y2 (point-major)
+          // S1, S2: fa = y_n (planes)
 #pragma unroll
           for (int q = 0; q < 4; ++q) fa[s][i][q] = ldg_nt((const double*)B.buf[BUF_YN] + q * ps, o);
-          if (mode == MODE_S2) load_pm_nt(B.buf[BUF_YA], o, fb[s][i]);
         }
       }
   };
@@ -495,10 +496,11 @@ __global__ __launch_bounds__((TX / RX) * (TY / RY)) __attribute__((amdgpu_waves_
           const int l = r * TX + c;
           CT f[4], aq[4];
           double al = 0.0;
+          // (ρ and p were checked in the ξ phase)
           if constexpr (MIX) {
-            face_terms_mix64<UP>(v[j], v[j][4], v[j][5], R, gamma, f, aq, err);
+            face_terms_mix64<UP>(v[j], v[j][4], v[j][5], R, gamma, f, aq, err, !OWNQ);
           } else {
-            face_terms<UP>(v[j][0], v[j][1], v[j][2], v[j][3], v[j][4], v[j][5], gamma, f, aq, al, err);
+            face_terms<UP>(v[j][0], v[j][1], v[j][2], v[j][3], v[j][4], v[j][5], gamma, f, aq, al, err, !OWNQ);
           }
 #pragma unroll
           for (int q = 0; q < 4; ++q) lds[q * NP + l] = f[q];
@@ -652,26 +654,28 @@ __global__ __launch_bounds__((TX / RX) * (TY / RY)) __attribute__((amdgpu_waves_
       // Y2 = y + dt/2 k1, Y3 = y + dt/2 k2, Y4 = y + dt k3 and
       // y_{n+1} = y + dt/6 (k1 + 2k2 + 2k3 + k4) = (Y2 + 2 Y3 + Y4 - y)/3 + dt/6 k4,
       // an identity of the tableau (dt k1 = 2(Y2 - y), dt k2 = 2(Y3 - y),
-      // dt k3 = Y4 - y). Stage 3, which holds Y3 and reads y and Y2 at its
-      // points, also forms W = Y2 + 2 Y3 - y over Y2 (no longer needed), so
-      // that stage 4 reads one operand: y_{n+1} = (W + Y4)/3 + dt/6 k4. No
-      // accumulator is kept between stages; the rounding differs from the
-      // reference's running sum by a few ulps of y.
+      // dt k3 = Y4 - y). Stage 2, which holds its input Y2 since the ξ phase
+      // and reads y to form Y3, also writes W = Y2 + 2 Y3 - y (into ACC), so
+      // that stage 3 reads only y and stage 4 one operand besides its input:
+      // y_{n+1} = (W + Y4)/3 + dt/6 k4. Y4 goes over Y2, which no stage reads
+      // after stage 2. No accumulator is kept between stages; the rounding
+      // differs from the reference's running sum by a few ulps of y.
       const double* qs = ownq[OWNQ ? s : 0];  // the stage input here
       // Y2, Y3, Y4 and W point-major, y_{n+1} in planes
       const double* fs = fa[s][i];
-      const double* fw = fb[s][i];
       if (mode == MODE_S0) {
         store_pm_nt(B.buf[BUF_YA], o, qs[0] + snext * k[0], qs[1] + snext * k[1], qs[2] + snext * k[2],
                     qs[3] + snext * k[3]);
       } else if (mode == MODE_S1) {
-        store_pm_nt(B.buf[BUF_YB], o, fs[0] + snext * k[0], fs[1] + snext * k[1], fs[2] + snext * k[2],
-                    fs[3] + snext * k[3]);
+        double y3[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) y3[q] = fs[q] + snext * k[q];
+        store_pm_nt(B.buf[BUF_YB], o, y3[0], y3[1], y3[2], y3[3]);
+        store_pm_nt(B.buf[BUF_ACC], o, fma_t(2.0, y3[0], qs[0]) - fs[0], fma_t(2.0, y3[1], qs[1]) - fs[1],
+                    fma_t(2.0, y3[2], qs[2]) - fs[2], fma_t(2.0, y3[3], qs[3]) - fs[3]);
       } else if (mode == MODE_S2) {
-        store_pm_nt(B.buf[BUF_ACC], o, fs[0] + snext * k[0], fs[1] + snext * k[1], fs[2] + snext * k[2],
+        store_pm_nt(B.buf[BUF_YA], o, fs[0] + snext * k[0], fs[1] + snext * k[1], fs[2] + snext * k[2],
                     fs[3] + snext * k[3]);
-        store_pm_nt(B.buf[BUF_YA], o, fma_t(2.0, qs[0], fw[0]) - fs[0], fma_t(2.0, qs[1], fw[1]) - fs[1],
-                    fma_t(2.0, qs[2], fw[2]) - fs[2], fma_t(2.0, qs[3], fw[3]) - fs[3]);
       } else {
 #pragma unroll
         for (int q = 0; q < 4; ++q) stg_nt(B.buf[BUF_YN] + q * ps, o, (fa[s][i][q] + qs[q]) * (1.0 / 3.0) + sacc * k[q]);

@@ -265,7 +265,7 @@ def time_steps(w, steps, warmup, tile=-1, schedule=-1):
     return {"value": w.points * steps / wall / 1e9, "wall": wall, "ms_per_step": wall * 1e3 / steps,
             "event_ms_per_step": ms_timed / steps, "stage_launch_ms": ms_timed / (4 * steps),
             "tile": list(s.tile_shape()), "tile_index": s.tile_index(),
-            "schedule": ["split", "merged"][s.schedule()]}
+            "schedule": ["split", "merged", "merged after the edge-SAT kernel"][s.schedule()]}
 
 
 def line(name, w, precision, t, steps, warmup, traffic_key=None, metric=METRIC):
@@ -766,8 +766,9 @@ def main():
     ap.add_argument("--no-mixed", action="store_true",
                     help="headline only: skip the mixed-form leg (throughput and 1000-step error vs FP64)")
     ap.add_argument("--tile", type=int, default=-1, help="force a tile shape index (-1: autotune)")
-    ap.add_argument("--schedule", type=int, default=-1, choices=[-1, 0, 1],
-                    help="force the stage schedule (0 split over two streams, 1 merged; -1: autotune)")
+    ap.add_argument("--schedule", type=int, default=-1, choices=[-1, 0, 1, 2],
+                    help="force the stage schedule (0 split over two streams, 1 merged, 2 merged after the "
+                         "edge-SAT kernel; -1: autotune)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain 2 GiB device copy (reference kernel for HBM byte counters)")
     ap.add_argument("--counters", nargs=4, metavar=("FETCH_CSV", "WRITE_CSV", "COPY_FETCH_CSV", "COPY_WRITE_CSV"),

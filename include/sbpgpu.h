@@ -364,11 +364,13 @@ int sbp_system_set_tile_shape(sbp_system sys, int shape_index);
 int sbp_tile_shape_count(void);
 int sbp_tile_shape_get(int index, int* tx, int* ty, int* ry);
 /* Stage schedule (no counterpart in the reference, whose blocks run one per
- * thread, multigrid/src/system.rs:160-366): 0 = split (edge SATs and the
- * tiles on block edges on a second stream, concurrent with the interior
- * tiles), 1 = merged (edge SATs, then one dispatch over all tiles, edge tiles
- * first), -1 = chosen by the autotuner at the tile shape (the default). The
- * results do not depend on it. */
+ * thread, multigrid/src/system.rs:160-366): 0 = split (the tiles on block
+ * edges, which evaluate their edge points' SATs themselves, on a second
+ * stream, concurrent with the interior tiles), 1 = merged (one dispatch over
+ * all tiles, edge tiles first, SATs evaluated in the edge tiles), 2 = merged
+ * with the SATs of every block edge evaluated first by a separate kernel,
+ * -1 = chosen by the autotuner at the tile shape (the default). The results
+ * do not depend on it. */
 int sbp_system_set_schedule(sbp_system sys, int schedule);
 /* Schedule used for the first operator group (after autotuning). */
 int sbp_system_schedule(sbp_system sys, int* schedule);

@@ -273,9 +273,14 @@ struct MixRef64 {
   double aqc[4];        // α_C q_C
 };
 
+// Every thread of a tile evaluates the reference terms of the tile's centre
+// point the same way (one reciprocal of ρ, the refined estimates frcp and
+// fsqrt): any reference state the whole tile shares drops out of the
+// stencils, so only its closeness to the tile's states matters.
 __device__ __forceinline__ MixRef64 mix_ref64(const double q[4], double jx, double jy, double gamma) {
   MixRef64 R;
-  const double u = q[1] / q[0], v = q[2] / q[0];
+  const double ir = frcp(q[0]);
+  const double u = q[1] * ir, v = q[2] * ir;
   const double p = (gamma - 1.0) * (q[3] - 0.5 * (q[1] * u + q[2] * v));
   R.ec[0] = q[1];
   R.ec[1] = q[1] * u + p;
@@ -286,7 +291,7 @@ __device__ __forceinline__ MixRef64 mix_ref64(const double q[4], double jx, doub
   R.fc[2] = q[2] * v + p;
   R.fc[3] = (q[3] + p) * v;
   const double uh = jx * u + jy * v;
-  const double al = fabs(uh) + sqrt(fmax(gamma * p / q[0] * (jx * jx + jy * jy), 0.0));
+  const double al = fabs(uh) + fsqrt(fmax(gamma * p * ir * (jx * jx + jy * jy), 1e-300));
 #pragma unroll
   for (int c = 0; c < 4; ++c) R.aqc[c] = al * q[c];
   return R;

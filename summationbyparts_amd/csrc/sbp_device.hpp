@@ -78,8 +78,8 @@ struct BlockDev {
   const expr::Program* eprog[4];  // EDGE_EVAL program (device copy)
   double* edata[4];               // EDGE_EVAL data at the current stage time (4 x edge length)
   // characteristic SAT of each edge at the current stage (4 x edge length,
-  // component-major), written by the edge-SAT kernel before the edge tiles
-  // add it to k
+  // component-major) when the edge-SAT kernel evaluates them
+  // (StageArgs::sat_buf)
   double* sat[4];
   int edge_iop[4];            // EDGE_INTERP: interpolation operator (InterpId)
   int edge_f2c[4];            // EDGE_INTERP: 1 = fine to coarse
@@ -87,7 +87,8 @@ struct BlockDev {
   int edge_npieces[4];
 };
 
-// One block edge whose SAT the edge-SAT kernel evaluates at every stage.
+// One block edge in a per-stage edge list (the edge-SAT kernel, the Eval
+// edges' data kernel).
 struct SatSeg {
   int32_t block, dir;
 };
@@ -109,6 +110,7 @@ struct StageArgs {
   double time_offset;  // c_s·dt added to *time for boundary data
   int mixed;  // FP32 right-hand side with FP64 state (SBP_MIXED)
   int swap;   // mixed form, odd steps: y_n lives in BUF_ACC and the accumulator in BUF_YN
+  int sat_buf;  // 1: the edge tiles read the edge SATs from BlockDev::sat (edge-SAT kernel)
   int rhs_in_sel, rhs_out_sel;  // MODE_RHS buffers: 0 = rhs_in / rhs_out, s > 0 = rk[s - 1]
 };
 

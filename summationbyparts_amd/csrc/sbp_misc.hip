@@ -124,9 +124,10 @@ hipError_t launch_metric_pairs(const double* metrics, int64_t ps, int64_t ld, in
 // and :869-915), one thread per edge point: B.sat[dir] = hi·τ·T((L + σ|L|)∘
 // (TI(q - g))) with q the stage input at the edge point and g its boundary
 // data (a neighbour's edge, interpolated or pieced together, or far-field
-// data evaluated here at the stage time). The stage kernel's edge tiles add
-// them to k; keeping the 4x4 characteristic algebra in this small kernel
-// keeps it out of the stage kernel's registers. Grid: (segment, points / 256).
+// data evaluated here at the stage time). The stage kernel's edge tiles read
+// them when the group's schedule evaluates the SATs here rather than in the
+// tiles (a long far-field chain of pow and exp then runs on the whole GPU
+// before the tiles, instead of inside them). Grid: (segment, points / 256).
 __global__ __launch_bounds__(256) void edge_sat_kernel(const StageArgs A, const SatSeg* __restrict__ segs, const int mode) {
   const SatSeg sg = segs[blockIdx.x];
   const BlockDev& B = A.blocks[sg.block];
@@ -157,9 +158,9 @@ __global__ __launch_bounds__(256) void edge_sat_kernel(const StageArgs A, const 
 }
 
 // Boundary data of the listed Eval edges at the stage time of `a` into
-// BlockDev::edata, before the edge-SAT kernel reads it: the expression
+// BlockDev::edata, before the stage kernel's edge tiles read it: the expression
 // interpreter's stack lives in this kernel's registers and scratch, not in
-// the SAT kernel's. Grid: (segment, points / 256).
+// the stage kernel's. Grid: (segment, points / 256).
 __global__ __launch_bounds__(256) void eval_edges_kernel(const StageArgs A, const SatSeg* __restrict__ segs) {
   const SatSeg sg = segs[blockIdx.x];
   const BlockDev& B = A.blocks[sg.block];
@@ -219,6 +220,7 @@ hipError_t launch_edge_sat(const StageArgs& a, const SatSeg* segs, int nsegs, in
   edge_sat_kernel<<<grid, 256, 0, stream>>>(a, segs, mode);
   return hipGetLastError();
 }
+
 
 // ---------------------------------------------------------------------------
 // Vortex initial state (Field::vortex, lib.rs:319-328).

@@ -227,7 +227,9 @@ struct Group {
   bool upwind;
   std::vector<int> blocks;
   int shape = -1;
-  int merged = -1;  // schedule: 0 split over two streams, 1 one dispatch (-1 not chosen yet)
+  // schedule: 0 split over two streams, 1 one dispatch, 2 the edge-SAT kernel
+  // and then one dispatch (-1 not chosen yet)
+  int sched = -1;
   // the tiles on block edges (they add the edge SATs), then the interior tiles
   int etile_offset = 0, entiles = 0;
   int tile_offset = 0, ntiles = 0;
@@ -244,10 +246,12 @@ struct sbp_system_s {
   BlockDev* d_blocks = nullptr;
   TileDev* d_tiles = nullptr;
   int* d_pieces = nullptr;  // MultiGrid piece lists of all edges
-  // per-edge SAT buffers (BlockDev::sat) and the edge list of the edge-SAT
-  // kernel, grouped by operator group (Group::sat_off / nsat)
-  double* d_sat = nullptr;
-  SatSeg* d_satsegs = nullptr;
+  // per-edge SAT buffers (BlockDev::sat, schedule 2) and the boundary data of
+  // the Eval edges at the current stage time (BlockDev::edata) in one device
+  // array; the edge lists of the edge-SAT and Eval-data kernels, grouped by
+  // operator group (Group::sat_off / nsat, eval_off / neval)
+  double* d_edata = nullptr;
+  SatSeg* d_evsegs = nullptr;
   // device copies of the Eval edges' expression programs (BlockDev::eprog)
   unsigned char* d_progs = nullptr;
   double* d_time = nullptr;      // system time
@@ -414,27 +418,32 @@ int normalize(sbp_system_s* s) {
   return SBP_OK;
 }
 
-// One stage of one group. Split schedule: the SATs of its block edges and
-// then its edge tiles on the context's second stream, its interior tiles on
-// `st`, concurrently; `st` continues once both are done (also under graph
-// capture: a fork and a join). Merged schedule: the edge SATs, then one
-// dispatch of the edge instantiation over all tiles, the edge tiles first
-// (it takes the interior path on interior tiles), all on `st`.
-hipError_t launch_group(const sbp_system_s* s, const Group& g, int shape, bool merged, int mode, const StageArgs& a,
+// One stage of one group (Eval edges get their boundary data at the stage
+// time from a small kernel first). Schedule 0, split: the edge tiles, which
+// evaluate the SATs of their block-edge points themselves, on the context's
+// second stream, the interior tiles on `st`, concurrently; `st` continues
+// once both are done (also under graph capture: a fork and a join).
+// Schedule 1, merged: one dispatch of the edge instantiation over all tiles,
+// the edge tiles first (it takes the interior path on interior tiles), on
+// `st`. Schedule 2: the edge-SAT kernel evaluates the SATs of every block
+// edge of the group, then the merged dispatch reads them.
+hipError_t launch_group(const sbp_system_s* s, const Group& g, int shape, int sched, int mode, const StageArgs& a,
                         hipStream_t st) {
   const sbp_context_s* c = s->ctx;
   hipError_t e;
-  if (merged) {
-    if ((e = launch_eval_edges(a, s->d_satsegs + g.eval_off, g.neval, g.eval_maxlen, st)) != hipSuccess) return e;
-    if ((e = launch_edge_sat(a, s->d_satsegs + g.sat_off, g.nsat, g.sat_maxlen, mode, st)) != hipSuccess) return e;
+  if (sched != 0) {
+    if ((e = launch_eval_edges(a, s->d_evsegs + g.eval_off, g.neval, g.eval_maxlen, st)) != hipSuccess) return e;
     StageArgs am = a;
+    if (sched == 2) {
+      if ((e = launch_edge_sat(a, s->d_evsegs + g.sat_off, g.nsat, g.sat_maxlen, mode, st)) != hipSuccess) return e;
+      am.sat_buf = 1;
+    }
     am.tiles = s->d_tiles + g.etile_offset;
     return launch_stage(g.op_eta, g.op_xi, g.upwind, true, shape, mode, am, g.entiles + g.ntiles, st);
   }
   if ((e = hipEventRecord(c->ev_fork, st)) != hipSuccess) return e;
   if ((e = hipStreamWaitEvent(c->aux, c->ev_fork, 0)) != hipSuccess) return e;
-  hipError_t ee = launch_eval_edges(a, s->d_satsegs + g.eval_off, g.neval, g.eval_maxlen, c->aux);
-  if (ee == hipSuccess) ee = launch_edge_sat(a, s->d_satsegs + g.sat_off, g.nsat, g.sat_maxlen, mode, c->aux);
+  hipError_t ee = launch_eval_edges(a, s->d_evsegs + g.eval_off, g.neval, g.eval_maxlen, c->aux);
   StageArgs ae = a;
   ae.tiles = s->d_tiles + g.etile_offset;
   if (ee == hipSuccess) ee = launch_stage(g.op_eta, g.op_xi, g.upwind, true, shape, mode, ae, g.entiles, c->aux);
@@ -451,16 +460,16 @@ hipError_t launch_group(const sbp_system_s* s, const Group& g, int shape, bool m
 // stage, then three rounds of the stage-1..3 kernels (modes S0, S1, S2 read
 // y_n and write only the accumulator and the stage inputs, so the state is
 // untouched), between two events.
-hipError_t time_group(const sbp_system_s* s, const Group& g, int shape, bool merged, hipEvent_t e0, hipEvent_t e1,
+hipError_t time_group(const sbp_system_s* s, const Group& g, int shape, int sched, hipEvent_t e0, hipEvent_t e1,
                       float& ms) {
   hipStream_t st = s->ctx->stream;
   const StageArgs a = stage_args(s, 0.0, 0, 0);
-  hipError_t e = launch_group(s, g, shape, merged, MODE_S0, a, st);
+  hipError_t e = launch_group(s, g, shape, sched, MODE_S0, a, st);
   if (e != hipSuccess) return e;
   if ((e = hipEventRecord(e0, st)) != hipSuccess) return e;
   for (int r = 0; r < 3; ++r)
     for (int m = MODE_S0; m <= MODE_S2; ++m)
-      if ((e = launch_group(s, g, shape, merged, m, a, st)) != hipSuccess) return e;
+      if ((e = launch_group(s, g, shape, sched, m, a, st)) != hipSuccess) return e;
   if ((e = hipEventRecord(e1, st)) != hipSuccess) return e;
   if ((e = hipEventSynchronize(e1)) != hipSuccess) return e;
   return hipEventElapsedTime(&ms, e0, e1);
@@ -495,15 +504,15 @@ int rebuild(sbp_system_s* s) {
     auto it = s->ctx->tuned.find(key);
     const bool cached = it != s->ctx->tuned.end() && s->forced_shape < 0 && s->forced_schedule < 0;
     g.shape = s->forced_shape >= 0 ? s->forced_shape : cached ? (it->second & 0xff) : -1;
-    g.merged = s->forced_schedule >= 0 ? s->forced_schedule : cached ? (it->second >> 8) : -1;
+    g.sched = s->forced_schedule >= 0 ? s->forced_schedule : cached ? (it->second >> 8) : -1;
   }
   bool need_tune = false, need_sched = false;
   std::vector<bool> tune_cache(s->groups.size());
   for (size_t gi = 0; gi < s->groups.size(); ++gi) {
     const Group& g = s->groups[gi];
     need_tune = need_tune || g.shape < 0;
-    need_sched = need_sched || g.merged < 0;
-    tune_cache[gi] = g.shape < 0 && g.merged < 0;
+    need_sched = need_sched || g.sched < 0;
+    tune_cache[gi] = g.shape < 0 && g.sched < 0;
   }
   auto upload = [&](void) -> int {
     tiles.clear();
@@ -522,17 +531,18 @@ int rebuild(sbp_system_s* s) {
     if (s->d_blocks) hipFree(s->d_blocks);
     if (s->d_tiles) hipFree(s->d_tiles);
     if (s->d_pieces) hipFree(s->d_pieces);
-    if (s->d_sat) hipFree(s->d_sat);
-    if (s->d_satsegs) hipFree(s->d_satsegs);
+    if (s->d_edata) hipFree(s->d_edata);
+    if (s->d_evsegs) hipFree(s->d_evsegs);
     if (s->d_progs) hipFree(s->d_progs);
     s->d_blocks = nullptr;
     s->d_tiles = nullptr;
     s->d_pieces = nullptr;
-    s->d_sat = nullptr;
-    s->d_satsegs = nullptr;
+    s->d_edata = nullptr;
+    s->d_evsegs = nullptr;
     s->d_progs = nullptr;
-    // SAT buffers: 4 x edge length doubles per block edge in one device
-    // array; the edge list in group order
+    // every block edge of each group in group order (SAT buffers of 4 x edge
+    // length doubles), then the Eval edges of each group with their data
+    // buffers
     std::vector<SatSeg> segs;
     std::vector<size_t> soff;
     size_t stotal = 0;
@@ -549,8 +559,6 @@ int rebuild(sbp_system_s* s) {
         }
       g.nsat = (int)segs.size() - g.sat_off;
     }
-    // Eval edges of each group, after all SAT edges, with their data buffers
-    // after the SAT buffers
     for (auto& g : s->groups) {
       g.eval_off = (int)segs.size();
       g.eval_maxlen = 0;
@@ -565,14 +573,15 @@ int rebuild(sbp_system_s* s) {
           }
       g.neval = (int)segs.size() - g.eval_off;
     }
-    SBP_TRY_HIP(hipMalloc(&s->d_sat, sizeof(double) * std::max<size_t>(stotal, 1)));
-    SBP_TRY_HIP(hipMalloc(&s->d_satsegs, sizeof(SatSeg) * std::max<size_t>(segs.size(), 1)));
-    SBP_TRY_HIP(hipMemsetAsync(s->d_sat, 0, sizeof(double) * std::max<size_t>(stotal, 1), st));
-    SBP_TRY_HIP(hipMemcpyAsync(s->d_satsegs, segs.data(), sizeof(SatSeg) * segs.size(), hipMemcpyHostToDevice, st));
+    SBP_TRY_HIP(hipMalloc(&s->d_edata, sizeof(double) * std::max<size_t>(stotal, 1)));
+    SBP_TRY_HIP(hipMalloc(&s->d_evsegs, sizeof(SatSeg) * std::max<size_t>(segs.size(), 1)));
+    SBP_TRY_HIP(hipMemsetAsync(s->d_edata, 0, sizeof(double) * std::max<size_t>(stotal, 1), st));
+    if (!segs.empty())
+      SBP_TRY_HIP(hipMemcpyAsync(s->d_evsegs, segs.data(), sizeof(SatSeg) * segs.size(), hipMemcpyHostToDevice, st));
     for (auto& g : s->groups) {
-      for (int k = g.sat_off; k < g.sat_off + g.nsat; ++k) devs[segs[k].block].sat[segs[k].dir] = s->d_sat + soff[k];
+      for (int k = g.sat_off; k < g.sat_off + g.nsat; ++k) devs[segs[k].block].sat[segs[k].dir] = s->d_edata + soff[k];
       for (int k = g.eval_off; k < g.eval_off + g.neval; ++k)
-        devs[segs[k].block].edata[segs[k].dir] = s->d_sat + soff[k];
+        devs[segs[k].block].edata[segs[k].dir] = s->d_edata + soff[k];
     }
     // Eval edges: their programs, 16-byte aligned, in one device array
     std::vector<unsigned char> progs;
@@ -638,7 +647,7 @@ int rebuild(sbp_system_s* s) {
           Group& g = s->groups[gi];
           if (saved[gi] >= 0) continue;
           float ms = 0.f;
-          const hipError_t w = time_group(s, g, sh, g.merged == 1, e0, e1, ms);
+          const hipError_t w = time_group(s, g, sh, g.sched < 0 ? 0 : g.sched, e0, e1, ms);
           if (w == hipErrorNotSupported) continue;  // configuration not built for this operator pair
           SBP_TRY_HIP(w);
           cfg_ms[gi][sh] = std::min(cfg_ms[gi][sh], ms);
@@ -659,22 +668,24 @@ int rebuild(sbp_system_s* s) {
       if (s->groups[gi].shape < 0) s->groups[gi].shape = best[gi];
   }
   if (need_sched) {
-    // schedule at the chosen shape: split and merged, two rounds each, the
+    // schedule at the chosen shape: each of the three, two rounds each, the
     // smaller time of each
     SBP_TRY(upload());
     hipEvent_t e0, e1;
     SBP_TRY_HIP(hipEventCreate(&e0));
     SBP_TRY_HIP(hipEventCreate(&e1));
     for (auto& g : s->groups) {
-      if (g.merged >= 0) continue;
-      float t[2] = {1e30f, 1e30f};
+      if (g.sched >= 0) continue;
+      float t[3] = {1e30f, 1e30f, 1e30f};
       for (int round = 0; round < 2; ++round)
-        for (int m = 0; m < 2; ++m) {
+        for (int m = 0; m < 3; ++m) {
           float ms = 0.f;
-          SBP_TRY_HIP(time_group(s, g, g.shape, m == 1, e0, e1, ms));
+          SBP_TRY_HIP(time_group(s, g, g.shape, m, e0, e1, ms));
           t[m] = std::min(t[m], ms);
         }
-      g.merged = t[1] < t[0] ? 1 : 0;
+      g.sched = 0;
+      for (int m = 1; m < 3; ++m)
+        if (t[m] < t[g.sched]) g.sched = m;
     }
     hipEventDestroy(e0);
     hipEventDestroy(e1);
@@ -686,7 +697,7 @@ int rebuild(sbp_system_s* s) {
     int64_t pts = 0;
     for (int bi : g.blocks) pts += (int64_t)s->blocks[bi].nx * s->blocks[bi].ny;
     s->ctx->tuned[std::make_tuple(g.op_eta, g.op_xi, tune_kind(s, g), pts, (int)g.blocks.size())] =
-        g.shape | (g.merged << 8);
+        g.shape | (g.sched << 8);
   }
   SBP_TRY(upload());
   destroy_graph(s);
@@ -702,12 +713,11 @@ int check_err(sbp_system_s* s) {
   return status_from_errword(w);
 }
 
-// One fused RK4 step: four stage launches per group, the edge SATs of each
-// stage and the time update (integrate::<Rk4>, utils/integrate/src/lib.rs:176-222).
+// One fused RK4 step: four stage launches per group and the time update (integrate::<Rk4>, utils/integrate/src/lib.rs:176-222).
 int enqueue_step(sbp_system_s* s, double dt, int swap, hipStream_t st) {
   for (int stage = 0; stage < 4; ++stage) {
     const StageArgs a = stage_args(s, dt, stage, swap);
-    for (const Group& g : s->groups) SBP_TRY_HIP(launch_group(s, g, g.shape, g.merged == 1, stage, a, st));
+    for (const Group& g : s->groups) SBP_TRY_HIP(launch_group(s, g, g.shape, g.sched, stage, a, st));
   }
   SBP_TRY_HIP(launch_advance_time(s->d_time, dt, st));
   return SBP_OK;
@@ -1110,9 +1120,6 @@ static int rhs_block(sbp_context ctx, bool up, int op_eta, int op_xi, int ny, in
     b.edge_ptr[d] = boundaries ? boundaries[d] : nullptr;
     b.edge_cs[d] = (d == SBP_NORTH || d == SBP_SOUTH) ? nx : ny;
   }
-  // the SATs of the four edges land in a per-call buffer after the metric
-  // pairs (4 x (2 nx + 2 ny) doubles)
-  const int64_t nsat = boundaries ? 4 * (2 * (int64_t)nx + 2 * (int64_t)ny) : 0;
   std::vector<TileDev> tiles;
   for (int ty = 0; ty < b.nty; ++ty)
     for (int tx = 0; tx < b.ntx; ++tx) tiles.push_back({0, (int16_t)tx, (int16_t)ty});
@@ -1120,23 +1127,13 @@ static int rhs_block(sbp_context ctx, bool up, int op_eta, int op_xi, int ny, in
   SBP_TRY_HIP(hipStreamSynchronize(st));
   // the stage kernel reads the metric pairs; build them from the caller's planes
   double* pairs = nullptr;
-  SBP_TRY_HIP(hipMalloc((void**)&pairs, sizeof(double) * (4 * b.ps + nsat) + sizeof(SatSeg) * 4));
+  SBP_TRY_HIP(hipMalloc((void**)&pairs, sizeof(double) * 4 * b.ps));
   struct FreeOnExit {
     double* p;
     ~FreeOnExit() { hipFree(p); }
   } free_pairs{pairs};
   SBP_TRY_HIP(launch_metric_pairs(metrics, b.ps, b.ld, ny, nx, pairs, st));
   b.mpair = pairs;
-  SatSeg* d_segs = reinterpret_cast<SatSeg*>(pairs + 4 * b.ps + nsat);
-  if (boundaries) {
-    double* sat = pairs + 4 * b.ps;
-    for (int d = 0; d < 4; ++d) {
-      b.sat[d] = sat;
-      sat += 4 * (int64_t)((d == SBP_NORTH || d == SBP_SOUTH) ? nx : ny);
-    }
-    const SatSeg segs[4] = {{0, SBP_NORTH}, {0, SBP_SOUTH}, {0, SBP_EAST}, {0, SBP_WEST}};
-    SBP_TRY_HIP(hipMemcpy(d_segs, segs, sizeof(segs), hipMemcpyHostToDevice));
-  }
   if (ctx->tmptiles_cap < (int)tiles.size()) {
     if (ctx->d_tmptiles) hipFree(ctx->d_tmptiles);
     ctx->d_tmptiles = nullptr;
@@ -1151,7 +1148,6 @@ static int rhs_block(sbp_context ctx, bool up, int op_eta, int op_xi, int ny, in
   a.time = nullptr;
   a.error = ctx->d_err;
   a.gamma = gamma;
-  if (boundaries) SBP_TRY_HIP(launch_edge_sat(a, d_segs, 4, std::max(nx, ny), MODE_RHS, st));
   SBP_TRY_HIP(launch_stage(op_eta, op_xi, up, true, shape, MODE_RHS, a, (int)tiles.size(), st));
   return ctx_check_err(ctx);
 }
@@ -1373,8 +1369,8 @@ int sbp_system_destroy(sbp_system s) {
   if (s->d_blocks) hipFree(s->d_blocks);
   if (s->d_tiles) hipFree(s->d_tiles);
   if (s->d_pieces) hipFree(s->d_pieces);
-  if (s->d_sat) hipFree(s->d_sat);
-  if (s->d_satsegs) hipFree(s->d_satsegs);
+  if (s->d_edata) hipFree(s->d_edata);
+  if (s->d_evsegs) hipFree(s->d_evsegs);
   if (s->d_progs) hipFree(s->d_progs);
   if (s->d_time) hipFree(s->d_time);
   if (s->d_err) hipFree(s->d_err);
@@ -1628,7 +1624,7 @@ int sbp_system_rhs(sbp_system s, double t, double* k_host) {
   StageArgs a = stage_args(s, 0.0, 0, 0);
   a.time = s->d_time_rhs;
   a.rk_b = a.rk_a = 0.0;
-  for (const Group& g : s->groups) SBP_TRY_HIP(launch_group(s, g, g.shape, g.merged == 1, MODE_RHS, a, st));
+  for (const Group& g : s->groups) SBP_TRY_HIP(launch_group(s, g, g.shape, g.sched, MODE_RHS, a, st));
   size_t off = 0;
   for (auto& h : s->blocks) {
     for (int c = 0; c < 4; ++c)
@@ -1782,7 +1778,7 @@ int sbp_system_profile(sbp_system s, double dt, int64_t nsteps, double* ms_total
         SBP_TRY_HIP(hipEventCreate(&a0));
         SBP_TRY_HIP(hipEventCreate(&a1));
         SBP_TRY_HIP(hipEventRecord(a0, st));
-        SBP_TRY_HIP(launch_group(s, g, g.shape, g.merged == 1, stage, a, st));
+        SBP_TRY_HIP(launch_group(s, g, g.shape, g.sched, stage, a, st));
         SBP_TRY_HIP(hipEventRecord(a1, st));
         ev.push_back(a0);
         ev.push_back(a1);
@@ -1864,7 +1860,7 @@ int sbp_system_advance_adaptive(sbp_system s, double dt, double* guess_dt, doubl
       a.rk_a = a.rk_b = 0.0;
       a.rhs_in_sel = i == 0 ? 0 : SET_STAGE + 1;
       a.rhs_out_sel = i + 1;
-      for (const Group& grp : s->groups) SBP_TRY_HIP(launch_group(s, grp, grp.shape, grp.merged == 1, MODE_RHS, a, st));
+      for (const Group& grp : s->groups) SBP_TRY_HIP(launch_group(s, grp, grp.shape, grp.sched, MODE_RHS, a, st));
     }
     SBP_TRY(combine(Bw, 4, g, -1));
     SBP_TRY(combine(Bs, 4, g, SET_SECOND));
@@ -1938,7 +1934,7 @@ int sbp_system_integrate(sbp_system s, int nstages, const double* A, const doubl
       a.rk_a = a.rk_b = 0.0;
       a.rhs_in_sel = i == 0 ? 0 : SET_STAGE + 1;
       a.rhs_out_sel = i + 1;
-      for (const Group& grp : s->groups) SBP_TRY_HIP(launch_group(s, grp, grp.shape, grp.merged == 1, MODE_RHS, a, st));
+      for (const Group& grp : s->groups) SBP_TRY_HIP(launch_group(s, grp, grp.shape, grp.sched, MODE_RHS, a, st));
     }
     SBP_TRY(combine(B, nstages, final_out.data()));
     SBP_TRY_HIP(launch_advance_time(s->d_time, dt, st));
@@ -1997,7 +1993,7 @@ int sbp_system_set_tile_shape(sbp_system s, int shape_index) {
 }
 
 int sbp_system_set_schedule(sbp_system s, int schedule) {
-  if (!s || schedule < -1 || schedule > 1) return SBP_ERR_INVALID;
+  if (!s || schedule < -1 || schedule > 2) return SBP_ERR_INVALID;
   s->forced_schedule = schedule;
   s->dirty = true;
   return SBP_OK;
@@ -2006,7 +2002,7 @@ int sbp_system_set_schedule(sbp_system s, int schedule) {
 int sbp_system_schedule(sbp_system s, int* schedule) {
   if (!s || !schedule) return SBP_ERR_INVALID;
   SBP_TRY(rebuild(s));
-  *schedule = s->groups.empty() ? 0 : s->groups[0].merged;
+  *schedule = s->groups.empty() ? 0 : s->groups[0].sched;
   return SBP_OK;
 }
 

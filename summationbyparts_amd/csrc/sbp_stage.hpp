@@ -23,11 +23,11 @@
 // the RK4 sum stay FP64; each stage input is y_n + Δ with the FP32 increment
 // Δ = a·dt·k stored by the previous stage, and the last stage sums y_n and
 // the increments in FP64.
-// The characteristic SATs of the block-edge points, evaluated for the stage by
-// the edge-SAT kernel, are added to k by the EDGE instantiation, which the
-// host launches for the tiles on block edges only (it also carries the
-// closure rows); neither instantiation carries the 4x4 characteristic
-// algebra.
+// The EDGE instantiation, which the host launches for the tiles on block
+// edges (alone, or over all tiles of a group with the edge tiles first), also
+// carries the closure rows and the characteristic SATs of the block-edge
+// points: a prologue evaluates them into LDS before the phases begin (when
+// few registers are live), the finish adds them to k.
 #pragma once
 #include <type_traits>
 
@@ -184,6 +184,63 @@ __global__ __launch_bounds__((TX / RX) * (TY / RY)) __attribute__((amdgpu_waves_
   for (int i = 0; i < RX; ++i)
 #pragma unroll
     for (int s = 0; s < RY; ++s) any = any || (xin[i] && yin[s]);
+
+  // Characteristic SATs of the block-edge points this tile writes
+  // (SAT_characteristic, lib.rs:1150-1252, boundary data as boundary_extracts
+  // gives it, lib.rs:804-915), evaluated before anything else is live (or,
+  // with A.sat_buf, read from the edge-SAT kernel's buffers) and kept in LDS
+  // for the finish: slots [0, TX) north, [TX, 2TX) south, [2TX, 2TX+TY) east,
+  // [2TX+TY, 2TX+2TY) west, by the point's offset from the tile origin along
+  // the edge; zero where an edge has no SAT.
+  constexpr int NSAT = EDGE ? 2 * TX + 2 * TY : 1;
+  __shared__ __align__(16) double satl[4 * NSAT];
+  if constexpr (EDGE) {
+    // the block's descriptor read through a lane broadcast (taken while every
+    // lane is active): the values the boundary data and the SAT use then live
+    // in vector registers, which are free here, and not in the scalar
+    // registers the stencils use
+    const BlockDev& BV = A.blocks[__shfl(T.block, 0)];
+    if (wy1 == ny || wy0 == 0 || wx1 == nx || wx0 == 0) {
+      for (int it = threadIdx.x; it < NSAT; it += NT) {
+        int dir, t;
+        bool ok;
+        if (it < 2 * TX) {
+          dir = it < TX ? SBP_NORTH : SBP_SOUTH;
+          t = x0 + (it < TX ? it : it - TX);
+          ok = (it < TX ? wy1 == ny : wy0 == 0) && t >= wx0 && t < wx1;
+        } else {
+          const int e = it - 2 * TX;
+          dir = e < TY ? SBP_EAST : SBP_WEST;
+          t = y0 + (e < TY ? e : e - TY);
+          ok = (e < TY ? wx1 == nx : wx0 == 0) && t >= wy0 && t < wy1;
+        }
+        double sat[4] = {0.0, 0.0, 0.0, 0.0};
+        if (ok && BV.edge_src[dir] != EDGE_NONE && A.sat_buf) {
+          // evaluated for this stage by the edge-SAT kernel
+          const int len = (dir == SBP_NORTH || dir == SBP_SOUTH) ? nx : ny;
+#pragma unroll
+          for (int c = 0; c < 4; ++c) sat[c] = ldg(BV.sat[dir], (uint32_t)(c * len + t));
+        } else if (ok && BV.edge_src[dir] != EDGE_NONE) {
+          const bool ns = dir == SBP_NORTH || dir == SBP_SOUTH;
+          const int gx = ns ? t : (dir == SBP_EAST ? nx - 1 : 0);
+          const int gy = ns ? (dir == SBP_NORTH ? ny - 1 : 0) : t;
+          const uint32_t o = (uint32_t)(gy * ld + gx);
+          double q[4], g[4];
+          load_stage_state(A, BV, mode, o, q);
+          edge_data(A, BV, dir, t, o, mode, g, err);
+          // (σ, τ): north (-1, +1), south (+1, -1), east (-1, +1), west (+1, -1);
+          // north/south use J·∇η, east/west J·∇ξ
+          const double sign = (dir == SBP_NORTH || dir == SBP_EAST) ? -1.0 : 1.0;
+          const int m0 = ns ? 3 : 1;
+          const double* mv = BV.metrics;
+          sat_point(sat, q, g, ns ? BV.hi_eta : BV.hi_xi, sign, -sign, ldg(mv, o), ldg(mv + m0 * ps, o),
+                    ldg(mv + (m0 + 1) * ps, o), gamma);
+        }
+#pragma unroll
+        for (int c = 0; c < 4; ++c) satl[4 * it + c] = sat[c];
+      }
+    }
+  }
 
   // state of the stage input at offset o (FP64; the mixed form adds Δ)
   const bool pm = !MIX && stage_input_pm(A, mode);  // point-major FP64 stage input
@@ -601,21 +658,20 @@ __global__ __launch_bounds__((TX / RX) * (TY / RY)) __attribute__((amdgpu_waves_
 #pragma unroll
     for (int q = 0; q < 4; ++q) k[q] = (double)acc[s][i][q] * ij;
     if constexpr (EDGE) {
-      // the characteristic SATs of the block-edge points (the edge-SAT kernel
-      // evaluated them for this stage), summed in the order of
-      // SAT_characteristics (north, south, east, west; corners get two)
+      // the characteristic SATs of the block-edge points (from the prologue),
+      // summed in the order of SAT_characteristics (north, south, east, west;
+      // corners get two)
       const int gx = xb + i, gy = yb + s;
       if (gx == 0 || gx == nx - 1 || gy == 0 || gy == ny - 1) {
         double sat[4] = {0.0, 0.0, 0.0, 0.0};
-        auto add = [&](int d, int t, int len) {
-          if (B.edge_src[d] == EDGE_NONE) return;
+        auto add = [&](int slot) {
 #pragma unroll
-          for (int q = 0; q < 4; ++q) sat[q] += ldg(B.sat[d], (uint32_t)(q * len + t));
+          for (int q = 0; q < 4; ++q) sat[q] += satl[4 * slot + q];
         };
-        if (gy == ny - 1) add(SBP_NORTH, gx, nx);
-        if (gy == 0) add(SBP_SOUTH, gx, nx);
-        if (gx == nx - 1) add(SBP_EAST, gy, ny);
-        if (gx == 0) add(SBP_WEST, gy, ny);
+        if (gy == ny - 1) add(gx - x0);
+        if (gy == 0) add(TX + gx - x0);
+        if (gx == nx - 1) add(2 * TX + gy - y0);
+        if (gx == 0) add(2 * TX + TY + gy - y0);
 #pragma unroll
         for (int q = 0; q < 4; ++q) k[q] += sat[q];
       }

@@ -132,7 +132,8 @@ class BlockSystem:
 
     def set_schedule(self, schedule):
         """Stage schedule: 0 split over two streams, 1 one dispatch over all
-        tiles, -1 autotuned (results do not depend on it)."""
+        tiles, 2 the edge SATs by a separate kernel and then one dispatch,
+        -1 autotuned (results do not depend on it)."""
         _lib.check(_lib.lib().sbp_system_set_schedule(self._h, schedule), "sbp_system_set_schedule")
 
     def schedule(self):

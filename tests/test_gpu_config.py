@@ -170,11 +170,12 @@ def test_full_run_and_error(gpu, name):
 
 
 @pytest.mark.parametrize("name", ["interpolation.json", "expressions_vortex.json", "tsection.json"])
-@pytest.mark.parametrize("schedule", [0, 1])
-def test_example_configuration_both_schedules(gpu, name, schedule):
+@pytest.mark.parametrize("schedule", [0, 1, 2])
+def test_example_configuration_each_schedule(gpu, name, schedule):
     """Interpolate, MultiGrid and Eval edges in each stage schedule (split:
-    edge SATs beside the interior tiles; merged: one dispatch over all
-    tiles), three steps per step against the oracle."""
+    edge tiles beside the interior tiles; merged: one dispatch over all
+    tiles; merged after the edge-SAT kernel), three steps per step against
+    the oracle."""
     from summationbyparts_amd import config
     rt = config.load(_text(name))
     sys_ = rt.system()

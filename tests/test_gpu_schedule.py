@@ -1,10 +1,12 @@
-"""Both stage schedules against the oracle: split (edge SATs and edge tiles on
-a second stream beside the interior tiles) and merged (edge SATs, then one
-dispatch of the edge instantiation over all tiles, edge tiles first). Each
-step at the drop-in bar (1e-12 relative max-norm per component) on layouts
-with far-field, Grid and periodic edges and unequal operator pairs, and the
-two schedules against each other: FP64 and mixed results are the same
-arithmetic at every point, so they agree to the last bits."""
+"""The three stage schedules against the oracle: split (edge tiles, which
+evaluate their edge points' SATs, on a second stream beside the interior
+tiles), merged (one dispatch of the edge instantiation over all tiles, edge
+tiles first) and merged after the edge-SAT kernel (the SATs of every block
+edge evaluated by a separate kernel, read by the edge tiles). Each step at
+the drop-in bar (1e-12 relative max-norm per component) on layouts with
+far-field, Grid and periodic edges and unequal operator pairs, and the
+schedules against each other: FP64 and mixed results are the same arithmetic
+at every point, so they agree to the last bits."""
 import numpy as np
 import pytest
 
@@ -61,7 +63,7 @@ def _systems(schedule, precision="fp64"):
     return sys_, ref
 
 
-@pytest.mark.parametrize("schedule", [0, 1])
+@pytest.mark.parametrize("schedule", [0, 1, 2])
 def test_schedule_against_oracle(gpu, schedule):
     sys_, ref = _systems(schedule)
     assert sys_.schedule() == schedule
@@ -83,26 +85,27 @@ def test_schedules_agree(gpu, precision):
     rounding follows the tiles' reference states, so the autotuner could
     otherwise pick shapes that round differently)."""
     out = []
-    for schedule in (0, 1):
+    for schedule in (0, 1, 2):
         s, _ = _systems(schedule, precision)
         s.set_tile_shape(0)
         s.set_dt(0.2 * 10.0 / 60.0 / 4.0)
         s.advance(10)
         out.append([s.get_state(i) for i in range(2)])
         s.close()
-    for a, b in zip(*out):
-        assert rel_maxnorm(a, b) < 1e-14
+    for other in out[1:]:
+        for a, b in zip(out[0], other):
+            assert rel_maxnorm(a, b) < 1e-14
 
 
 def test_schedule_c2_block_and_autotune(gpu):
-    """A periodic 300x300 curvilinear upwind9 block in both schedules and
+    """A periodic 300x300 curvilinear upwind9 block in each schedule and
     autotuned (one RHS each against the oracle); the autotuned system
     reports the schedule it picked."""
     from summationbyparts_amd import _lib, euler, operators
     n = 300
     X, Y = curvilinear(n, n)
     ks = []
-    for schedule in (0, 1, -1):
+    for schedule in (0, 1, 2, -1):
         s = euler.System(X, Y, operators.Upwind9())
         s.set_schedule(schedule)
         s.vortex(0.0, _vp())
@@ -111,7 +114,7 @@ def test_schedule_c2_block_and_autotune(gpu):
         s.set_state(0, q)
         s.set_rhs_kind(_lib.RHS_UPWIND)
         ks.append(s.rhs(0.0)[0])
-        assert s.schedule() in (0, 1) and (schedule < 0 or s.schedule() == schedule)
+        assert s.schedule() in (0, 1, 2) and (schedule < 0 or s.schedule() == schedule)
         s.close()
     ref = O.System([(X, Y)], [(O.UPWIND9, O.UPWIND9)])
     ref.set_rhs_kind(O.RHS_UPWIND)

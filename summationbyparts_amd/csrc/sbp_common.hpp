@@ -99,6 +99,8 @@ struct Blk;
 template <class OP>
 struct Blk<OP, SBP_DIFF> {
   static constexpr int M = OP::M, N = OP::N, D = OP::D, H = OP::H;
+  static constexpr double end_sign = -1.0;
+  static constexpr int table = 0;  // position of its start block in a DIFF, DISS pair of tables
   __device__ __forceinline__ static constexpr double diag(int d) { return OP::diff_diag[d]; }
   __device__ __forceinline__ static double start(int r, int k) { return OP::diff_start[r][k]; }
   __device__ __forceinline__ static double end(int r, int k) { return -OP::diff_start[M - 1 - r][N - 1 - k]; }
@@ -107,6 +109,8 @@ struct Blk<OP, SBP_DIFF> {
 template <class OP>
 struct Blk<OP, SBP_DISS> {
   static constexpr int M = OP::M, N = OP::N, D = OP::D, H = OP::H;
+  static constexpr double end_sign = 1.0;
+  static constexpr int table = 1;
   __device__ __forceinline__ static constexpr double diag(int d) { return OP::diss_diag[d]; }
   __device__ __forceinline__ static double start(int r, int k) { return OP::diss_start[r][k]; }
   __device__ __forceinline__ static double end(int r, int k) { return OP::diss_start[M - 1 - r][N - 1 - k]; }
@@ -183,6 +187,36 @@ __device__ __forceinline__ void row_sum4(const T* line, int qs, int stride, int 
       for (int q = 0; q < 4; ++q) out[q] = s[q];
       return;
     }
+  }
+#pragma unroll
+  for (int q = 0; q < 4; ++q) out[q] = interior_sum<B>(line + q * qs + (i - r0) * stride, stride);
+}
+
+// row_sum4 of a closure row (start or end block) with the start block's
+// coefficients read from a copy in LDS (`tab`, M x N row-major): a closure
+// row's coefficient row differs from lane to lane, and LDS serves such reads
+// at a fraction of the constant tables' latency. Interior rows as row_sum4.
+template <class B, typename T>
+__device__ __forceinline__ void row_sum4_tab(const T* line, int qs, int stride, int i, int n, int r0, const double* tab,
+                                             T out[4]) {
+  if (i < B::M || i >= n - B::M) {
+    const bool st = i < B::M;
+    // end rows: start row M-1-r read backwards, with the kind's sign
+    const int r = st ? i : i - (n - B::M);
+    const double* row = st ? tab + i * B::N : tab + (B::M - 1 - r) * B::N + (B::N - 1);
+    const int dk = st ? 1 : -1;
+    const T sg = st ? T(1) : T(B::end_sign);
+    const int b0 = st ? -r0 : n - B::N - r0;
+    T s[4] = {T(0), T(0), T(0), T(0)};
+#pragma unroll
+    for (int k = 0; k < B::N; ++k) {
+      const T c = sg * T(row[dk * k]);
+#pragma unroll
+      for (int q = 0; q < 4; ++q) s[q] += c * line[q * qs + (b0 + k) * stride];
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) out[q] = s[q];
+    return;
   }
 #pragma unroll
   for (int q = 0; q < 4; ++q) out[q] = interior_sum<B>(line + q * qs + (i - r0) * stride, stride);

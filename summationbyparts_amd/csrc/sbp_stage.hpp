@@ -242,6 +242,37 @@ __global__ __launch_bounds__((TX / RX) * (TY / RY)) __attribute__((amdgpu_waves_
     }
   }
 
+  // Closure coefficients (start blocks of DIFF and, for upwind, DISS; the ξ
+  // operator's, then the η operator's unless it is the same) copied into LDS
+  // by the tiles that write closure rows: their stencils read a coefficient
+  // row per lane (row_sum4_tab).
+  constexpr bool SAMEOP = std::is_same<OX, OY>::value;
+  constexpr int TABX = (UP ? 2 : 1) * OX::M * OX::N, TABY = (UP ? 2 : 1) * OY::M * OY::N;
+  constexpr int NTAB = EDGE ? (SAMEOP ? TABX : TABX + TABY) : 1;
+  __shared__ double ctab[NTAB];
+  const bool xclos = !(x0 >= OX::M && x0 + TX <= nx - OX::M);
+  const bool yclos = !(y0 >= OY::M && y0 + TY <= ny - OY::M);
+  if constexpr (EDGE) {
+    if (xclos || yclos) {
+      auto fill = [&](auto op, int base) {
+        using OP = decltype(op);
+        constexpr int MN = OP::M * OP::N;
+        for (int e = tid; e < (UP ? 2 : 1) * MN; e += NT) {
+          const int kind = e / MN, r = (e % MN) / OP::N, k = e % OP::N;
+          if constexpr (UP) {
+            ctab[base + e] = kind == 0 ? OP::diff_start[r][k] : OP::diss_start[r][k];
+          } else {
+            ctab[base + e] = OP::diff_start[r][k];
+          }
+        }
+      };
+      fill(OX{}, 0);
+      if constexpr (!SAMEOP) fill(OY{}, TABX);
+    }
+  }
+  const double* tabx = ctab;
+  const double* taby = ctab + (SAMEOP ? 0 : TABX);
+
   // state of the stage input at offset o (FP64; the mixed form adds Δ)
   const bool pm = !MIX && stage_input_pm(A, mode);  // point-major FP64 stage input
   // (dk: where to keep the mixed form's increment Δ of the point as well)
@@ -265,13 +296,13 @@ __global__ __launch_bounds__((TX / RX) * (TY / RY)) __attribute__((amdgpu_waves_
   const uint32_t oc = (uint32_t)(min(y0 + TY / 2, ny - 1) * ld + min(x0 + TX / 2, nx - 1));
 
   CT acc[RY][RX][4];
-  // metric products at the own output points (OWNQ): J·ξx, J·ξy from the ξ
-  // phase (parked in LDS, each thread reading back its own slots, to keep
-  // them out of registers), J·ηx, J·ηy from the η phase;
-  // J = J·ξx J·ηy - J·ξy J·ηx (= y_η x_ξ - x_η y_ξ, sbp/src/grid.rs:110-131)
-  // replaces a load of J.
+  // metric products at the own output points (OWNQ), all four loaded in the
+  // ξ phase: J·ηx, J·ηy stay in registers for the η phase, and
+  // J = J·ξx J·ηy - J·ξy J·ηx (= y_η x_ξ - x_η y_ξ, sbp/src/grid.rs:110-131),
+  // which replaces a load of J, is parked in LDS for the finish (each thread
+  // reading back its own slots, to keep it out of registers).
   double ownm[OWNQ ? RY : 1][2];
-  __shared__ double xsave[OWNQ ? 2 * RY * NT : 1];
+  __shared__ double jsave[OWNQ ? RY * NT : 1];
   // stage input at the own output points (OWNQ): loaded in the ξ phase, used
   // again by the η phase (which loads only J·ηx, J·ηy there) and the finish
   double ownq[OWNQ ? RY : 1][4];
@@ -341,8 +372,7 @@ __global__ __launch_bounds__((TX / RX) * (TY / RY)) __attribute__((amdgpu_waves_
         if (i0 == 0) {
 #pragma unroll
           for (int s2 = 0; s2 < RY; ++s2) {
-            xsave[(2 * s2) * NT + tid] = v[s2][4];
-            xsave[(2 * s2 + 1) * NT + tid] = v[s2][5];
+            jsave[s2 * NT + tid] = v[s2][4] * ownm[s2][1] - v[s2][5] * ownm[s2][0];
 #pragma unroll
             for (int q = 0; q < 4; ++q) ownq[s2][q] = v[s2][q];
           }
@@ -394,7 +424,8 @@ __global__ __launch_bounds__((TX / RX) * (TY / RY)) __attribute__((amdgpu_waves_
 #pragma unroll
             for (int i = 0; i < RX; ++i) {
               CT o[4] = {CT(0), CT(0), CT(0), CT(0)};
-              if (xin[i]) row_sum4<BB>(line + slot * NP, NP, 1, xb + i, nx, rx0, o);
+              if (xin[i])
+                row_sum4_tab<BB>(line + slot * NP, NP, 1, xb + i, nx, rx0, tabx + BB::table * OX::M * OX::N, o);
 #pragma unroll
               for (int q = 0; q < 4; ++q) acc[s][i][q] = first ? sg * o[q] : fma_t(sg, o[q], acc[s][i][q]);
             }
@@ -460,7 +491,7 @@ __global__ __launch_bounds__((TX / RX) * (TY / RY)) __attribute__((amdgpu_waves_
       for (int i = 0; i < RX; ++i) {
         const uint32_t o = (xin[i] && yin[s]) ? (uint32_t)((yb + s) * ld + xb + i) : (uint32_t)(y0 * ld + x0);
         if constexpr (OWNQ) {
-          fJ[s][i] = xsave[(2 * s) * NT + tid] * ownm[s][1] - xsave[(2 * s + 1) * NT + tid] * ownm[s][0];
+          fJ[s][i] = jsave[s * NT + tid];
         } else {
           fJ[s][i] = ldg(met, o);
         }
@@ -587,7 +618,9 @@ __global__ __launch_bounds__((TX / RX) * (TY / RY)) __attribute__((amdgpu_waves_
 #pragma unroll
             for (int i = 0; i < RX; ++i) {
               CT o[4] = {CT(0), CT(0), CT(0), CT(0)};
-              if (yin[s] && xin[i]) row_sum4<BB>(lds + slot * NP + tc * RX + i, NP, TX, yb + s, ny, ry0, o);
+              if (yin[s] && xin[i])
+                row_sum4_tab<BB>(lds + slot * NP + tc * RX + i, NP, TX, yb + s, ny, ry0,
+                                 taby + BB::table * OY::M * OY::N, o);
 #pragma unroll
               for (int q = 0; q < 4; ++q) acc[s][i][q] = fma_t(sg, o[q], acc[s][i][q]);
             }

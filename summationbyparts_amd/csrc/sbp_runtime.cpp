@@ -654,11 +654,48 @@ int rebuild(sbp_system_s* s) {
         }
         for (size_t gi = 0; gi < s->groups.size(); ++gi) s->groups[gi].shape = saved[gi];
       }
+    // the three fastest configurations of each group again, three more
+    // rounds each: shapes within a few percent of each other trade places
+    // from one timing to the next, and the runs that follow last much longer
+    constexpr int kFinalists = 3;
+    std::vector<std::vector<int>> fin(s->groups.size());
+    for (size_t gi = 0; gi < s->groups.size(); ++gi) {
+      std::vector<int> order(kNumTileShapes);
+      for (int sh = 0; sh < kNumTileShapes; ++sh) order[sh] = sh;
+      std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return cfg_ms[gi][a] < cfg_ms[gi][b]; });
+      for (int k = 0; k < kFinalists && k < kNumTileShapes; ++k)
+        if (cfg_ms[gi][order[k]] < 1e30f) fin[gi].push_back(order[k]);
+    }
+    std::vector<std::vector<float>> fin_ms(s->groups.size(), std::vector<float>(kFinalists, 1e30f));
+    for (int round = 0; round < 3; ++round)
+      for (int k = 0; k < kFinalists; ++k) {
+        std::vector<int> saved;
+        bool any = false;
+        for (size_t gi = 0; gi < s->groups.size(); ++gi) {
+          Group& g = s->groups[gi];
+          saved.push_back(g.shape);
+          if (g.shape < 0 && k < (int)fin[gi].size()) {
+            g.shape = fin[gi][k];
+            any = true;
+          }
+        }
+        if (any) {
+          SBP_TRY(upload());
+          for (size_t gi = 0; gi < s->groups.size(); ++gi) {
+            Group& g = s->groups[gi];
+            if (saved[gi] >= 0 || k >= (int)fin[gi].size()) continue;
+            float ms = 0.f;
+            SBP_TRY_HIP(time_group(s, g, g.shape, g.sched < 0 ? 0 : g.sched, e0, e1, ms));
+            fin_ms[gi][k] = std::min(fin_ms[gi][k], ms);
+          }
+        }
+        for (size_t gi = 0; gi < s->groups.size(); ++gi) s->groups[gi].shape = saved[gi];
+      }
     for (size_t gi = 0; gi < s->groups.size(); ++gi)
-      for (int sh = 0; sh < kNumTileShapes; ++sh)
-        if (cfg_ms[gi][sh] < best_ms[gi]) {
-          best_ms[gi] = cfg_ms[gi][sh];
-          best[gi] = sh;
+      for (int k = 0; k < (int)fin[gi].size(); ++k)
+        if (fin_ms[gi][k] < best_ms[gi]) {
+          best_ms[gi] = fin_ms[gi][k];
+          best[gi] = fin[gi][k];
         }
     hipEventDestroy(e0);
     hipEventDestroy(e1);
@@ -668,8 +705,8 @@ int rebuild(sbp_system_s* s) {
       if (s->groups[gi].shape < 0) s->groups[gi].shape = best[gi];
   }
   if (need_sched) {
-    // schedule at the chosen shape: each of the three, two rounds each, the
-    // smaller time of each
+    // schedule at the chosen shape: each of the three, four rounds each, the
+    // smaller time of each (they differ by a few percent at most)
     SBP_TRY(upload());
     hipEvent_t e0, e1;
     SBP_TRY_HIP(hipEventCreate(&e0));
@@ -677,7 +714,7 @@ int rebuild(sbp_system_s* s) {
     for (auto& g : s->groups) {
       if (g.sched >= 0) continue;
       float t[3] = {1e30f, 1e30f, 1e30f};
-      for (int round = 0; round < 2; ++round)
+      for (int round = 0; round < 4; ++round)
         for (int m = 0; m < 3; ++m) {
           float ms = 0.f;
           SBP_TRY_HIP(time_group(s, g, g.shape, m, e0, e1, ms));

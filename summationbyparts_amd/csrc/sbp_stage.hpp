@@ -397,8 +397,14 @@ __global__ __launch_bounds__((TX / RX) * (TY / RY)) __attribute__((amdgpu_waves_
           for (int q = 0; q < 4; ++q) lds[q * NP + l] = f[q];
           if constexpr (UP) {
             if constexpr (VAR == 0) {
+              // own points keep α only: α·q is formed again from the stage
+              // input held in registers (the same products)
+              if (OWNQ && it < RY) {
+                cache[it][0] = al;
+              } else {
 #pragma unroll
-              for (int q = 0; q < 4; ++q) cache[it][q] = aq[q];
+                for (int q = 0; q < 4; ++q) cache[it][q] = aq[q];
+              }
             } else if constexpr (VAR == 1) {
               cache[it][0] = al;
             } else {
@@ -435,13 +441,13 @@ __global__ __launch_bounds__((TX / RX) * (TY / RY)) __attribute__((amdgpu_waves_
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
           const CT* lq = line + (slot + q) * NP;
+          // the window covers the ξ halo: the edge instantiation took the
+          // closure path above on tiles without it, and the interior
+          // instantiation only gets tiles off the block borders (x0 >= TX >
+          // HX, so the region starts at x0 - HX; rows it writes read at most
+          // HX columns past them, inside the block and the region)
           CT o[RX];
-          if (interior) {
-            row_window<BB, RX>(lq + (xb - rx0 - HX), o);
-          } else {
-#pragma unroll
-            for (int i = 0; i < RX; ++i) o[i] = xin[i] ? row_sum<BB, false>(lq, 1, xb + i, nx, rx0) : CT(0);
-          }
+          row_window<BB, RX>(lq + (xb - rx0 - HX), o);
 #pragma unroll
           for (int i = 0; i < RX; ++i) acc[s][i][q] = first ? sg * o[i] : fma_t(sg, o[i], acc[s][i][q]);
         }
@@ -458,8 +464,13 @@ __global__ __launch_bounds__((TX / RX) * (TY / RY)) __attribute__((amdgpu_waves_
           if (xi_pt(it, r, c)) {
             const int l = r * WX + c;
             if constexpr (NC == 4) {
+              if (OWNQ && it < RY) {
 #pragma unroll
-              for (int q = 0; q < 4; ++q) lds[q * NP + l] = cache[it][q];
+                for (int q = 0; q < 4; ++q) lds[q * NP + l] = cache[it][0] * ownq[OWNQ ? it : 0][q];
+              } else {
+#pragma unroll
+                for (int q = 0; q < 4; ++q) lds[q * NP + l] = cache[it][q];
+              }
             } else {
               double qq[4];
               load_q((uint32_t)((y0 + r) * ld + rx0 + c), qq);
@@ -565,10 +576,13 @@ __global__ __launch_bounds__((TX / RX) * (TY / RY)) __attribute__((amdgpu_waves_
         const bool ok = i0 + j < ITYE && eta_pt(i0 + j, r, c);
         const uint32_t o = ok ? (uint32_t)((ry0 + r) * ld + x0 + c) : (uint32_t)(ry0 * ld + x0);
         if (OWNQ && i0 + j < RY) {
+          // the own points' stage input from registers; their η metrics are
+          // loaded again (a cache hit) rather than held through the ξ phase
 #pragma unroll
           for (int q = 0; q < 4; ++q) v[j][q] = ownq[OWNQ ? i0 + j : 0][q];
-          v[j][4] = ownm[OWNQ ? i0 + j : 0][0];
-          v[j][5] = ownm[OWNQ ? i0 + j : 0][1];
+          const d2 me = ldg(meta, o);
+          v[j][4] = me.x;
+          v[j][5] = me.y;
         } else {
           load_q(o, v[j]);
           const d2 me = ldg(meta, o);
@@ -594,8 +608,14 @@ __global__ __launch_bounds__((TX / RX) * (TY / RY)) __attribute__((amdgpu_waves_
           for (int q = 0; q < 4; ++q) lds[q * NP + l] = f[q];
           if constexpr (UP) {
             if constexpr (VAR == 0) {
+              // own points keep α only: α·q is formed again from the stage
+              // input held in registers (the same products)
+              if (OWNQ && it < RY) {
+                cache[it][0] = al;
+              } else {
 #pragma unroll
-              for (int q = 0; q < 4; ++q) cache[it][q] = aq[q];
+                for (int q = 0; q < 4; ++q) cache[it][q] = aq[q];
+              }
             } else if constexpr (VAR == 1) {
               cache[it][0] = al;
             } else {
@@ -630,16 +650,11 @@ __global__ __launch_bounds__((TX / RX) * (TY / RY)) __attribute__((amdgpu_waves_
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
         const CT* col = lds + (slot + q) * NP + tc * RX;
+        // the window covers the η halo (as in the ξ pass: closure tiles took
+        // the edge path, the interior instantiation's tiles start at
+        // y0 >= TY > HY)
         CT o[RY][RX];
-        if (interior) {
-          col_window<BB, RY, RX>(col + (yb - ry0 - HY) * TX, TX, o);
-        } else {
-#pragma unroll
-          for (int s = 0; s < RY; ++s)
-#pragma unroll
-            for (int i = 0; i < RX; ++i)
-              o[s][i] = (yin[s] && xin[i]) ? row_sum<BB, false>(col + i, TX, yb + s, ny, ry0) : CT(0);
-        }
+        col_window<BB, RY, RX>(col + (yb - ry0 - HY) * TX, TX, o);
 #pragma unroll
         for (int s = 0; s < RY; ++s)
 #pragma unroll
@@ -657,8 +672,13 @@ __global__ __launch_bounds__((TX / RX) * (TY / RY)) __attribute__((amdgpu_waves_
           if (eta_pt(it, r, c)) {
             const int l = r * TX + c;
             if constexpr (NC == 4) {
+              if (OWNQ && it < RY) {
 #pragma unroll
-              for (int q = 0; q < 4; ++q) lds[q * NP + l] = cache[it][q];
+                for (int q = 0; q < 4; ++q) lds[q * NP + l] = cache[it][0] * ownq[OWNQ ? it : 0][q];
+              } else {
+#pragma unroll
+                for (int q = 0; q < 4; ++q) lds[q * NP + l] = cache[it][q];
+              }
             } else {
               double qq[4];
               load_q((uint32_t)((ry0 + r) * ld + x0 + c), qq);

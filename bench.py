@@ -265,7 +265,8 @@ def time_steps(w, steps, warmup, tile=-1, schedule=-1):
     return {"value": w.points * steps / wall / 1e9, "wall": wall, "ms_per_step": wall * 1e3 / steps,
             "event_ms_per_step": ms_timed / steps, "stage_launch_ms": ms_timed / (4 * steps),
             "tile": list(s.tile_shape()), "tile_index": s.tile_index(),
-            "schedule": ["split", "merged", "merged after the edge-SAT kernel"][s.schedule()]}
+            "schedule": ["split", "merged", "merged after the edge-SAT kernel"][s.schedule()],
+            "tile_order": ["XCD bands", "row-major"][s.tile_order()]}
 
 
 def line(name, w, precision, t, steps, warmup, traffic_key=None, metric=METRIC):
@@ -274,7 +275,7 @@ def line(name, w, precision, t, steps, warmup, traffic_key=None, metric=METRIC):
     if traffic_key is not None:
         traffic, source = stored_traffic(traffic_key, precision, t["tile_index"], w.points)
     cfg = {"workload": w.desc, **w.cfg, "precision": precision, "tile": t["tile"], "tile_index": t["tile_index"],
-           "schedule": t["schedule"],
+           "schedule": t["schedule"], "tile_order": t["tile_order"],
            "event_ms_per_step": t["event_ms_per_step"], "stage_launch_ms": t["stage_launch_ms"],
            "points": w.points}
     return {"metric": metric, "value": t["value"], "unit": UNIT, "n_gpus": 1, "steps": steps, "warmup": warmup,

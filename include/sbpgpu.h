@@ -374,6 +374,10 @@ int sbp_tile_shape_get(int index, int* tx, int* ty, int* ry);
 int sbp_system_set_schedule(sbp_system sys, int schedule);
 /* Schedule used for the first operator group (after autotuning). */
 int sbp_system_schedule(sbp_system sys, int* schedule);
+/* Tile order of the first operator group's dispatches (no counterpart in the
+ * reference; tuned with the schedule): 0 = each XCD takes a contiguous band
+ * of tiles, 1 = row-major order. */
+int sbp_system_tile_order(sbp_system sys, int* order);
 
 #ifdef __cplusplus
 }

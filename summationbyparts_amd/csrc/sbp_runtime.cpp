@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -230,6 +231,10 @@ struct Group {
   // schedule: 0 split over two streams, 1 one dispatch, 2 the edge-SAT kernel
   // and then one dispatch (-1 not chosen yet)
   int sched = -1;
+  // order of the tiles in a dispatch (tuned with the schedule): 0 = each
+  // XCD gets a contiguous band (workgroup g runs on XCD g mod 8), 1 = plain
+  // row-major order within each block
+  int order = 0;
   // the tiles on block edges (they add the edge SATs), then the interior tiles
   int etile_offset = 0, entiles = 0;
   int tile_offset = 0, ntiles = 0;
@@ -366,7 +371,8 @@ void xcd_order(const std::vector<TileDev>& nat, std::vector<TileDev>& out) {
       if (start[x] + j < start[x + 1]) out.push_back(nat[start[x] + j]);
 }
 
-// The group's interior tiles and its tiles on block edges, each list in XCD order.
+// The group's interior tiles and its tiles on block edges, each list in XCD
+// band order (order 0) or row-major order (order 1).
 void build_tiles(const sbp_system_s* s, const Group& g, int shape, std::vector<TileDev>& interior,
                  std::vector<TileDev>& edge) {
   const TileShape& ts = kTileShapes[shape];
@@ -379,6 +385,11 @@ void build_tiles(const sbp_system_s* s, const Group& g, int shape, std::vector<T
         const TileDev t{bi, (int16_t)tx, (int16_t)ty};
         (tx == 0 || tx == kx - 1 || ty == 0 || ty == ky - 1 ? ne : ni).push_back(t);
       }
+  }
+  if (g.order == 1) {
+    interior = ni;
+    edge = ne;
+    return;
   }
   xcd_order(ni, interior);
   xcd_order(ne, edge);
@@ -504,7 +515,8 @@ int rebuild(sbp_system_s* s) {
     auto it = s->ctx->tuned.find(key);
     const bool cached = it != s->ctx->tuned.end() && s->forced_shape < 0 && s->forced_schedule < 0;
     g.shape = s->forced_shape >= 0 ? s->forced_shape : cached ? (it->second & 0xff) : -1;
-    g.sched = s->forced_schedule >= 0 ? s->forced_schedule : cached ? (it->second >> 8) : -1;
+    g.sched = s->forced_schedule >= 0 ? s->forced_schedule : cached ? ((it->second >> 8) & 0xf) : -1;
+    g.order = cached ? (it->second >> 12) : 0;
   }
   bool need_tune = false, need_sched = false;
   std::vector<bool> tune_cache(s->groups.size());
@@ -705,24 +717,42 @@ int rebuild(sbp_system_s* s) {
       if (s->groups[gi].shape < 0) s->groups[gi].shape = best[gi];
   }
   if (need_sched) {
-    // schedule at the chosen shape: each of the three, four rounds each, the
-    // smaller time of each (they differ by a few percent at most)
-    SBP_TRY(upload());
+    // schedule and tile order at the chosen shape: each of the three
+    // schedules in each of the two orders, four rounds each, the smaller time
+    // of each (they differ by a few percent at most)
     hipEvent_t e0, e1;
     SBP_TRY_HIP(hipEventCreate(&e0));
     SBP_TRY_HIP(hipEventCreate(&e1));
-    for (auto& g : s->groups) {
+    std::vector<std::array<std::array<float, 3>, 2>> t(s->groups.size());
+    for (auto& tg : t)
+      for (auto& to : tg) to.fill(1e30f);
+    for (int o = 0; o < 2; ++o) {
+      for (auto& g : s->groups)
+        if (g.sched < 0) g.order = o;
+      SBP_TRY(upload());
+      for (size_t gi = 0; gi < s->groups.size(); ++gi) {
+        Group& g = s->groups[gi];
+        if (g.sched >= 0) continue;
+        for (int round = 0; round < 4; ++round)
+          for (int m = 0; m < 3; ++m) {
+            float ms = 0.f;
+            SBP_TRY_HIP(time_group(s, g, g.shape, m, e0, e1, ms));
+            t[gi][o][m] = std::min(t[gi][o][m], ms);
+          }
+      }
+    }
+    for (size_t gi = 0; gi < s->groups.size(); ++gi) {
+      Group& g = s->groups[gi];
       if (g.sched >= 0) continue;
-      float t[3] = {1e30f, 1e30f, 1e30f};
-      for (int round = 0; round < 4; ++round)
-        for (int m = 0; m < 3; ++m) {
-          float ms = 0.f;
-          SBP_TRY_HIP(time_group(s, g, g.shape, m, e0, e1, ms));
-          t[m] = std::min(t[m], ms);
-        }
-      g.sched = 0;
-      for (int m = 1; m < 3; ++m)
-        if (t[m] < t[g.sched]) g.sched = m;
+      int bo = 0, bm = 0;
+      for (int o = 0; o < 2; ++o)
+        for (int m = 0; m < 3; ++m)
+          if (t[gi][o][m] < t[gi][bo][bm]) {
+            bo = o;
+            bm = m;
+          }
+      g.order = bo;
+      g.sched = bm;
     }
     hipEventDestroy(e0);
     hipEventDestroy(e1);
@@ -734,7 +764,7 @@ int rebuild(sbp_system_s* s) {
     int64_t pts = 0;
     for (int bi : g.blocks) pts += (int64_t)s->blocks[bi].nx * s->blocks[bi].ny;
     s->ctx->tuned[std::make_tuple(g.op_eta, g.op_xi, tune_kind(s, g), pts, (int)g.blocks.size())] =
-        g.shape | (g.sched << 8);
+        g.shape | (g.sched << 8) | (g.order << 12);
   }
   SBP_TRY(upload());
   destroy_graph(s);
@@ -2040,6 +2070,13 @@ int sbp_system_schedule(sbp_system s, int* schedule) {
   if (!s || !schedule) return SBP_ERR_INVALID;
   SBP_TRY(rebuild(s));
   *schedule = s->groups.empty() ? 0 : s->groups[0].sched;
+  return SBP_OK;
+}
+
+int sbp_system_tile_order(sbp_system s, int* order) {
+  if (!s || !order) return SBP_ERR_INVALID;
+  SBP_TRY(rebuild(s));
+  *order = s->groups.empty() ? 0 : s->groups[0].order;
   return SBP_OK;
 }
 

@@ -141,6 +141,12 @@ class BlockSystem:
         _lib.check(_lib.lib().sbp_system_schedule(self._h, ctypes.byref(i)), "sbp_system_schedule")
         return i.value
 
+    def tile_order(self):
+        """Order of the tiles in a dispatch: 0 XCD bands, 1 row-major (tuned with the schedule)."""
+        i = ctypes.c_int()
+        _lib.check(_lib.lib().sbp_system_tile_order(self._h, ctypes.byref(i)), "sbp_system_tile_order")
+        return i.value
+
     # ---- state
     def set_state(self, block, q):
         q = np.ascontiguousarray(q, dtype=np.float64)

@@ -66,7 +66,8 @@ MIXED_TOL = 1e-8
 TRAFFIC_FILES = {("c2", "fp64"): "stage_kernel_upwind9_4096_traffic.json",
                  ("c2", "mixed"): "stage_kernel_upwind9_4096_mixed_traffic.json",
                  ("c3", "fp64"): "stage_kernel_c3_24_blocks_upwind4_traffic.json",
-                 ("c5", "fp64"): "stage_kernel_c5_256_solves_upwind9_traffic.json"}
+                 ("c5", "fp64"): "stage_kernel_c5_256_solves_upwind9_traffic.json",
+                 ("c5", "mixed"): "stage_kernel_c5_256_solves_upwind9_mixed_traffic.json"}
 
 
 def _counter_rows(path, counter):

@@ -374,9 +374,12 @@ int sbp_tile_shape_get(int index, int* tx, int* ty, int* ry);
 int sbp_system_set_schedule(sbp_system sys, int schedule);
 /* Schedule used for the first operator group (after autotuning). */
 int sbp_system_schedule(sbp_system sys, int* schedule);
-/* Tile order of the first operator group's dispatches (no counterpart in the
- * reference; tuned with the schedule): 0 = each XCD takes a contiguous band
- * of tiles, 1 = row-major order. */
+/* Tile order of the dispatches (no counterpart in the reference; tuned with
+ * the schedule): 0 = each XCD takes a contiguous band of tiles, 1 = row-major
+ * order, -1 = chosen by the autotuner (the default). The results do not
+ * depend on it. */
+int sbp_system_set_tile_order(sbp_system sys, int order);
+/* Tile order of the first operator group's dispatches (after autotuning). */
 int sbp_system_tile_order(sbp_system sys, int* order);
 
 #ifdef __cplusplus

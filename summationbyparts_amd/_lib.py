@@ -107,6 +107,7 @@ def _declare(L):
         "sbp_system_set_schedule": ([c_void_p, c_int], c_int),
         "sbp_system_schedule": ([c_void_p, ip], c_int),
         "sbp_system_tile_order": ([c_void_p, ip], c_int),
+        "sbp_system_set_tile_order": ([c_void_p, c_int], c_int),
         "sbp_system_set_bc_interpolate": ([c_void_p, c_int, c_int, c_int, c_int], c_int),
         "sbp_system_advance_adaptive": ([c_void_p, c_double, dp, c_double], c_int),
         "sbp_system_set_bc_multigrid": ([c_void_p, c_int, c_int, c_int, ip], c_int),
@@ -147,7 +148,7 @@ EXPORTED = [
     "sbp_system_get_metrics", "sbp_system_vortex", "sbp_system_vortex_block", "sbp_system_set_dt_scale",
     "sbp_system_error_vortex_block", "sbp_system_rhs", "sbp_system_advance", "sbp_system_get_time",
     "sbp_system_set_time", "sbp_system_max_dt", "sbp_system_error_vortex", "sbp_system_energy", "sbp_system_check",
-    "sbp_system_profile", "sbp_system_tile_shape", "sbp_system_set_tile_shape", "sbp_system_tile_index", "sbp_system_set_schedule", "sbp_system_schedule", "sbp_system_tile_order", "sbp_system_advance_timed",
+    "sbp_system_profile", "sbp_system_tile_shape", "sbp_system_set_tile_shape", "sbp_system_tile_index", "sbp_system_set_schedule", "sbp_system_schedule", "sbp_system_tile_order", "sbp_system_set_tile_order", "sbp_system_advance_timed",
     "sbp_system_set_bc_interpolate", "sbp_system_set_bc_multigrid", "sbp_interpolate", "sbp_interp_info",
     "sbp_system_advance_adaptive",
     "sbp_interp_coefficients",

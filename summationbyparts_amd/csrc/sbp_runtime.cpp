@@ -270,6 +270,7 @@ struct sbp_system_s {
   int parity = 0;
   int forced_shape = -1;
   int forced_schedule = -1;
+  int forced_order = -1;
   bool dirty = true;  // device descriptors / tiles need rebuilding
   // captured steps: graph of `graph_steps` RK4 steps for step size graph_dt
   hipGraphExec_t graph = nullptr;
@@ -513,10 +514,11 @@ int rebuild(sbp_system_s* s) {
     auto key = std::make_tuple(g.op_eta, g.op_xi, tune_kind(s, g), pts, (int)g.blocks.size());
     // cached choices: shape | schedule << 8, for groups tuned without a forced shape or schedule
     auto it = s->ctx->tuned.find(key);
-    const bool cached = it != s->ctx->tuned.end() && s->forced_shape < 0 && s->forced_schedule < 0;
+    const bool cached =
+        it != s->ctx->tuned.end() && s->forced_shape < 0 && s->forced_schedule < 0 && s->forced_order < 0;
     g.shape = s->forced_shape >= 0 ? s->forced_shape : cached ? (it->second & 0xff) : -1;
     g.sched = s->forced_schedule >= 0 ? s->forced_schedule : cached ? ((it->second >> 8) & 0xf) : -1;
-    g.order = cached ? (it->second >> 12) : 0;
+    g.order = s->forced_order >= 0 ? s->forced_order : cached ? (it->second >> 12) : 0;
   }
   bool need_tune = false, need_sched = false;
   std::vector<bool> tune_cache(s->groups.size());
@@ -524,7 +526,7 @@ int rebuild(sbp_system_s* s) {
     const Group& g = s->groups[gi];
     need_tune = need_tune || g.shape < 0;
     need_sched = need_sched || g.sched < 0;
-    tune_cache[gi] = g.shape < 0 && g.sched < 0;
+    tune_cache[gi] = g.shape < 0 && g.sched < 0 && s->forced_order < 0;
   }
   auto upload = [&](void) -> int {
     tiles.clear();
@@ -727,6 +729,7 @@ int rebuild(sbp_system_s* s) {
     for (auto& tg : t)
       for (auto& to : tg) to.fill(1e30f);
     for (int o = 0; o < 2; ++o) {
+      if (s->forced_order >= 0 && o != s->forced_order) continue;
       for (auto& g : s->groups)
         if (g.sched < 0) g.order = o;
       SBP_TRY(upload());
@@ -2070,6 +2073,13 @@ int sbp_system_schedule(sbp_system s, int* schedule) {
   if (!s || !schedule) return SBP_ERR_INVALID;
   SBP_TRY(rebuild(s));
   *schedule = s->groups.empty() ? 0 : s->groups[0].sched;
+  return SBP_OK;
+}
+
+int sbp_system_set_tile_order(sbp_system s, int order) {
+  if (!s || order < -1 || order > 1) return SBP_ERR_INVALID;
+  s->forced_order = order;
+  s->dirty = true;
   return SBP_OK;
 }
 

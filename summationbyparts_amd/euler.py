@@ -141,6 +141,11 @@ class BlockSystem:
         _lib.check(_lib.lib().sbp_system_schedule(self._h, ctypes.byref(i)), "sbp_system_schedule")
         return i.value
 
+    def set_tile_order(self, order):
+        """Tile order of the dispatches: 0 XCD bands, 1 row-major, -1 autotuned
+        (results do not depend on it)."""
+        _lib.check(_lib.lib().sbp_system_set_tile_order(self._h, order), "sbp_system_set_tile_order")
+
     def tile_order(self):
         """Order of the tiles in a dispatch: 0 XCD bands, 1 row-major (tuned with the schedule)."""
         i = ctypes.c_int()

@@ -54,12 +54,14 @@ BYTES_PER_POINT_STEP = {"fp64": 672.0, "mixed": 576.0}
 # (96 B each); stage 4 reads y_n, Δ_3, metrics, Δ_1, Δ_2 and writes y_{n+1}
 # (144 B): 416 B.
 OWN_BYTES_PER_POINT_STEP = {"fp64": 512.0, "mixed": 416.0}
-# advance_adaptive's embedded Bogacki-Shampine step, unfused: four RHS passes
-# (stage input, 4 metric planes in, k out: 96 B each), the stage inputs
-# (y_n + k0; y_n + k1; y_n + k0..k2: 96, 96, 160 B), fut (160 B), the
-# embedded solution (192 B), the H-norm error (64 B) and the copy of an
-# accepted step (64 B): 1216 B per point per accepted step.
-BYTES_PER_POINT_ADAPTIVE_STEP = 1216.0
+# advance_adaptive's embedded Bogacki-Shampine trial step, fused into four
+# stage launches: stage 1 reads y_n and the metric pairs and writes Y2
+# (96 B); stage 2 reads Y2, the metrics and y_n and writes Y3 (128 B);
+# stage 3 reads Y3, the metrics, y_n and Y2 and writes the solution (160 B);
+# stage 4 reads the solution, the metrics, y_n and Y2 and writes only the
+# per-tile error sums (128 B): 512 B per point per trial step (the accepted
+# solution is copied back once per call, not per step).
+BYTES_PER_POINT_ADAPTIVE_STEP = 512.0
 MIXED_TOL = 1e-8
 # HBM bytes per stage launch from rocprofv3 counters (bench.py --counters),
 # per workload and precision
@@ -655,17 +657,17 @@ def run_adaptive(n, steps, warmup, no_cpu):
     pts = n * n
     ms = wall * 1e3 / steps
     unit = "Gpoint-steps/s"
-    # the roofline of one accepted step taken as a whole (its RHS, stage and
-    # error passes are separate launches)
+    # the roofline of one accepted step taken as a whole (four stage launches,
+    # the per-block error sums and their read-back)
     out = {"metric": "Gpoint-embedded-RK-steps/s (advance_adaptive, Bogacki-Shampine)", "value": pts * steps / wall / 1e9,
            "unit": unit, "n_gpus": 1, "steps": steps, "warmup": warmup, "ms_per_step": ms,
            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
            "config": {"workload": f"advance_adaptive on the {n}x{n} curvilinear upwind9 block, periodic, all steps accepted",
                       "nx": n, "ny": n, "operator": "upwind9"},
            "roofline": roofline("fp64", pts, ms / 4.0, bytes_per_point_step=BYTES_PER_POINT_ADAPTIVE_STEP,
-                                definition="1216 B per point per accepted step of the unfused embedded step "
-                                           "(RHS passes, stage inputs, fut, embedded solution, H-norm error, copy), "
-                                           "over the step's wall time")}
+                                definition="512 B per point per trial step of the fused embedded step "
+                                           "(four stage launches; the embedded solution is never stored, its "
+                                           "H-norm error is summed per tile), over the step's wall time")}
     if not no_cpu:
         m = 512
         Xs, Ys = curvilinear(m, m)

@@ -222,6 +222,50 @@ __device__ __forceinline__ void row_sum4_tab(const T* line, int qs, int stride, 
   for (int q = 0; q < 4; ++q) out[q] = interior_sum<B>(line + q * qs + (i - r0) * stride, stride);
 }
 
+// Neumaier (compensated) sums: a running sum and its compensation.
+struct KSum {
+  double s, c;
+};
+
+__device__ __forceinline__ void ksum_add(KSum& a, double v) {
+  const double t = a.s + v;
+  if (fabs(a.s) >= fabs(v))
+    a.c += (a.s - t) + v;
+  else
+    a.c += (v - t) + a.s;
+  a.s = t;
+}
+
+__device__ __forceinline__ void ksum_merge(KSum& a, const KSum& b) {
+  ksum_add(a, b.s);
+  a.c += b.c;
+}
+
+// Lane 0 of each wavefront ends with the merge of its 64 lanes.
+__device__ __forceinline__ KSum wave_ksum(KSum a) {
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) {
+    const KSum b = {__shfl_down(a.s, off, 64), __shfl_down(a.c, off, 64)};
+    ksum_merge(a, b);
+  }
+  return a;
+}
+
+// h2_err weight of point (i, j) of an n x m block (lib.rs:389-421): the
+// diagonal norms [H, 1.., rev(H)] of both operators, scaled by 1/(n-1)
+// (1/(n-2) for h2 operators).
+template <class OX, class OY>
+__device__ __forceinline__ double hweight(int i, int n, int j, int m) {
+  double hx = 1.0, hy = 1.0;
+  if (i < OX::M) hx = OX::h[i];
+  else if (i >= n - OX::M) hx = OX::h[n - 1 - i];
+  if (j < OY::M) hy = OY::h[j];
+  else if (j >= m - OY::M) hy = OY::h[m - 1 - j];
+  const double fx = OX::is_h2 ? 1.0 / (double)(n - 2) : 1.0 / (double)(n - 1);
+  const double fy = 1.0 / (OY::is_h2 ? (double)(m - 2) : (double)(m - 1));
+  return (hx * fx) * (hy * fy);
+}
+
 // Tile placement along one axis of length n with K = ceil(n/T) tiles: tile t
 // computes [x0, x0+T) and writes [w0, w1). The last tile is right-aligned and
 // owns every end-closure row, so a halo of H points always covers the rows a
@@ -448,7 +492,9 @@ __device__ __forceinline__ void vortex_point(const VortexDev& vp, double gamma, 
 // Y2, Y3, Y4 and W are internal to a step and stored point-major:
 // the four components of point o at doubles 4o .. 4o + 3, two 16-byte loads
 // per point; y_n and every buffer a caller sees keep the plane layout.
-__device__ __forceinline__ bool stage_input_pm(const StageArgs& A, int mode) { return !A.mixed && mode >= MODE_S1; }
+__device__ __forceinline__ bool stage_input_pm(const StageArgs& A, int mode) {
+  return !A.mixed && ((mode >= MODE_S1 && mode <= MODE_S3) || mode == MODE_BS1 || mode == MODE_BS2);
+}
 __device__ __forceinline__ void load_pm(const double* base, uint32_t o, double q[4]) {
   const d2* p = reinterpret_cast<const d2*>(base);
   const d2 a = ldg(p, 2 * o), b = ldg(p, 2 * o + 1);
@@ -481,6 +527,11 @@ __device__ __forceinline__ const double* stage_input(const StageArgs& A, const B
     case MODE_S1: return B.buf[BUF_YA];
     case MODE_S2: return B.buf[BUF_YB];
     case MODE_S3: return B.buf[BUF_YA];
+    // embedded Bogacki-Shampine stages: y_n, Y2, Y3, the solution
+    case MODE_BS0: return A.swap ? B.buf[BUF_ACC] : B.buf[BUF_YN];
+    case MODE_BS1: return B.buf[BUF_YA];
+    case MODE_BS2: return B.buf[BUF_YB];
+    case MODE_BS3: return A.swap ? B.buf[BUF_YN] : B.buf[BUF_ACC];
     default: return A.rhs_in_sel ? B.rk[A.rhs_in_sel - 1] : B.rhs_in;
   }
 }
@@ -508,7 +559,7 @@ __device__ __forceinline__ f4* delta_out(const BlockDev& B, int mode) { return c
 // The state stage `mode` reads at element offset o.
 __device__ __forceinline__ void load_stage_state(const StageArgs& A, const BlockDev& B, int mode, uint32_t o, double q[4]) {
   const uint32_t ps = (uint32_t)B.ps;
-  if (A.mixed && mode >= MODE_S0) {
+  if (A.mixed && mode >= MODE_S0 && mode <= MODE_S3) {
     const double* yn = yn_buf(B, A);
 #pragma unroll
     for (int c = 0; c < 4; ++c) q[c] = ldg(yn + c * ps, o);

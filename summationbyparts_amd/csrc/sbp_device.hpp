@@ -37,8 +37,23 @@ enum EdgeSrc : int {
   EDGE_NONE = 101,              // no SAT on this edge (RHS_no_SAT)
 };
 
-// Stage-kernel mode: RHS only, or one of the four fused RK4 stages.
-enum StageMode : int { MODE_RHS = -1, MODE_S0 = 0, MODE_S1 = 1, MODE_S2 = 2, MODE_S3 = 3 };
+// Stage-kernel mode: RHS only, one of the four fused RK4 stages, or one of
+// the four fused stages of an embedded Bogacki-Shampine trial step
+// (advance_adaptive; FP64 only): BS0 writes Y2 into YA, BS1 Y3 into YB, BS2
+// the solution into the FP64 buffer that does not hold y_n (StageArgs::swap
+// says which does), BS3 writes no state, only the H-norm sums of
+// (y_n - embedded solution) of each tile into StageArgs::partials.
+enum StageMode : int {
+  MODE_RHS = -1,
+  MODE_S0 = 0,
+  MODE_S1 = 1,
+  MODE_S2 = 2,
+  MODE_S3 = 3,
+  MODE_BS0 = 4,
+  MODE_BS1 = 5,
+  MODE_BS2 = 6,
+  MODE_BS3 = 7
+};
 
 struct VortexDev {
   int32_t n;
@@ -53,6 +68,7 @@ struct VortexDev {
 struct BlockDev {
   int nx, ny, ld, op_xi, op_eta, upwind;
   int ntx, nty;  // tiles along ξ and η for the active tile shape
+  int64_t tile_off;  // slot of the block's first tile in per-tile arrays (StageArgs::partials)
   int64_t ps;
   const double* metrics;
   // metric pairs as the stage kernel reads them: (J·ξx, J·ξy) of point o at
@@ -109,8 +125,9 @@ struct StageArgs {
   double rk_a;  // a_s: weight of the next stage input (a_s·dt_block)
   double time_offset;  // c_s·dt added to *time for boundary data
   int mixed;  // FP32 right-hand side with FP64 state (SBP_MIXED)
-  int swap;   // mixed form, odd steps: y_n lives in BUF_ACC and the accumulator in BUF_YN
+  int swap;   // mixed form (odd steps) and the BS modes: y_n lives in BUF_ACC, the other FP64 buffer is BUF_YN
   int sat_buf;  // 1: the edge tiles read the edge SATs from BlockDev::sat (edge-SAT kernel)
+  double* partials;  // MODE_BS3: (sum, compensation) per tile, at BlockDev::tile_off + ty·ntx + tx
   int rhs_in_sel, rhs_out_sel;  // MODE_RHS buffers: 0 = rhs_in / rhs_out, s > 0 = rk[s - 1]
 };
 

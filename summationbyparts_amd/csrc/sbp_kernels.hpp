@@ -68,6 +68,9 @@ struct H2Args {
   int ny, nx;
 };
 hipError_t launch_h2_sum(int op_eta, int op_xi, const H2Args& args, double* scratch, hipStream_t stream);
+// advance_adaptive: per block, the compensated sum of its tiles' (sum,
+// compensation) pairs (BlockDev::tile_off, ntx·nty slots) into out[2b], out[2b+1].
+hipError_t launch_block_sums(const BlockDev* blocks, int nblocks, const double* tile_sums, double* out, hipStream_t stream);
 // max |û| and max |v̂| (system.rs:561-578) into scratch[2·kReduceMaxWG],
 // scratch[2·kReduceMaxWG + 1] (kReduceScratch doubles of scratch).
 hipError_t launch_maxspeed(const double* q, const double* metrics, int64_t ps, int64_t ld, int ny, int nx, double* scratch,

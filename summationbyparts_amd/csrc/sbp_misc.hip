@@ -252,34 +252,6 @@ hipError_t launch_vortex(const VortexDev& vp, double gamma, double t, const doub
 // a wavefront combines its 64 lanes with cross-lane shuffles and error-free
 // TwoSum merges, the four wavefronts of a workgroup meet in LDS once, and a
 // one-workgroup pass merges the per-workgroup partials into the result.
-struct KSum {
-  double s, c;
-};
-
-__device__ __forceinline__ void ksum_add(KSum& a, double v) {
-  const double t = a.s + v;
-  if (fabs(a.s) >= fabs(v))
-    a.c += (a.s - t) + v;
-  else
-    a.c += (v - t) + a.s;
-  a.s = t;
-}
-
-__device__ __forceinline__ void ksum_merge(KSum& a, const KSum& b) {
-  ksum_add(a, b.s);
-  a.c += b.c;
-}
-
-// Lane 0 of each wavefront ends with the merge of its 64 lanes.
-__device__ __forceinline__ KSum wave_ksum(KSum a) {
-#pragma unroll
-  for (int off = 32; off >= 1; off >>= 1) {
-    const KSum b = {__shfl_down(a.s, off, 64), __shfl_down(a.c, off, 64)};
-    ksum_merge(a, b);
-  }
-  return a;
-}
-
 // Thread 0 of a 256-thread workgroup ends with the merge of all its threads.
 __device__ __forceinline__ KSum block_ksum(KSum a) {
   __shared__ KSum wsum[4];
@@ -298,19 +270,6 @@ __device__ __forceinline__ double wave_max(double v) {
 #pragma unroll
   for (int off = 32; off >= 1; off >>= 1) v = fmax(v, __shfl_down(v, off, 64));
   return v;
-}
-
-template <class OX, class OY>
-__device__ __forceinline__ double hweight(int i, int n, int j, int m) {
-  // h2_err weights (lib.rs:389-421): [H, 1.., rev(H)] scaled by 1/(n-1) (1/(n-2) for h2)
-  double hx = 1.0, hy = 1.0;
-  if (i < OX::M) hx = OX::h[i];
-  else if (i >= n - OX::M) hx = OX::h[n - 1 - i];
-  if (j < OY::M) hy = OY::h[j];
-  else if (j >= m - OY::M) hy = OY::h[m - 1 - j];
-  const double fx = OX::is_h2 ? 1.0 / (double)(n - 2) : 1.0 / (double)(n - 1);
-  const double fy = 1.0 / (OY::is_h2 ? (double)(m - 2) : (double)(m - 1));
-  return (hx * fx) * (hy * fy);
 }
 
 int reduce_workgroups(int ny) { return ny < kReduceMaxWG ? ny : kReduceMaxWG; }
@@ -344,6 +303,28 @@ __global__ __launch_bounds__(256) void h2_partial_kernel(const H2Args R, double*
     partials[2 * blockIdx.x] = acc.s;
     partials[2 * blockIdx.x + 1] = acc.c;
   }
+}
+
+// One workgroup per block: the block's per-tile (sum, compensation) pairs of
+// advance_adaptive's error, merged into out[2b], out[2b + 1].
+__global__ __launch_bounds__(256) void block_sums_kernel(const BlockDev* __restrict__ blocks,
+                                                         const double* __restrict__ tile_sums, double* __restrict__ out) {
+  const BlockDev& B = blocks[blockIdx.x];
+  const int64_t n = (int64_t)B.ntx * B.nty;
+  const double* p = tile_sums + 2 * B.tile_off;
+  KSum acc = {0.0, 0.0};
+  for (int64_t i = threadIdx.x; i < n; i += 256) ksum_merge(acc, KSum{p[2 * i], p[2 * i + 1]});
+  acc = block_ksum(acc);
+  if (threadIdx.x == 0) {
+    out[2 * blockIdx.x] = acc.s;
+    out[2 * blockIdx.x + 1] = acc.c;
+  }
+}
+
+hipError_t launch_block_sums(const BlockDev* blocks, int nblocks, const double* tile_sums, double* out, hipStream_t stream) {
+  if (nblocks <= 0) return hipSuccess;
+  block_sums_kernel<<<nblocks, 256, 0, stream>>>(blocks, tile_sums, out);
+  return hipGetLastError();
 }
 
 // One workgroup: the n per-workgroup (sum, compensation) pairs into

@@ -263,6 +263,13 @@ struct sbp_system_s {
   double* d_time_rhs = nullptr;  // explicit time for sbp_system_rhs
   unsigned* d_err = nullptr;
   double* d_partials = nullptr;
+  // advance_adaptive: per-tile error sums (2 doubles per tile of every
+  // block, BlockDev::tile_off), their per-block totals, a pinned host copy
+  int64_t total_tiles = 0;
+  double* d_tile_sums = nullptr;
+  int64_t tile_sums_cap = 0;
+  double* d_block_sums = nullptr;
+  double* h_block_sums = nullptr;
   int partials_cap = 0;
   int rhs_kind = SBP_RHS_AUTO;
   int precision = SBP_FP64;
@@ -541,6 +548,12 @@ int rebuild(sbp_system_s* s) {
       g.ntiles = (int)ti.size();
       tiles.insert(tiles.end(), ti.begin(), ti.end());
       for (int bi : g.blocks) devs[bi] = make_dev(s, bi, sh);
+    }
+    // per-tile slots (advance_adaptive's error sums): blocks in order
+    s->total_tiles = 0;
+    for (auto& d : devs) {
+      d.tile_off = s->total_tiles;
+      s->total_tiles += (int64_t)d.ntx * d.nty;
     }
     if (s->d_blocks) hipFree(s->d_blocks);
     if (s->d_tiles) hipFree(s->d_tiles);
@@ -1445,6 +1458,9 @@ int sbp_system_destroy(sbp_system s) {
   if (s->d_time) hipFree(s->d_time);
   if (s->d_err) hipFree(s->d_err);
   if (s->d_partials) hipFree(s->d_partials);
+  if (s->d_tile_sums) hipFree(s->d_tile_sums);
+  if (s->d_block_sums) hipFree(s->d_block_sums);
+  if (s->h_block_sums) hipHostFree(s->h_block_sums);
   delete s;
   return SBP_OK;
 }
@@ -1884,72 +1900,68 @@ int sbp_system_advance_adaptive(sbp_system s, double dt, double* guess_dt, doubl
   if (!gamma_get(gamma)) return SBP_ERR_GAMMA;
   SBP_TRY_HIP(hipSetDevice(s->ctx->device));
   hipStream_t st = s->ctx->stream;
-  SBP_TRY(ensure_work_sets(s, 6));
   SBP_TRY(normalize(s));
   SBP_TRY(rebuild(s));
-  // BogackiShampine (integrate/src/lib.rs:138-151): A, B, C (C carries its
-  // leading 0, so integrate's stage time t + dt·C[i-1] (:190-196) places
-  // stage i at the previous stage's node), embedded weights B*
-  static const double A[3][3] = {{1.0 / 2.0, 0.0, 0.0}, {0.0, 3.0 / 4.0, 0.0}, {2.0 / 9.0, 1.0 / 3.0, 4.0 / 9.0}};
-  static const double Bw[4] = {2.0 / 9.0, 1.0 / 3.0, 4.0 / 9.0, 0.0};
-  static const double Bs[4] = {7.0 / 24.0, 1.0 / 4.0, 1.0 / 3.0, 1.0 / 8.0};
+  const int nb = (int)s->blocks.size();
+  if (s->tile_sums_cap < s->total_tiles) {
+    if (s->d_tile_sums) hipFree(s->d_tile_sums);
+    s->d_tile_sums = nullptr;
+    s->tile_sums_cap = 0;
+    SBP_TRY_HIP(hipMalloc(&s->d_tile_sums, sizeof(double) * 2 * s->total_tiles));
+    s->tile_sums_cap = s->total_tiles;
+  }
+  if (!s->d_block_sums) SBP_TRY_HIP(hipMalloc(&s->d_block_sums, sizeof(double) * 2 * nb));
+  if (!s->h_block_sums) SBP_TRY_HIP(hipHostMalloc(&s->h_block_sums, sizeof(double) * 2 * nb, 0));
+  // Each trial step is four fused stage launches per group (modes BS0-BS3,
+  // the stage kernel's finish), then one workgroup per block sums its tiles'
+  // error terms. BogackiShampine (integrate/src/lib.rs:138-151): C carries
+  // its leading 0, so integrate's stage time t + g·C[i-1] (:190-196) places
+  // stage i at the previous stage's node; the next-stage weights 1/2, 3/4
+  // and the solution's 4/9 (with the tableau identities, see the kernel),
+  // the embedded solution's 1/8 of the last derivative.
   static const double C[4] = {0.0, 1.0 / 2.0, 3.0 / 4.0, 1.0};
-  enum { SET_STAGE = 4, SET_SECOND = 5 };  // work sets after k0..k3
-  auto set_of = [](const HostBlock& h, int r) { return h.d_rk + (int64_t)r * 4 * h.ps; };
-  // out = prev + Σ (w_j·g)·k_j over the nonzero weights, in order
-  auto combine = [&](const double* w, int nw, double g, int out_set) -> int {
-    for (auto& h : s->blocks) {
-      LinComb lc{};
-      for (int j = 0; j < nw; ++j) {
-        if (w[j] == 0.0) continue;
-        lc.k[lc.nk] = set_of(h, j);
-        lc.s[lc.nk] = w[j] * g;
-        ++lc.nk;
-      }
-      double* out = out_set < 0 ? h.d_buf[BUF_YA] : set_of(h, out_set);
-      SBP_TRY_HIP(launch_lincomb(out, h.d_buf[BUF_YN], lc, h.ps, h.ld, h.ny, h.nx, st));
-    }
-    return SBP_OK;
-  };
+  static const double Anext[3] = {1.0 / 2.0, 3.0 / 4.0, 4.0 / 9.0};
   // advance_adaptive (euler/src/lib.rs:149-185): the loop's time starts at 0
   // and grows by the trial step inside integrate and once more on acceptance;
   // the error is Field::h2_err of the previous state against the embedded
-  // (second) solution; accepted steps grow the guess by 1.05, rejected ones
-  // shrink it by 0.8. The boundary data sees the loop's stage times.
+  // (second) solution, summed over blocks (not in quadrature, as
+  // System::error); accepted steps grow the guess by 1.05, rejected ones
+  // shrink it by 0.8. The boundary data sees the loop's stage times. An
+  // accepted solution lives in the other FP64 buffer (`swap`) and is copied
+  // back once at the end.
   double time = 0.0, g = *guess_dt;
+  int swap = 0;
   while (time < dt) {
     for (int i = 0; i < 4; ++i) {
-      double stime = time;
-      if (i > 0) {
-        SBP_TRY(combine(A[i - 1], i, g, SET_STAGE));
-        stime = time + g * C[i - 1];
-      }
-      StageArgs a = stage_args(s, 0.0, 0, 0);
+      StageArgs a = stage_args(s, g, 0, 0);
       a.time = nullptr;
-      a.time_offset = stime;
-      a.rk_a = a.rk_b = 0.0;
-      a.rhs_in_sel = i == 0 ? 0 : SET_STAGE + 1;
-      a.rhs_out_sel = i + 1;
-      for (const Group& grp : s->groups) SBP_TRY_HIP(launch_group(s, grp, grp.shape, grp.sched, MODE_RHS, a, st));
+      a.time_offset = i == 0 ? time : time + g * C[i - 1];
+      a.rk_a = i < 3 ? Anext[i] : 0.0;
+      a.rk_b = i == 3 ? 1.0 / 8.0 : 0.0;
+      a.swap = swap;
+      a.partials = s->d_tile_sums;
+      for (const Group& grp : s->groups) SBP_TRY_HIP(launch_group(s, grp, grp.shape, grp.sched, MODE_BS0 + i, a, st));
     }
-    SBP_TRY(combine(Bw, 4, g, -1));
-    SBP_TRY(combine(Bs, 4, g, SET_SECOND));
-    time += g;
+    SBP_TRY_HIP(launch_block_sums(s->d_blocks, nb, s->d_tile_sums, s->d_block_sums, st));
+    SBP_TRY_HIP(hipMemcpyAsync(s->h_block_sums, s->d_block_sums, sizeof(double) * 2 * nb, hipMemcpyDeviceToHost, st));
+    SBP_TRY_HIP(hipStreamSynchronize(st));
     double err = 0.0;
-    for (int bi = 0; bi < (int)s->blocks.size(); ++bi) {
-      long double sum;
-      SBP_TRY(h2_sum(s, bi, set_of(s->blocks[bi], SET_SECOND), nullptr, 0.0, sum));
+    for (int bi = 0; bi < nb; ++bi) {
+      const long double sum = (long double)s->h_block_sums[2 * bi] + (long double)s->h_block_sums[2 * bi + 1];
       err += std::sqrt((double)sum);
     }
+    time += g;
     if (err < maxerr) {
       time += g;
-      for (auto& h : s->blocks)
-        SBP_TRY_HIP(launch_copy_planes(h.d_buf[BUF_YA], h.ps, h.ld, h.d_buf[BUF_YN], h.ps, h.ld, 4, h.ny, h.nx, st));
+      swap ^= 1;
       g *= 1.05;
     } else {
       g *= 0.8;
     }
   }
+  if (swap)
+    for (auto& h : s->blocks)
+      SBP_TRY_HIP(launch_copy_planes(h.d_buf[BUF_ACC], h.ps, h.ld, h.d_buf[BUF_YN], h.ps, h.ld, 4, h.ny, h.nx, st));
   *guess_dt = g;
   SBP_TRY_HIP(hipStreamSynchronize(st));
   return check_err(s);

@@ -38,6 +38,7 @@ namespace sbp {
 template <class OX, class OY, int TX, int TY, int RX, int RY, int VAR = 0, int VBYTES = 8>
 struct StageGeom {
   static constexpr int NT = (TX / RX) * (TY / RY);
+  static_assert(NT % 64 == 0, "whole wavefronts per workgroup");
   // values per point held in LDS at once: 4 (two passes) or 8 (one pass)
   static constexpr int NV = VAR == 2 ? 8 : 4;
   static constexpr int HX = OX::H, HY = OY::H;
@@ -156,9 +157,9 @@ __global__ __launch_bounds__((TX / RX) * (TY / RY)) __attribute__((amdgpu_waves_
   tile_place(T.ty, B.nty, ny, TY, OY::M, y0, wy0, wy1);
   // stage input: y_n (FP64 planes), a point-major FP64 stage input, or in the
   // mixed form y_n plus the FP32 increment set (4 floats per point)
-  const bool mixs = MIX && mode >= MODE_S0;
+  const bool mixs = MIX && mode >= MODE_S0 && mode <= MODE_S3;
   const double* __restrict__ in = mixs ? yn_buf(B, A) : stage_input(A, B, mode);
-  const f4* __restrict__ din = (MIX && mode >= MODE_S1) ? delta_in(B, mode) : nullptr;
+  const f4* __restrict__ din = (MIX && mode >= MODE_S1 && mode <= MODE_S3) ? delta_in(B, mode) : nullptr;
   const double* __restrict__ met = B.metrics;
   // metric pairs: one 16-byte load per region point and phase
   const d2* __restrict__ mxi = reinterpret_cast<const d2*>(B.mpair);
@@ -491,8 +492,9 @@ __global__ __launch_bounds__((TX / RX) * (TY / RY)) __attribute__((amdgpu_waves_
   // Finish operands at the output points, all loaded before any store.
   // FP64 (see the finish), with q the stage input held from the η phase:
   //   S0 q = y_n;  S1 fa = y_n, q = Y2;  S2 fa = y_n, q = Y3;  S3 fa = W, q = Y4
+  //   BS0 q = y_n;  BS1 fa = y_n, q = Y2;  BS2, BS3 fa = y_n, fb = Y2, q = Y3 / the solution
   // mixed: S3 fd = Δ_1, Δ_2 (S0-S2 need no operand)
-  double fJ[RY][RX], fa[RY][RX][4];
+  double fJ[RY][RX], fa[RY][RX][4], fb[RY][RX][4];
   f4 fd[MIX ? RY : 1][MIX ? RX : 1][2];
   static_assert(OWNQ, "the stage combination takes the stage input from the η phase");
   auto load_finish = [&]() {
@@ -511,6 +513,13 @@ __global__ __launch_bounds__((TX / RX) * (TY / RY)) __attribute__((amdgpu_waves_
           if (mode == MODE_S3) {
 #pragma unroll
             for (int t = 0; t < 2; ++t) fd[MIX ? s : 0][MIX ? i : 0][t] = ldg_nt(delta_set(B, t), o);
+          }
+        } else if (mode >= MODE_BS0) {
+          if (mode >= MODE_BS1) {
+            const double* yn = A.swap ? B.buf[BUF_ACC] : B.buf[BUF_YN];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) fa[s][i][q] = ldg_nt(yn + q * ps, o);
+            if (mode >= MODE_BS2) load_pm_nt(B.buf[BUF_YA], o, fb[s][i]);
           }
         } else if (mode == MODE_S3) {
           load_pm_nt(B.buf[BUF_ACC], o, fa[s][i]);  // W, point-major
@@ -697,6 +706,7 @@ __global__ __launch_bounds__((TX / RX) * (TY / RY)) __attribute__((amdgpu_waves_
 
   // ------------------------------------------------------------ finish
   load_finish();
+  KSum esum = {0.0, 0.0};  // MODE_BS3: this thread's error terms
   // (b_s·dt)·k and (a_s·dt)·k with this block's step (integrate, lib.rs:188-211)
   const double dtb = A.dt * B.dt_scale;
   const double sacc = A.rk_b * dtb, snext = A.rk_a * dtb;
@@ -772,7 +782,38 @@ __global__ __launch_bounds__((TX / RX) * (TY / RY)) __attribute__((amdgpu_waves_
       const double* qs = ownq[OWNQ ? s : 0];  // the stage input here
       // Y2, Y3, Y4 and W point-major, y_{n+1} in planes
       const double* fs = fa[s][i];
-      if (mode == MODE_S0) {
+      if (mode >= MODE_BS0) {
+        // Embedded Bogacki-Shampine trial step (integrate/src/lib.rs:138-151,
+        // 176-251) with step g: Y2 = y + g/2 k0, Y3 = y + 3g/4 k1, the
+        // solution f = y + g (2/9 k0 + 1/3 k1 + 4/9 k2) and the embedded
+        // z = y + g (7/24 k0 + 1/4 k1 + 1/3 k2 + 1/8 k3). With the tableau's
+        // identities g k0 = 2 (Y2 - y), g k1 = 4/3 (Y3 - y):
+        //   f = y + 4/9 ((Y2 - y) + (Y3 - y)) + 4/9 g k2,
+        //   z - y = 1/4 (Y2 - y) + 3/4 (f - y) + g/8 k3,
+        // so BS3 needs y, Y2 and its own input f, and z is never stored: its
+        // H-norm distance from y (advance_adaptive's error, lib.rs:170-176)
+        // is summed per tile. (snext: g/2, 3g/4, 4g/9; sacc: g/8.)
+        const double* f2 = fb[s][i];
+        if (mode == MODE_BS0) {
+          store_pm_nt(B.buf[BUF_YA], o, qs[0] + snext * k[0], qs[1] + snext * k[1], qs[2] + snext * k[2],
+                      qs[3] + snext * k[3]);
+        } else if (mode == MODE_BS1) {
+          store_pm_nt(B.buf[BUF_YB], o, fs[0] + snext * k[0], fs[1] + snext * k[1], fs[2] + snext * k[2],
+                      fs[3] + snext * k[3]);
+        } else if (mode == MODE_BS2) {
+          double* yo = A.swap ? B.buf[BUF_YN] : B.buf[BUF_ACC];
+#pragma unroll
+          for (int q = 0; q < 4; ++q)
+            stg_nt(yo + q * ps, o, fs[q] + ((4.0 / 9.0) * ((f2[q] - fs[q]) + (qs[q] - fs[q])) + snext * k[q]));
+        } else {
+          const double w = hweight<OX, OY>(xb + i, nx, yb + s, ny);
+#pragma unroll
+          for (int q = 0; q < 4; ++q) {
+            const double d = 0.25 * (f2[q] - fs[q]) + 0.75 * (qs[q] - fs[q]) + sacc * k[q];
+            ksum_add(esum, d * d * w);
+          }
+        }
+      } else if (mode == MODE_S0) {
         store_pm_nt(B.buf[BUF_YA], o, qs[0] + snext * k[0], qs[1] + snext * k[1], qs[2] + snext * k[2],
                     qs[3] + snext * k[3]);
       } else if (mode == MODE_S1) {
@@ -788,6 +829,23 @@ __global__ __launch_bounds__((TX / RX) * (TY / RY)) __attribute__((amdgpu_waves_
       } else {
 #pragma unroll
         for (int q = 0; q < 4; ++q) stg_nt(B.buf[BUF_YN] + q * ps, o, (fa[s][i][q] + qs[q]) * (1.0 / 3.0) + sacc * k[q]);
+      }
+    }
+  }
+  if constexpr (!MIX) {
+    if (mode == MODE_BS3) {
+      // the tile's sum of the adaptive step's error terms, compensated
+      __shared__ KSum wsum[NT / 64];
+      esum = wave_ksum(esum);
+      if ((tid & 63) == 0) wsum[tid >> 6] = esum;
+      __syncthreads();
+      if (tid == 0) {
+        KSum t = wsum[0];
+#pragma unroll
+        for (int w = 1; w < NT / 64; ++w) ksum_merge(t, wsum[w]);
+        const int64_t slot = B.tile_off + (int64_t)T.ty * B.ntx + T.tx;
+        A.partials[2 * slot] = t.s;
+        A.partials[2 * slot + 1] = t.c;
       }
     }
   }

@@ -348,30 +348,49 @@ __device__ __forceinline__ void face_terms(double rho, double ru, double rv, dou
 // 1.15x larger 1000-step error against the FP64 form.)
 struct MixRef64 {
   double ec[4], fc[4];  // Cartesian fluxes E(q_C), F(q_C)
-  double aqc[4];        // α_C q_C
+  double aqc[4];        // α_C q_C (α_C with the phase's metric pair at the centre)
 };
 
-// Every thread of a tile evaluates the reference terms of the tile's centre
-// point the same way (one reciprocal of ρ, the refined estimates frcp and
-// fsqrt): any reference state the whole tile shares drops out of the
-// stencils, so only its closeness to the tile's states matters.
-__device__ __forceinline__ MixRef64 mix_ref64(const double q[4], double jx, double jy, double gamma) {
-  MixRef64 R;
+// Reference terms of one tile in the per-tile layout of StageArgs::mixref
+// (16 doubles): E(q_C) at [0, 4), F(q_C) at [4, 8), α_ξ,C q_C at [8, 12)
+// (α with the centre's J·∇ξ), α_η,C q_C at [12, 16) (J·∇η). The mixed form
+// evaluates them once per tile and RK4 step, from y_n at the tile's centre
+// (mix_ref_kernel): any state the whole tile shares drops out of the
+// stencils, so only its closeness to the tile's stage inputs matters, and
+// the stage input moves by O(dt) within a step.
+__device__ __forceinline__ void mix_ref_terms(const double q[4], double jxx, double jxy, double jex, double jey,
+                                              double gamma, double r[16]) {
   const double ir = frcp(q[0]);
   const double u = q[1] * ir, v = q[2] * ir;
   const double p = (gamma - 1.0) * (q[3] - 0.5 * (q[1] * u + q[2] * v));
-  R.ec[0] = q[1];
-  R.ec[1] = q[1] * u + p;
-  R.ec[2] = q[2] * u;
-  R.ec[3] = (q[3] + p) * u;
-  R.fc[0] = q[2];
-  R.fc[1] = q[1] * v;
-  R.fc[2] = q[2] * v + p;
-  R.fc[3] = (q[3] + p) * v;
-  const double uh = jx * u + jy * v;
-  const double al = fabs(uh) + fsqrt(fmax(gamma * p * ir * (jx * jx + jy * jy), 1e-300));
+  r[0] = q[1];
+  r[1] = q[1] * u + p;
+  r[2] = q[2] * u;
+  r[3] = (q[3] + p) * u;
+  r[4] = q[2];
+  r[5] = q[1] * v;
+  r[6] = q[2] * v + p;
+  r[7] = (q[3] + p) * v;
+  const double c2 = gamma * p * ir;
+  const double ax = fabs(jxx * u + jxy * v) + fsqrt(fmax(c2 * (jxx * jxx + jxy * jxy), 1e-300));
+  const double ae = fabs(jex * u + jey * v) + fsqrt(fmax(c2 * (jex * jex + jey * jey), 1e-300));
 #pragma unroll
-  for (int c = 0; c < 4; ++c) R.aqc[c] = al * q[c];
+  for (int c = 0; c < 4; ++c) {
+    r[8 + c] = ax * q[c];
+    r[12 + c] = ae * q[c];
+  }
+}
+
+// The reference terms of one phase (0: ξ, 1: η) from a tile's 16 doubles;
+// the address is the same for the whole workgroup, so these are scalar loads.
+__device__ __forceinline__ MixRef64 load_mix_ref(const double* __restrict__ rf, int phase) {
+  MixRef64 R;
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    R.ec[c] = rf[c];
+    R.fc[c] = rf[4 + c];
+    R.aqc[c] = rf[8 + 4 * phase + c];
+  }
   return R;
 }
 

@@ -68,7 +68,7 @@ struct VortexDev {
 struct BlockDev {
   int nx, ny, ld, op_xi, op_eta, upwind;
   int ntx, nty;  // tiles along ξ and η for the active tile shape
-  int64_t tile_off;  // slot of the block's first tile in per-tile arrays (StageArgs::partials)
+  int64_t tile_off;  // slot of the block's first tile in per-tile arrays (StageArgs::partials, mixref)
   int64_t ps;
   const double* metrics;
   // metric pairs as the stage kernel reads them: (J·ξx, J·ξy) of point o at
@@ -129,6 +129,10 @@ struct StageArgs {
   int sat_buf;  // 1: the edge tiles read the edge SATs from BlockDev::sat (edge-SAT kernel)
   double* partials;  // MODE_BS3: (sum, compensation) per tile, at BlockDev::tile_off + ty·ntx + tx
   int rhs_in_sel, rhs_out_sel;  // MODE_RHS buffers: 0 = rhs_in / rhs_out, s > 0 = rk[s - 1]
+  // mixed form: reference face terms of every tile (16 doubles at slot
+  // BlockDev::tile_off + ty·ntx + tx), written by mix_ref_kernel before the
+  // first stage of a step (and before a right-hand-side launch)
+  double* mixref;
 };
 
 }  // namespace sbp

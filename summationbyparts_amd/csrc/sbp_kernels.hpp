@@ -34,6 +34,9 @@ hipError_t launch_metrics_finish(const double* dx_dxi, const double* dx_deta, co
 // Characteristic SATs of the listed edges at stage `mode` into BlockDev::sat
 // (maxlen: the longest listed edge).
 hipError_t launch_edge_sat(const StageArgs& a, const SatSeg* segs, int nsegs, int maxlen, int mode, hipStream_t stream);
+// Mixed form: reference face terms of the `ntiles` tiles at a.tiles (tile
+// shape `shape`) from the state stage `mode` reads, into a.mixref.
+hipError_t launch_mix_ref(const StageArgs& a, int shape, int mode, int ntiles, hipStream_t stream);
 // Expressions of `prog` (a device copy) on a block at time t into 4 planes.
 hipError_t launch_eval_field(const expr::Program* prog, double t, const double* xc, const double* yc, double* out, int64_t ps,
                              int64_t ld, int ny, int nx, unsigned* err, hipStream_t stream);

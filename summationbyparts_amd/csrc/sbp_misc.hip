@@ -221,6 +221,41 @@ hipError_t launch_edge_sat(const StageArgs& a, const SatSeg* segs, int nsegs, in
   return hipGetLastError();
 }
 
+// Mixed form: the reference face terms of each listed tile of a TX x TY
+// tiling (mix_ref_terms) from the state stage `mode` reads at the tile's
+// centre point, one thread per tile, into StageArgs::mixref. The stage
+// kernels of the step (or of the right-hand-side launch) read them instead
+// of evaluating them in every thread of the tile.
+__global__ __launch_bounds__(256) void mix_ref_kernel(const StageArgs A, const int mode, const int TX, const int TY,
+                                                      const int ntiles) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= ntiles) return;
+  const TileDev T = A.tiles[i];
+  const BlockDev& B = A.blocks[T.block];
+  // tile origin as the stage kernel places it (tile_place; the closure
+  // width only moves the written range, not the origin)
+  int x0, y0, w0, w1;
+  tile_place(T.tx, B.ntx, B.nx, TX, 0, x0, w0, w1);
+  tile_place(T.ty, B.nty, B.ny, TY, 0, y0, w0, w1);
+  const uint32_t oc = (uint32_t)(min(y0 + TY / 2, B.ny - 1) * B.ld + min(x0 + TX / 2, B.nx - 1));
+  double q[4], r[16];
+  load_stage_state(A, B, mode, oc, q);
+  const d2* mp = reinterpret_cast<const d2*>(B.mpair);
+  const d2 mx = mp[oc], me = mp[B.ps + oc];
+  mix_ref_terms(q, mx.x, mx.y, me.x, me.y, A.gamma, r);
+  double* out = A.mixref + 16 * (B.tile_off + (int64_t)T.ty * B.ntx + T.tx);
+#pragma unroll
+  for (int c = 0; c < 16; ++c) out[c] = r[c];
+}
+
+hipError_t launch_mix_ref(const StageArgs& a, int shape, int mode, int ntiles, hipStream_t stream) {
+  if (ntiles <= 0) return hipSuccess;
+  if (shape < 0 || shape >= kNumTileShapes || !a.mixref) return hipErrorInvalidValue;
+  const TileShape& ts = kTileShapes[shape];
+  mix_ref_kernel<<<(ntiles + 255) / 256, 256, 0, stream>>>(a, mode, ts.tx, ts.ty, ntiles);
+  return hipGetLastError();
+}
+
 
 // ---------------------------------------------------------------------------
 // Vortex initial state (Field::vortex, lib.rs:319-328).

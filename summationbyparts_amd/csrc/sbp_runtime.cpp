@@ -271,6 +271,10 @@ struct sbp_system_s {
   double* d_block_sums = nullptr;
   double* h_block_sums = nullptr;
   int partials_cap = 0;
+  // mixed form: the tiles' reference face terms (StageArgs::mixref, 16
+  // doubles per tile slot)
+  double* d_mixref = nullptr;
+  int64_t mixref_cap = 0;
   int rhs_kind = SBP_RHS_AUTO;
   int precision = SBP_FP64;
   // mixed form: 1 while y_n lives in the accumulator buffer (odd steps)
@@ -424,6 +428,7 @@ StageArgs stage_args(const sbp_system_s* s, double dt, int stage, int swap) {
   a.time_offset = stage == 0 ? 0.0 : dt * C[stage - 1];
   a.mixed = s->precision == SBP_MIXED;
   a.swap = a.mixed ? swap : 0;
+  a.mixref = s->d_mixref;
   return a;
 }
 
@@ -450,6 +455,13 @@ hipError_t launch_group(const sbp_system_s* s, const Group& g, int shape, int sc
                         hipStream_t st) {
   const sbp_context_s* c = s->ctx;
   hipError_t e;
+  // mixed form: the tiles' reference face terms, once per step (from y_n,
+  // ahead of the first stage) or per right-hand-side launch
+  if (a.mixed && (mode == MODE_S0 || mode == MODE_RHS)) {
+    StageArgs ar = a;
+    ar.tiles = s->d_tiles + g.etile_offset;
+    if ((e = launch_mix_ref(ar, shape, mode, g.entiles + g.ntiles, st)) != hipSuccess) return e;
+  }
   if (sched != 0) {
     if ((e = launch_eval_edges(a, s->d_evsegs + g.eval_off, g.neval, g.eval_maxlen, st)) != hipSuccess) return e;
     StageArgs am = a;
@@ -554,6 +566,13 @@ int rebuild(sbp_system_s* s) {
     for (auto& d : devs) {
       d.tile_off = s->total_tiles;
       s->total_tiles += (int64_t)d.ntx * d.nty;
+    }
+    if (s->precision == SBP_MIXED && s->mixref_cap < s->total_tiles) {
+      if (s->d_mixref) hipFree(s->d_mixref);
+      s->d_mixref = nullptr;
+      s->mixref_cap = 0;
+      SBP_TRY_HIP(hipMalloc(&s->d_mixref, sizeof(double) * 16 * s->total_tiles));
+      s->mixref_cap = s->total_tiles;
     }
     if (s->d_blocks) hipFree(s->d_blocks);
     if (s->d_tiles) hipFree(s->d_tiles);
@@ -1458,6 +1477,7 @@ int sbp_system_destroy(sbp_system s) {
   if (s->d_time) hipFree(s->d_time);
   if (s->d_err) hipFree(s->d_err);
   if (s->d_partials) hipFree(s->d_partials);
+  if (s->d_mixref) hipFree(s->d_mixref);
   if (s->d_tile_sums) hipFree(s->d_tile_sums);
   if (s->d_block_sums) hipFree(s->d_block_sums);
   if (s->h_block_sums) hipHostFree(s->h_block_sums);

@@ -18,7 +18,8 @@
 //     (J·ηx, J·ηy), and η-stencils.
 //   finish: k = (idx_ξ·Sξ + idx_η·Sη)/J, RK4 combination, store.
 // Mixed form (MIX): the face terms are FP64, rounded to FP32 as deviations
-// from those of a per-tile reference state (face_terms_mix64), LDS and
+// from those of a per-tile reference state (face_terms_mix64; the reference
+// terms come from mix_ref_kernel once per step), LDS and
 // stencil sums are FP32, y_n and
 // the RK4 sum stay FP64; each stage input is y_n + Δ with the FP32 increment
 // Δ = a·dt·k stored by the previous stage, and the last stage sums y_n and
@@ -293,8 +294,9 @@ __global__ __launch_bounds__((TX / RX) * (TY / RY)) __attribute__((amdgpu_waves_
       }
     }
   };
-  // reference state of the mixed form: the tile's centre point
-  const uint32_t oc = (uint32_t)(min(y0 + TY / 2, ny - 1) * ld + min(x0 + TX / 2, nx - 1));
+  // mixed form: the tile's reference terms (mix_ref_kernel, this step's y_n
+  // at the tile's centre)
+  const double* __restrict__ mref = MIX ? A.mixref + 16 * (B.tile_off + (int64_t)T.ty * B.ntx + T.tx) : nullptr;
 
   CT acc[RY][RX][4];
   // metric products at the own output points (OWNQ), all four loaded in the
@@ -318,11 +320,7 @@ __global__ __launch_bounds__((TX / RX) * (TY / RY)) __attribute__((amdgpu_waves_
     const int xw = rx1 - rx0;
     const bool interior = x0 >= OX::M && x0 + TX <= nx - OX::M;
     MixRef64 R;
-    if constexpr (MIX) {
-      double qc[4];
-      load_q(oc, qc);
-      R = mix_ref64(qc, ldg(met + ps, oc), ldg(met + 2 * ps, oc), gamma);
-    }
+    if constexpr (MIX) R = load_mix_ref(mref, 0);
     // region point (tile row r, column c from rx0) of this thread's
     // iteration it: own output points first (OWNQ), then the halo columns
     auto xi_pt = [&](int it, int& r, int& c) -> bool {
@@ -539,11 +537,7 @@ __global__ __launch_bounds__((TX / RX) * (TY / RY)) __attribute__((amdgpu_waves_
     const int xend = min(x0 + TX, nx);
     const bool interior = y0 >= OY::M && y0 + TY <= ny - OY::M;
     MixRef64 R;
-    if constexpr (MIX) {
-      double qc[4];
-      load_q(oc, qc);
-      R = mix_ref64(qc, ldg(met + 3 * ps, oc), ldg(met + 4 * ps, oc), gamma);
-    }
+    if constexpr (MIX) R = load_mix_ref(mref, 1);
     // region point (row r from ry0, column c from x0) of this thread's
     // iteration it: own output points first (OWNQ), then the halo rows
     const int yendr = min(y0 + TY, ny);

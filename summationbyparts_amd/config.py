@@ -301,12 +301,37 @@ def load(text):
     return Configuration.loads(text).into_runtime()
 
 
-def run(rt, sys_, steps=None):
-    """main.rs:96-147 without output: dt = max_dt, ntime = round(T/dt) steps
-    (or `steps`); returns (dt, steps taken)."""
+def run(rt, sys_, steps=None, output=None, number_of_outputs=None):
+    """main.rs:96-147: dt = max_dt, ntime = round(T/dt) steps (or `steps`);
+    returns (dt, steps taken).
+
+    output: a path for the HDF5 file of the run (hdf5.Output, the layout of
+    system.rs:28-117), written as main.rs does: the state at step 0, at the
+    end of each stretch of steps_between_outputs = max(number_of_outputs /
+    ntime, 1) steps (ntime without number_of_outputs), and at the end. The
+    reference's loop takes the next output step as max(itime +
+    steps_between_outputs, ntime) (main.rs:123), so its outputs are step 0
+    and step ntime, as here."""
     dt = sys_.max_dt()
     sys_.set_dt(dt)
     # f64::round rounds half away from zero
     ntime = int(math.floor(rt.integration_time / dt + 0.5)) if steps is None else steps
-    sys_.advance(ntime)
+    if output is None:
+        sys_.advance(ntime)
+        return dt, ntime
+    from .hdf5 import Output
+    if number_of_outputs is not None:
+        between = max(number_of_outputs // ntime, 1)  # ntime = 0 fails as n / ntime does in the reference
+    else:
+        between = ntime
+    with Output(output, rt.names, rt.grids) as out:
+        out.output(0, sys_.fields())
+        itime = 0
+        while itime < ntime:
+            nexttime = max(itime + between, ntime)
+            sys_.advance(nexttime - itime)
+            itime = nexttime
+            if itime != ntime:
+                out.output(itime, sys_.fields())
+        out.output(ntime, sys_.fields())
     return dt, ntime

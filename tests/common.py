@@ -1,5 +1,9 @@
 """Shared inputs of the tests: grids, vortex parameters and seeded noise of
 the benchmark workloads, and comparison helpers."""
+import json
+import os
+import subprocess
+
 import numpy as np
 
 GAMMA = 1.4
@@ -127,3 +131,55 @@ def c5_batch(seed=5, count=256, lo=48, hi=512):
         out.append(dict(nx=nx, ny=ny, mach=mach, eps=eps, amp=amp, rstar=1.0,
                         dt=0.2 * min(1.0 / (nx - 1), 1.0 / (ny - 1))))
     return out
+
+
+# ---------------------------------------------------------------------------
+# HDF5 files read back by the HDF5 library: the image's conda Python carries
+# h5py on libhdf5 1.10 (the system Python has no HDF5 module)
+H5PY = "/opt/conda/bin/python3.9"
+
+
+def have_h5py():
+    if not os.path.exists(H5PY):
+        return False
+    r = subprocess.run([H5PY, "-c", "import h5py"], capture_output=True)
+    return r.returncode == 0
+
+
+# reads every dataset, link and storage property of a file with h5py
+_READER = r"""
+import h5py, json, sys, numpy as np
+out = {"datasets": {}, "links": {}, "groups": []}
+with h5py.File(sys.argv[1], "r") as f:
+    def visit(g, prefix):
+        for name in g:
+            path = prefix + name
+            link = g.get(name, getlink=True)
+            if isinstance(link, h5py.SoftLink):
+                out["links"][path] = link.path
+                continue
+            obj = g[name]
+            if isinstance(obj, h5py.Group):
+                out["groups"].append(path)
+                visit(obj, path + "/")
+            else:
+                np.save(sys.argv[2] + "/" + path.replace("/", "__") + ".npy", obj[()])
+                out["datasets"][path] = {
+                    "dtype": str(obj.dtype), "shape": list(obj.shape),
+                    "maxshape": [m for m in obj.maxshape], "chunks": list(obj.chunks) if obj.chunks else None,
+                    "compression": obj.compression, "compression_opts": obj.compression_opts,
+                    "shuffle": bool(obj.shuffle)}
+    visit(f, "")
+    out["grid_t"] = {g: [int(v) for v in f[g]["t"][()]] for g in out["groups"]}
+print(json.dumps(out))
+"""
+
+
+def read_h5(path, tmp):
+    d = tmp / "arrays"
+    d.mkdir(exist_ok=True)
+    r = subprocess.run([H5PY, "-c", _READER, str(path), str(d)], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    info = json.loads(r.stdout)
+    arrays = {p: np.load(d / (p.replace("/", "__") + ".npy")) for p in info["datasets"]}
+    return info, arrays

@@ -169,6 +169,31 @@ def test_full_run_and_error(gpu, name):
     assert abs(e - e_ref) <= 1e-8 * e_ref
 
 
+def test_run_with_hdf5_output(gpu, tmp_path):
+    """main.rs with an output file: the state at step 0 and at step ntime of
+    every grid in the reference's layout (system.rs:28-117, 591-616), read
+    back by libhdf5 (h5py) where the image has it."""
+    from common import have_h5py, read_h5
+    from summationbyparts_amd import config
+    rt = config.load(_text("quad.json"))
+    sys_ = rt.system()
+    first = sys_.fields()
+    path = tmp_path / "quad.h5"
+    dt, nsteps = config.run(rt, sys_, steps=25, output=path)
+    last = sys_.fields()
+    assert nsteps == 25 and os.path.getsize(path) > 0
+    if not have_h5py():
+        pytest.skip("no Python with h5py (libhdf5) in this image to read the file back")
+    info, arr = read_h5(path, tmp_path)
+    assert arr["t"].tolist() == [0, 25]
+    assert sorted(info["groups"]) == sorted(rt.names)
+    for i, name in enumerate(rt.names):
+        assert np.array_equal(arr[f"{name}/x"], rt.grids[i][0]) and np.array_equal(arr[f"{name}/y"], rt.grids[i][1])
+        for c, var in enumerate(("rho", "rhou", "rhov", "e")):
+            assert np.array_equal(arr[f"{name}/{var}"][0], first[i][c])
+            assert np.array_equal(arr[f"{name}/{var}"][1], last[i][c])
+
+
 @pytest.mark.parametrize("name", ["interpolation.json", "expressions_vortex.json", "tsection.json"])
 @pytest.mark.parametrize("schedule", [0, 1, 2])
 def test_example_configuration_each_schedule(gpu, name, schedule):

@@ -126,10 +126,16 @@ __device__ __forceinline__ void col_window(const T* p, int stride, T out[RS][RX]
 // Threads own RX adjacent ξ-columns and RY consecutive η-rows of the tile.
 // The edge instantiation asks for 4 waves per SIMD (without the hint the
 // upwind9 edge tiles land 2 VGPRs above that budget, at 130).
-template <class OX, class OY, bool UP, int TX, int TY, int RX, int RY, int VAR, bool MIX = false, bool EDGE = false>
+// ADAPT: the instantiation of the embedded Bogacki-Shampine modes (BS0-BS3,
+// FP64 only), whose finish holds two operand sets; the RK4 and
+// right-hand-side modes run in the other one, which needs fewer registers
+// (the FP64 interior kernel 100 -> 92 VGPRs: 5 instead of 4 waves per SIMD).
+template <class OX, class OY, bool UP, int TX, int TY, int RX, int RY, int VAR, bool MIX = false, bool EDGE = false,
+          bool ADAPT = false>
 __global__ __launch_bounds__((TX / RX) * (TY / RY)) __attribute__((amdgpu_waves_per_eu(EDGE ? 4 : 1, 8))) void stage_kernel(
     const StageArgs A, const int mode) {
   static_assert(!MIX || VAR == 2, "the mixed form keeps flux and α·q together in LDS");
+  static_assert(!(MIX && ADAPT), "the adaptive modes are FP64");
   using CT = std::conditional_t<MIX, float, double>;  // face terms, LDS, stencil sums
   using G = StageGeom<OX, OY, TX, TY, RX, RY, VAR, sizeof(CT)>;
   constexpr int NT = G::NT, HX = G::HX, HY = G::HY, WX = G::WX, NP = G::NP;
@@ -142,10 +148,14 @@ __global__ __launch_bounds__((TX / RX) * (TY / RY)) __attribute__((amdgpu_waves_
   constexpr bool OWNQ = RX == 1 && RY <= LCH;
   constexpr int ITYE = OWNQ ? RY + (2 * G::HY * TX + NT - 1) / NT : G::ITY;
   constexpr int ITXE = OWNQ ? RY + (2 * G::HX * TY + NT - 1) / NT : G::ITX;
-  // load batches per phase: all region points of a thread at once when they
-  // are at most four (one memory round trip per phase), else LCH at a time
+  // load batches per phase: the ξ phase loads all region points of a thread
+  // at once when they are at most four (one memory round trip), else LCH at
+  // a time; the η phase at most three at once, else two at a time (its
+  // batch of four held 48 VGPRs, the kernel's peak: 32x16 tiles went from
+  // 100 to 94 VGPRs in FP64 and 100 to 84 in the mixed form, 5 waves per
+  // SIMD instead of 4, +2.5 % and +4 %)
   constexpr int LBX = ITXE <= 4 ? ITXE : LCH;
-  constexpr int LBY = ITYE <= 4 ? ITYE : LCH;
+  constexpr int LBY = ITYE <= 3 ? ITYE : 2;
   __shared__ __align__(16) CT lds[NV * NP];
 
   const TileDev T = A.tiles[blockIdx.x];
@@ -512,7 +522,7 @@ __global__ __launch_bounds__((TX / RX) * (TY / RY)) __attribute__((amdgpu_waves_
 #pragma unroll
             for (int t = 0; t < 2; ++t) fd[MIX ? s : 0][MIX ? i : 0][t] = ldg_nt(delta_set(B, t), o);
           }
-        } else if (mode >= MODE_BS0) {
+        } else if (ADAPT) {
           if (mode >= MODE_BS1) {
             const double* yn = A.swap ? B.buf[BUF_ACC] : B.buf[BUF_YN];
 #pragma unroll
@@ -776,7 +786,7 @@ __global__ __launch_bounds__((TX / RX) * (TY / RY)) __attribute__((amdgpu_waves_
       const double* qs = ownq[OWNQ ? s : 0];  // the stage input here
       // Y2, Y3, Y4 and W point-major, y_{n+1} in planes
       const double* fs = fa[s][i];
-      if (mode >= MODE_BS0) {
+      if constexpr (ADAPT) {
         // Embedded Bogacki-Shampine trial step (integrate/src/lib.rs:138-151,
         // 176-251) with step g: Y2 = y + g/2 k0, Y3 = y + 3g/4 k1, the
         // solution f = y + g (2/9 k0 + 1/3 k1 + 4/9 k2) and the embedded
@@ -826,7 +836,7 @@ __global__ __launch_bounds__((TX / RX) * (TY / RY)) __attribute__((amdgpu_waves_
       }
     }
   }
-  if constexpr (!MIX) {
+  if constexpr (ADAPT) {
     if (mode == MODE_BS3) {
       // the tile's sum of the adaptive step's error terms, compensated
       __shared__ KSum wsum[NT / 64];

@@ -20,10 +20,19 @@ namespace sbp {
 // Kernel configurations (tile TX x TY, RY points per thread, caching
 // variant); the list matches kTileShapes in sbp_misc.hip.
 // The mixed form always keeps flux and α·q together in LDS (VAR 2).
+// The embedded Bogacki-Shampine modes (FP64) run in their own instantiation.
 template <class OX, class OY, bool UP, bool MIX, bool EDGE, int TX, int TY, int RX, int RY, int VAR>
 static void launch_one(int mode, const StageArgs& args, int ntiles, hipStream_t stream) {
-  stage_kernel<OX, OY, UP, TX, TY, RX, RY, (MIX ? 2 : VAR), MIX, EDGE>
-      <<<ntiles, (TX / RX) * (TY / RY), 0, stream>>>(args, mode);
+  if constexpr (MIX) {
+    stage_kernel<OX, OY, UP, TX, TY, RX, RY, 2, true, EDGE, false>
+        <<<ntiles, (TX / RX) * (TY / RY), 0, stream>>>(args, mode);
+  } else if (mode >= MODE_BS0) {
+    stage_kernel<OX, OY, UP, TX, TY, RX, RY, VAR, false, EDGE, true>
+        <<<ntiles, (TX / RX) * (TY / RY), 0, stream>>>(args, mode);
+  } else {
+    stage_kernel<OX, OY, UP, TX, TY, RX, RY, VAR, false, EDGE, false>
+        <<<ntiles, (TX / RX) * (TY / RY), 0, stream>>>(args, mode);
+  }
 }
 
 // Every configuration for equal operator pairs; pairs of different ξ and η

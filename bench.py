@@ -88,7 +88,17 @@ def _counter_rows(path, counter):
 def _step_kernel(name):
     """Dispatches that belong to a stage launch: the stage kernel (interior
     and edge-tile instantiations), the per-stage edge SATs and Eval edges."""
-    return any(k in name for k in ("stage_kernel", "edge_sat_kernel", "eval_edges_kernel"))
+    return any(k in name for k in ("stage_kernel", "edge_sat_kernel", "eval_edges_kernel", "mix_ref_kernel"))
+
+
+def _is_interior_dispatch(name):
+    """A stage-kernel dispatch of the interior instantiation: the EDGE template
+    argument (the tenth: OX, OY, UP, TX, TY, RX, RY, VAR, MIX, EDGE, ADAPT)
+    is false."""
+    if "stage_kernel<" not in name:
+        return False
+    args = name.split("(")[0].split("<", 1)[1].rsplit(">", 1)[0].split(", ")
+    return len(args) >= 10 and args[9] == "false"
 
 
 def summarize_counters(fetch_csv, write_csv, copy_fetch_csv, copy_write_csv, workload, tile_index, precision, points):
@@ -109,8 +119,7 @@ def summarize_counters(fetch_csv, write_csv, copy_fetch_csv, copy_write_csv, wor
 
     def per_launch(path, counter, scale):
         rows = [(k, v * 1024.0 * scale) for _, k, v in _counter_rows(path, counter) if _step_kernel(k)]
-        interior = [i for i, (k, _) in enumerate(rows)
-                    if "stage_kernel" in k and k.rstrip(")").split("(")[0].endswith("false>")]
+        interior = [i for i, (k, _) in enumerate(rows) if _is_interior_dispatch(k)]
         if len(interior) < 8 or len(interior) % 4:
             raise SystemExit(f"{path}: {len(interior)} interior stage dispatches (want whole steps, at least 2)")
         kernel = rows[interior[0]][0]

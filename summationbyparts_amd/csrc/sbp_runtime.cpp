@@ -488,18 +488,21 @@ hipError_t launch_group(const sbp_system_s* s, const Group& g, int shape, int sc
 }
 
 // Autotuner timing of one group at one shape and schedule: a warm-up
-// stage, then three rounds of the stage-1..3 kernels (modes S0, S1, S2 read
-// y_n and write only the accumulator and the stage inputs, so the state is
-// untouched), between two events.
+// stage, then three rounds of the stage kernels that leave the state alone,
+// between two events: modes S0, S1, S2 read y_n and write only stage inputs
+// and increments; the mixed form's S3 too (it writes y_{n+1} into the other
+// FP64 buffer, y_n's until the host swaps the roles), so its heaviest stage
+// is timed with the others. (The FP64 S3 writes y_{n+1} over y_n.)
 hipError_t time_group(const sbp_system_s* s, const Group& g, int shape, int sched, hipEvent_t e0, hipEvent_t e1,
                       float& ms) {
   hipStream_t st = s->ctx->stream;
   const StageArgs a = stage_args(s, 0.0, 0, 0);
+  const int last = s->precision == SBP_MIXED ? MODE_S3 : MODE_S2;
   hipError_t e = launch_group(s, g, shape, sched, MODE_S0, a, st);
   if (e != hipSuccess) return e;
   if ((e = hipEventRecord(e0, st)) != hipSuccess) return e;
   for (int r = 0; r < 3; ++r)
-    for (int m = MODE_S0; m <= MODE_S2; ++m)
+    for (int m = MODE_S0; m <= last; ++m)
       if ((e = launch_group(s, g, shape, sched, m, a, st)) != hipSuccess) return e;
   if ((e = hipEventRecord(e1, st)) != hipSuccess) return e;
   if ((e = hipEventSynchronize(e1)) != hipSuccess) return e;

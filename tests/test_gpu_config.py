@@ -192,6 +192,13 @@ def test_run_with_hdf5_output(gpu, tmp_path):
         for c, var in enumerate(("rho", "rhou", "rhov", "e")):
             assert np.array_equal(arr[f"{name}/{var}"][0], first[i][c])
             assert np.array_equal(arr[f"{name}/{var}"][1], last[i][c])
+    # with number_of_outputs the reference's loop still jumps to ntime
+    # (next output step max(itime + steps_between_outputs, ntime), main.rs:123)
+    sys2 = rt.system()
+    path2 = tmp_path / "quad_n.h5"
+    config.run(rt, sys2, steps=10, output=path2, number_of_outputs=5)
+    info2, arr2 = read_h5(path2, tmp_path)
+    assert arr2["t"].tolist() == [0, 10]
 
 
 @pytest.mark.parametrize("name", ["interpolation.json", "expressions_vortex.json", "tsection.json"])

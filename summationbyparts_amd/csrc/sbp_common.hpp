@@ -196,17 +196,38 @@ __device__ __forceinline__ void row_sum4(const T* line, int qs, int stride, int 
 // coefficients read from a copy in LDS (`tab`, M x N row-major): a closure
 // row's coefficient row differs from lane to lane, and LDS serves such reads
 // at a fraction of the constant tables' latency. Interior rows as row_sum4.
+// A wavefront holding both closure and interior rows takes the closure path
+// for all of them, interior rows with the interior diagonal padded by N - D
+// zeros on each side (`diagp`, 2N - D entries in LDS) over an N-wide window
+// inside the loaded region [r0, ...): the same taps and products, without
+// running both paths in one wavefront.
 template <class B, typename T>
 __device__ __forceinline__ void row_sum4_tab(const T* line, int qs, int stride, int i, int n, int r0, const double* tab,
-                                             T out[4]) {
-  if (i < B::M || i >= n - B::M) {
-    const bool st = i < B::M;
-    // end rows: start row M-1-r read backwards, with the kind's sign
-    const int r = st ? i : i - (n - B::M);
-    const double* row = st ? tab + i * B::N : tab + (B::M - 1 - r) * B::N + (B::N - 1);
-    const int dk = st ? 1 : -1;
-    const T sg = st ? T(1) : T(B::end_sign);
-    const int b0 = st ? -r0 : n - B::N - r0;
+                                             const double* diagp, T out[4]) {
+  constexpr int PAD = B::N - B::D;
+  const bool clos = i < B::M || i >= n - B::M;
+  if (__any(clos)) {
+    const double* row;
+    int dk, b0;
+    T sg;
+    if (clos) {
+      const bool st = i < B::M;
+      // end rows: start row M-1-r read backwards, with the kind's sign
+      const int r = st ? i : i - (n - B::M);
+      row = st ? tab + i * B::N : tab + (B::M - 1 - r) * B::N + (B::N - 1);
+      dk = st ? 1 : -1;
+      sg = st ? T(1) : T(B::end_sign);
+      b0 = st ? -r0 : n - B::N - r0;
+    } else {
+      // window [w, w + N) holds the stencil [i - H, i + H]; it starts PAD
+      // before the stencil unless that leaves the region (the region's end
+      // is at least H + 1 past i, and the region is at least N wide)
+      const int w = max(i - B::H - PAD, r0);
+      row = diagp + (w - (i - B::H - PAD));
+      dk = 1;
+      sg = T(1);
+      b0 = w - r0;
+    }
     T s[4] = {T(0), T(0), T(0), T(0)};
 #pragma unroll
     for (int k = 0; k < B::N; ++k) {

@@ -258,8 +258,13 @@ __global__ __launch_bounds__((TX / RX) * (TY / RY)) __attribute__((amdgpu_waves_
   // operator's, then the η operator's unless it is the same) copied into LDS
   // by the tiles that write closure rows: their stencils read a coefficient
   // row per lane (row_sum4_tab).
+  // Per operator: the start blocks (DIFF, then DISS), then the interior
+  // diagonals padded by N - D zeros on each side (DIFF, then DISS), for the
+  // wavefronts that hold closure and interior rows together.
   constexpr bool SAMEOP = std::is_same<OX, OY>::value;
-  constexpr int TABX = (UP ? 2 : 1) * OX::M * OX::N, TABY = (UP ? 2 : 1) * OY::M * OY::N;
+  constexpr int NK = UP ? 2 : 1;
+  constexpr int LPX = 2 * OX::N - OX::D, LPY = 2 * OY::N - OY::D;
+  constexpr int TABX = NK * (OX::M * OX::N + LPX), TABY = NK * (OY::M * OY::N + LPY);
   constexpr int NTAB = EDGE ? (SAMEOP ? TABX : TABX + TABY) : 1;
   __shared__ double ctab[NTAB];
   const bool xclos = !(x0 >= OX::M && x0 + TX <= nx - OX::M);
@@ -268,14 +273,27 @@ __global__ __launch_bounds__((TX / RX) * (TY / RY)) __attribute__((amdgpu_waves_
     if (xclos || yclos) {
       auto fill = [&](auto op, int base) {
         using OP = decltype(op);
-        constexpr int MN = OP::M * OP::N;
-        for (int e = tid; e < (UP ? 2 : 1) * MN; e += NT) {
-          const int kind = e / MN, r = (e % MN) / OP::N, k = e % OP::N;
-          if constexpr (UP) {
-            ctab[base + e] = kind == 0 ? OP::diff_start[r][k] : OP::diss_start[r][k];
+        constexpr int MN = OP::M * OP::N, LP = 2 * OP::N - OP::D, PAD = OP::N - OP::D;
+        for (int e = tid; e < NK * (MN + LP); e += NT) {
+          double v;
+          if (e < NK * MN) {
+            const int kind = e / MN, r = (e % MN) / OP::N, k = e % OP::N;
+            if constexpr (UP) {
+              v = kind == 0 ? OP::diff_start[r][k] : OP::diss_start[r][k];
+            } else {
+              v = OP::diff_start[r][k];
+            }
           } else {
-            ctab[base + e] = OP::diff_start[r][k];
+            const int f = e - NK * MN, kind = f / LP, d = f % LP - PAD;
+            if (d < 0 || d >= OP::D) {
+              v = 0.0;
+            } else if constexpr (UP) {
+              v = kind == 0 ? OP::diff_diag[d] : OP::diss_diag[d];
+            } else {
+              v = OP::diff_diag[d];
+            }
           }
+          ctab[base + e] = v;
         }
       };
       fill(OX{}, 0);
@@ -440,7 +458,8 @@ __global__ __launch_bounds__((TX / RX) * (TY / RY)) __attribute__((amdgpu_waves_
             for (int i = 0; i < RX; ++i) {
               CT o[4] = {CT(0), CT(0), CT(0), CT(0)};
               if (xin[i])
-                row_sum4_tab<BB>(line + slot * NP, NP, 1, xb + i, nx, rx0, tabx + BB::table * OX::M * OX::N, o);
+                row_sum4_tab<BB>(line + slot * NP, NP, 1, xb + i, nx, rx0, tabx + BB::table * OX::M * OX::N,
+                                 tabx + NK * OX::M * OX::N + BB::table * LPX, o);
 #pragma unroll
               for (int q = 0; q < 4; ++q) acc[s][i][q] = first ? sg * o[q] : fma_t(sg, o[q], acc[s][i][q]);
             }
@@ -653,7 +672,7 @@ __global__ __launch_bounds__((TX / RX) * (TY / RY)) __attribute__((amdgpu_waves_
               CT o[4] = {CT(0), CT(0), CT(0), CT(0)};
               if (yin[s] && xin[i])
                 row_sum4_tab<BB>(lds + slot * NP + tc * RX + i, NP, TX, yb + s, ny, ry0,
-                                 taby + BB::table * OY::M * OY::N, o);
+                                 taby + BB::table * OY::M * OY::N, taby + NK * OY::M * OY::N + BB::table * LPY, o);
 #pragma unroll
               for (int q = 0; q < 4; ++q) acc[s][i][q] = fma_t(sg, o[q], acc[s][i][q]);
             }

@@ -15,22 +15,147 @@ const TileShape kTileShapes[kNumTileShapes] = {
 
 // ---------------------------------------------------------------------------
 // Operator application along one axis (SbpOperator2d::diffxi/diffeta,
-// UpwindOperator2d::dissxi/disseta; algos.rs:101-187 and :371-519). One
-// thread per output point; along ξ the taps are neighbours in the row, along
-// η they are the rows above and below (coalesced across the wavefront).
-template <class OP, int KIND, int AXIS>
-__global__ __launch_bounds__(256) void apply_op_kernel(const double* __restrict__ prev, int64_t ldp, double* __restrict__ fut,
+// UpwindOperator2d::dissxi/disseta; algos.rs:101-187 and :371-519). Each
+// thread computes kApplyR consecutive outputs along the axis from one window
+// of kApplyR + D - 1 inputs (closure rows, the first and last M, take the
+// start or end block per output). Along ξ a workgroup first stages its four
+// rows (256 outputs each, with N columns of margin on both sides, enough for
+// any closure row) and the start block in LDS, and its outputs in LDS for
+// coalesced stores; along η
+// the window is a column, its loads coalesced across the wavefront's
+// columns.
+constexpr int kApplyR = 4;
+
+template <class B>
+__device__ __forceinline__ void window_sums(const double* w, int stride, double out[kApplyR]) {
+#pragma unroll
+  for (int r = 0; r < kApplyR; ++r) out[r] = 0.0;
+#pragma unroll
+  for (int k = 0; k < kApplyR + B::D - 1; ++k) {
+    const double v = w[k * stride];
+#pragma unroll
+    for (int r = 0; r < kApplyR; ++r) {
+      const int d = k - r;
+      if (d >= 0 && d < B::D && B::diag(d) != 0.0) out[r] += B::diag(d) * v;
+    }
+  }
+}
+
+// LDS index of row element k along ξ: one pad double after every four, so
+// that lanes four elements apart (each thread's window starts four columns
+// after its neighbour's) hit different banks
+__device__ __forceinline__ constexpr int xi_pad(int k) { return k + (k >> 2); }
+
+template <class OP, int KIND>
+__global__ __launch_bounds__(256) void apply_xi_kernel(const double* __restrict__ prev, int64_t ldp, double* __restrict__ fut,
                                                        int64_t ldf, int ny, int nx, double idx) {
   using B = Blk<OP, KIND>;
+  // margin HM = N columns on both sides: an end-closure row reads up to N - 1
+  // columns to its left
+  constexpr int TW = 64 * kApplyR, HM = B::N, LW = TW + 2 * HM, C = HM - B::H;
+  __shared__ double rows[4][xi_pad(LW) + 1];
+  __shared__ double outs[4][xi_pad(TW)];
+  __shared__ double tab[B::M * B::N];  // the start block, for closure rows
+  const int lane = threadIdx.x & 63, ry = threadIdx.x >> 6;
+  const int x0 = blockIdx.x * TW, y = blockIdx.y * 4 + ry;
+  const int r0 = x0 - HM;  // LDS element k holds column r0 + k
+  const bool clos = x0 < B::M || x0 + TW > nx - B::M;  // the workgroup writes closure columns
+  if (clos)
+    for (int e = threadIdx.x; e < B::M * B::N; e += 256) tab[e] = B::start(e / B::N, e % B::N);
+  if (y < ny) {
+    const double* p = prev + (int64_t)y * ldp;
+    for (int k = lane; k < LW; k += 64) {
+      const int gx = r0 + k;
+      rows[ry][xi_pad(k)] = (gx >= 0 && gx < nx) ? p[gx] : 0.0;
+    }
+  }
+  __syncthreads();
+  // outputs go through LDS too, so that the stores are coalesced
+  const int xa = x0 + lane * kApplyR;
+  const double* l = rows[ry];
+  if (y < ny && xa < nx) {
+    double o[kApplyR];
+    if (xa >= B::M && xa + kApplyR <= nx - B::M) {
+      // window: elements 4·lane + C + k, k < kApplyR + D - 1
+      const double* w = l + 5 * lane;
+#pragma unroll
+      for (int r = 0; r < kApplyR; ++r) o[r] = 0.0;
+#pragma unroll
+      for (int k = 0; k < kApplyR + B::D - 1; ++k) {
+        const double v = w[xi_pad(C + k)];
+#pragma unroll
+        for (int r = 0; r < kApplyR; ++r) {
+          const int d = k - r;
+          if (d >= 0 && d < B::D && B::diag(d) != 0.0) o[r] += B::diag(d) * v;
+        }
+      }
+    } else {
+      // closure columns (start or end block, end rows read the start block
+      // flipped with the kind's sign), interior columns of the same thread,
+      // and columns past the end
+#pragma unroll
+      for (int r = 0; r < kApplyR; ++r) {
+        const int i = xa + r;
+        double sum = 0.0;
+        if (i < B::M) {
+#pragma unroll
+          for (int k = 0; k < B::N; ++k) sum += tab[i * B::N + k] * l[xi_pad(k - r0)];
+        } else if (i >= nx - B::M && i < nx) {
+          const int rr = B::M - 1 - (i - (nx - B::M));
+#pragma unroll
+          for (int k = 0; k < B::N; ++k)
+            sum += (B::end_sign * tab[rr * B::N + (B::N - 1 - k)]) * l[xi_pad(nx - B::N + k - r0)];
+        } else if (i < nx) {
+#pragma unroll
+          for (int d = 0; d < B::D; ++d)
+            if (B::diag(d) != 0.0) sum += B::diag(d) * l[xi_pad(i - B::H + d - r0)];
+        }
+        o[r] = sum;
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < kApplyR; ++r) outs[ry][xi_pad(lane * kApplyR + r)] = o[r] * idx;
+  }
+  __syncthreads();
+  if (y >= ny) return;
+  double* f = fut + (int64_t)y * ldf;
+#pragma unroll
+  for (int r = 0; r < kApplyR; ++r) {
+    const int x = x0 + r * 64 + lane;
+    if (x < nx) f[x] = outs[ry][xi_pad(r * 64 + lane)];
+  }
+}
+
+template <class OP, int KIND>
+__global__ __launch_bounds__(256) void apply_eta_kernel(const double* __restrict__ prev, int64_t ldp, double* __restrict__ fut,
+                                                        int64_t ldf, int ny, int nx, double idx) {
+  using B = Blk<OP, KIND>;
   const int x = blockIdx.x * 64 + (threadIdx.x & 63);
-  const int y = blockIdx.y * 4 + (threadIdx.x >> 6);
-  if (x >= nx || y >= ny) return;
-  double s;
-  if (AXIS == 0)
-    s = row_sum<B>(prev + (int64_t)y * ldp, 1, x, nx, 0);
-  else
-    s = row_sum<B>(prev + x, (int)ldp, y, ny, 0);
-  fut[(int64_t)y * ldf + x] = s * idx;
+  const int ya = (blockIdx.y * 4 + (threadIdx.x >> 6)) * kApplyR;
+  if (x >= nx || ya >= ny) return;
+  const double* p = prev + x;
+  double o[kApplyR];
+  if (ya >= B::M && ya + kApplyR <= ny - B::M) {
+    window_sums<B>(p + (int64_t)(ya - B::H) * ldp, (int)ldp, o);
+  } else {
+#pragma unroll
+    for (int r = 0; r < kApplyR; ++r) o[r] = ya + r < ny ? row_sum<B>(p, (int)ldp, ya + r, ny, 0) : 0.0;
+  }
+#pragma unroll
+  for (int r = 0; r < kApplyR; ++r)
+    if (ya + r < ny) fut[(int64_t)(ya + r) * ldf + x] = o[r] * idx;
+}
+
+template <class OP, int KIND>
+static void launch_apply_axis(int axis, const double* prev, int64_t ldp, double* fut, int64_t ldf, int ny, int nx, double idx,
+                              hipStream_t stream) {
+  if (axis == 0) {
+    dim3 grid((nx + 64 * kApplyR - 1) / (64 * kApplyR), (ny + 3) / 4);
+    apply_xi_kernel<OP, KIND><<<grid, 256, 0, stream>>>(prev, ldp, fut, ldf, ny, nx, idx);
+  } else {
+    dim3 grid((nx + 63) / 64, (ny + 4 * kApplyR - 1) / (4 * kApplyR));
+    apply_eta_kernel<OP, KIND><<<grid, 256, 0, stream>>>(prev, ldp, fut, ldf, ny, nx, idx);
+  }
 }
 
 template <class OP>
@@ -39,18 +164,11 @@ static hipError_t apply_op_t(int kind, int axis, const double* prev, int64_t ldp
   const int n = axis == 0 ? nx : ny;
   const double dx = OP::is_h2 ? 1.0 / (double)(n - 2) : 1.0 / (double)(n - 1);
   const double idx = 1.0 / dx;
-  dim3 grid((nx + 63) / 64, (ny + 3) / 4);
   if (kind == SBP_DIFF) {
-    if (axis == 0)
-      apply_op_kernel<OP, SBP_DIFF, 0><<<grid, 256, 0, stream>>>(prev, ldp, fut, ldf, ny, nx, idx);
-    else
-      apply_op_kernel<OP, SBP_DIFF, 1><<<grid, 256, 0, stream>>>(prev, ldp, fut, ldf, ny, nx, idx);
+    launch_apply_axis<OP, SBP_DIFF>(axis, prev, ldp, fut, ldf, ny, nx, idx, stream);
   } else {
     if constexpr (OP::is_upwind) {
-      if (axis == 0)
-        apply_op_kernel<OP, SBP_DISS, 0><<<grid, 256, 0, stream>>>(prev, ldp, fut, ldf, ny, nx, idx);
-      else
-        apply_op_kernel<OP, SBP_DISS, 1><<<grid, 256, 0, stream>>>(prev, ldp, fut, ldf, ny, nx, idx);
+      launch_apply_axis<OP, SBP_DISS>(axis, prev, ldp, fut, ldf, ny, nx, idx, stream);
     } else {
       return hipErrorInvalidValue;
     }

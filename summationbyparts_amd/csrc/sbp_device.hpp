@@ -133,6 +133,12 @@ struct StageArgs {
   // BlockDev::tile_off + ty·ntx + tx), written by mix_ref_kernel before the
   // first stage of a step (and before a right-hand-side launch)
   double* mixref;
+  // RK4 step without a separate time update: thread 0 of workgroup 0 of one
+  // dispatch writes *time_w = *time (stage S0: the step's start time into the
+  // slot stages S1-S3 read) or *time + dt (stage S1: the next step's time into
+  // the slot S0 reads); no dispatch of the same stage reads *time_w. Null:
+  // no write.
+  double* time_w;
 };
 
 }  // namespace sbp

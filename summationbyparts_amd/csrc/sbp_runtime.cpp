@@ -480,6 +480,11 @@ hipError_t launch_group(const sbp_system_s* s, const Group& g, int shape, int sc
   if (ee == hipSuccess) ee = launch_stage(g.op_eta, g.op_xi, g.upwind, true, shape, mode, ae, g.entiles, c->aux);
   StageArgs ai = a;
   ai.tiles = s->d_tiles + g.tile_offset;
+  // one of the two dispatches carries the time update of StageArgs::time_w
+  if (g.entiles > 0)
+    ai.time_w = nullptr;
+  else
+    ae.time_w = nullptr;
   const hipError_t ei = ee == hipSuccess ? launch_stage(g.op_eta, g.op_xi, g.upwind, false, shape, mode, ai, g.ntiles, st) : ee;
   // join even after an error, so that a capture in progress stays balanced
   if ((e = hipEventRecord(c->ev_join, c->aux)) != hipSuccess) return e;
@@ -818,13 +823,23 @@ int check_err(sbp_system_s* s) {
   return status_from_errword(w);
 }
 
-// One fused RK4 step: four stage launches per group and the time update (integrate::<Rk4>, utils/integrate/src/lib.rs:176-222).
+// One fused RK4 step: four stage launches per group, the time update
+// (integrate::<Rk4>, utils/integrate/src/lib.rs:176-222) carried by the first
+// two: stage S0 copies the step's start time d_time[0] into d_time[2], which
+// stages S1-S3 read, and S1 writes d_time[2] + dt into d_time[0] (no
+// dispatch of a stage reads the slot that stage writes).
 int enqueue_step(sbp_system_s* s, double dt, int swap, hipStream_t st) {
+  double* const t_step = s->d_time;
+  double* const t_stage = s->d_time + 2;
   for (int stage = 0; stage < 4; ++stage) {
-    const StageArgs a = stage_args(s, dt, stage, swap);
-    for (const Group& g : s->groups) SBP_TRY_HIP(launch_group(s, g, g.shape, g.sched, stage, a, st));
+    StageArgs a = stage_args(s, dt, stage, swap);
+    if (stage > 0) a.time = t_stage;
+    for (size_t gi = 0; gi < s->groups.size(); ++gi) {
+      StageArgs ag = a;
+      if (gi == 0 && stage < 2) ag.time_w = stage == 0 ? t_stage : t_step;
+      SBP_TRY_HIP(launch_group(s, s->groups[gi], s->groups[gi].shape, s->groups[gi].sched, stage, ag, st));
+    }
   }
-  SBP_TRY_HIP(launch_advance_time(s->d_time, dt, st));
   return SBP_OK;
 }
 
@@ -1441,9 +1456,11 @@ int sbp_system_create(sbp_context ctx, int nblocks, const int* ny, const int* nx
     if (rc == SBP_OK) rc = from_hip(launch_metric_pairs(h.d_metrics, h.ps, h.ld, h.ny, h.nx, h.d_mpair, st));
   }
   if (rc != SBP_OK) return fail(rc);
-  if ((rc = from_hip(hipMalloc(&s->d_time, sizeof(double) * 2))) != SBP_OK) return fail(rc);
+  // [0] the system time, [1] the explicit time of sbp_system_rhs, [2] the
+  // stage time base inside an RK4 step (enqueue_step)
+  if ((rc = from_hip(hipMalloc(&s->d_time, sizeof(double) * 3))) != SBP_OK) return fail(rc);
   s->d_time_rhs = s->d_time + 1;
-  if ((rc = from_hip(hipMemsetAsync(s->d_time, 0, sizeof(double) * 2, st))) != SBP_OK) return fail(rc);
+  if ((rc = from_hip(hipMemsetAsync(s->d_time, 0, sizeof(double) * 3, st))) != SBP_OK) return fail(rc);
   if ((rc = from_hip(hipMalloc(&s->d_err, sizeof(unsigned)))) != SBP_OK) return fail(rc);
   if ((rc = from_hip(hipMemsetAsync(s->d_err, 0, sizeof(unsigned), st))) != SBP_OK) return fail(rc);
   unsigned w = 0;

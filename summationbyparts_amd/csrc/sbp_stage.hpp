@@ -873,6 +873,12 @@ __global__ __launch_bounds__((TX / RX) * (TY / RY)) __attribute__((amdgpu_waves_
     }
   }
   if (err) record_error(A.error, err);
+  // the time update of an RK4 step (StageArgs::time_w), after the tile's
+  // own work so that the store orders none of it
+  if (A.time_w && blockIdx.x == 0 && tid == 0) {
+    const double t = *A.time;
+    *A.time_w = mode == MODE_S0 ? t : t + A.dt;
+  }
 }
 
 }  // namespace sbp

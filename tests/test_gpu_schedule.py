@@ -144,3 +144,38 @@ def test_schedule_c2_block_and_autotune(gpu):
     for k in ks:
         assert rel_global(k, k_ref) < 5e-12
     assert rel_global(ks[0], ks[1]) < 1e-15 and rel_global(ks[0], ks[2]) < 1e-15
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+@pytest.mark.parametrize("schedule", [0, 1, 2])
+def test_time_update_in_stage_dispatches(gpu, schedule, precision):
+    """The RK4 step's time update is a store by one thread of the first two
+    stages' dispatches (stage S0 copies the start time into a second slot,
+    which stages S1-S3 read; S1 writes that slot + dt into the system time).
+    With two operator groups, after single steps and after multi-step calls
+    (graphs of several steps), the device time equals the host's running sum
+    t += dt (integrate::<Rk4>, utils/integrate/src/lib.rs:176-222) bit for
+    bit, and the far-field data follow it: the state equals that of a system
+    stepped once per call. Both systems run one tile shape: the mixed form's
+    per-tile reference terms make its rounding depend on the tile, which the
+    autotuner picks by timing."""
+    sys_, _ = _systems(schedule, precision)
+    sys_.set_tile_shape(0)
+    dt = 0.2 * 10.0 / 60.0 / 4.0
+    sys_.set_dt(dt)
+    sys_.time = 0.3
+    t = 0.3
+    for n in (1, 3, 16, 5):
+        sys_.advance(n)
+        for _ in range(n):
+            t += dt
+        assert sys_.time == t, (n, sys_.time, t)
+    one, _ = _systems(schedule, precision)
+    one.set_tile_shape(0)
+    one.set_dt(dt)
+    one.time = 0.3
+    for _ in range(25):
+        one.advance(1)
+    assert one.time == t
+    for i in range(2):
+        assert np.array_equal(one.get_state(i), sys_.get_state(i)), i

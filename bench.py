@@ -273,7 +273,7 @@ def time_steps(w, steps, warmup, tile=-1, schedule=-1):
     wall = time.perf_counter() - t0
     # mean duration of one stage launch (edge SATs and edge tiles on the
     # second stream, concurrent with the interior tiles) over the timed
-    # region: 4 launches per step plus a 1-thread time update
+    # region: 4 launches per step (the first two also carry the time update)
     return {"value": w.points * steps / wall / 1e9, "wall": wall, "ms_per_step": wall * 1e3 / steps,
             "event_ms_per_step": ms_timed / steps, "stage_launch_ms": ms_timed / (4 * steps),
             "tile": list(s.tile_shape()), "tile_index": s.tile_index(),
